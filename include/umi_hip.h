@@ -1,0 +1,121 @@
+/*
+ * umi_hip.h — C ABI of libumi_hip, the MI355X (gfx950) UMI operator library.
+ *
+ * Plain C: pointers, sizes and status codes only.  Device pointers are HIP
+ * device allocations (umi_malloc or any hipMalloc'd memory); every compute
+ * entry point is asynchronous on the stream it is given (NULL = the legacy
+ * default stream) and returns UMI_OK or an error code, with a readable
+ * message in umi_last_error().  Nothing here aborts.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (johnjohnlin/UMI, file:line).
+ */
+#ifndef UMI_HIP_H
+#define UMI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int umi_status;
+enum {
+    UMI_OK = 0,
+    UMI_ERR_INVALID = 1,     /* bad argument / descriptor                    */
+    UMI_ERR_HIP = 2,         /* HIP runtime error (message has the details)  */
+    UMI_ERR_UNSUPPORTED = 3, /* shape or parameter outside the built kernels */
+    UMI_ERR_NO_DEVICE = 4
+};
+
+typedef void *umi_stream; /* a hipStream_t; NULL = default stream */
+
+/* copy directions for umi_memcpy (values equal hipMemcpyKind) */
+enum { UMI_H2H = 0, UMI_H2D = 1, UMI_D2H = 2, UMI_D2D = 3, UMI_DEFAULT = 4 };
+
+/* ---------------------------------------------------------------------------
+ * Runtime.  Replaces the cudaMalloc / cudaMemcpy / cudaFree calls of
+ * src/SyncedMemory.h:19,22,65,74 and the PackUtility upload of
+ * src/UMI_gpu.h:374-381.
+ * ------------------------------------------------------------------------- */
+const char *umi_last_error(void);
+umi_status umi_device_count(int *n);
+umi_status umi_set_device(int device);
+umi_status umi_device_name(char *buf, size_t len); /* gcnArchName + marketing name */
+umi_status umi_malloc(void **ptr, size_t bytes);
+umi_status umi_free(void *ptr);
+umi_status umi_host_alloc(void **ptr, size_t bytes); /* pinned host memory */
+umi_status umi_host_free(void *ptr);
+umi_status umi_memcpy(void *dst, const void *src, size_t bytes, int kind);
+umi_status umi_memcpy_async(void *dst, const void *src, size_t bytes, int kind, umi_stream s);
+umi_status umi_memset(void *ptr, int value, size_t bytes);
+umi_status umi_synchronize(void);
+umi_status umi_stream_create(umi_stream *s);
+umi_status umi_stream_destroy(umi_stream s);
+umi_status umi_stream_synchronize(umi_stream s);
+/* Launch a kernel compiled in the caller's translation unit (the generic
+ * Execute<> engine instantiates its kernel from the user's strategy struct;
+ * src/UMI_gpu.h:610 is the launch it replaces). */
+umi_status umi_launch_kernel(const void *func, unsigned gx, unsigned gy, unsigned gz,
+                             unsigned bx, unsigned by, unsigned bz, size_t shmem,
+                             void **args, umi_stream s);
+
+/* Event timer on a stream (hipEvent pair).  No reference counterpart: the
+ * reference times nothing (src/Timer.h is never included). */
+typedef struct umi_timer_s *umi_timer;
+umi_status umi_timer_create(umi_timer *t);
+umi_status umi_timer_destroy(umi_timer t);
+umi_status umi_timer_start(umi_timer t, umi_stream s);
+umi_status umi_timer_stop(umi_timer t, umi_stream s);
+umi_status umi_timer_elapsed_ms(umi_timer t, float *ms); /* waits for the stop event */
+
+/* Graph capture of a stream (hipGraph).  Used to replay a fixed sequence of
+ * launches; replaces nothing in the reference (it launches on the legacy
+ * default stream, src/UMI_gpu.h:610). */
+typedef struct umi_graph_s *umi_graph;
+umi_status umi_graph_begin(umi_stream s);
+umi_status umi_graph_end(umi_stream s, umi_graph *g);
+umi_status umi_graph_launch(umi_graph g, umi_stream s);
+umi_status umi_graph_destroy(umi_graph g);
+
+/* ---------------------------------------------------------------------------
+ * Full-search block motion estimation, uint8 frames.
+ * Replaces the MotionEst strategy launch of main/full_me.cu:34-64
+ * (SAD volume over a (2R+1)^2 window, clamp-to-edge boundary, the
+ * reference's WRAP mode = clamp, src/UMI.h:205-218) and adds the argmin.
+ *
+ *   SAD(by,bx,dy,dx) = sum_{py,px<B} | cur[y+py][x+px]
+ *                        - ref[clamp(y+py+dy,0,H-1)][clamp(x+px+dx,0,W-1)] |
+ *   with y = by*B, x = bx*B, dy,dx in [-R, R]; nby = H/B, nbx = W/B (floor,
+ *   like main/full_me.cu:25).
+ *
+ * mv_sad  (nullable): int32 [nby][nbx][3] = (dy, dx, SAD) of the FIRST
+ *          minimum in raster order over (dy, dx) starting at (-R, -R).
+ * volume  (nullable): int32 [nby][nbx][2R+1][2R+1], the full SAD volume
+ *          (the output layout of main/full_me.cu:56-62).
+ * B in {8, 16}; 1 <= R <= 64; pitch >= W (bytes).
+ * ------------------------------------------------------------------------- */
+umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                 int B, int R, int32_t *mv_sad, int32_t *volume, umi_stream s);
+
+/* ---------------------------------------------------------------------------
+ * SAD cost volume, uint8 pair.  Replaces the MotionEst launch of
+ * main/cost_volume.cu:35-65, generalised to two images and a 2-D window:
+ *
+ *   cost[y][x][i] = sum_{|dy|<=KY, |dx|<=KX}
+ *       | L[cy][clamp(x+dx)] - R[cy][clamp(x+dx+s0+step*i)] |,
+ *   cy = clamp(y+dy), i in [0, D), clamps to [0,H-1] / [0,W-1]
+ *   (clamp applied to each final coordinate, as main/cost_volume.cu:73-74).
+ *
+ * The reference example is L = R, KY = 0, KX = 3, s0 = -7, step = +1, D = 15;
+ * stereo matching with disparity d = i uses s0 = 0, step = -1.
+ * out: int32 [H][W][D].  0 <= KX, KY <= 3; step = +1 or -1; D >= 1.
+ * ------------------------------------------------------------------------- */
+umi_status umi_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D,
+                              int s0, int step, int KY, int KX, int32_t *out, umi_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UMI_HIP_H */

@@ -1,0 +1,116 @@
+/*
+ * umi_oracle.c — CPU oracle for the UMI operator workloads.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the CPU check
+ * loops that johnjohnlin/UMI uses to validate each example (main/*.cu), with
+ * the generalisations the MI355X library implements (two frames, uint8
+ * pixels, argmin, 2-D windows).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load it, and only as the checker.  The
+ * library never calls it.
+ *
+ * Every function cites the reference loop it follows.  How each is pinned
+ * (closed forms, brute force, invariants) is listed in the tests that load it.
+ * Build: oracle/Makefile (gcc, -O2, -ffp-contract=off: float expressions are
+ * evaluated exactly as written, fused multiply-adds only where fmaf() is
+ * spelled out).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/*
+ * Full-search block motion estimation.
+ * Follows main/full_me.cu:66-87: for each block (raster), for each search
+ * offset (raster, dy outer), SAD over the block with the searched pixel
+ * clamped to the frame (max(min(.,h-1),0), full_me.cu:77-78).  Generalised:
+ * cur/ref are two uint8 frames, offsets run -R..R, and the first minimum in
+ * raster order (strict <) is reported as (dy, dx, sad).
+ * Blocks are processed for block rows [by_begin, by_end) only (a bounded
+ * sample for timing); pass 0, H/B for the whole frame.
+ */
+void oracle_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int B, int R,
+                              int by_begin, int by_end, int32_t *mv_sad, int32_t *volume) {
+    const int nbx = W / B, S = 2 * R + 1;
+    for (int by = by_begin; by < by_end; ++by) {
+        for (int bx = 0; bx < nbx; ++bx) {
+            const int y = by * B, x = bx * B;
+            const size_t blk = (size_t)by * nbx + bx;
+            int32_t best = INT32_MAX, bdy = 0, bdx = 0;
+            for (int sy = 0; sy < S; ++sy) {
+                for (int sx = 0; sx < S; ++sx) {
+                    int32_t sum = 0;
+                    for (int py = 0; py < B; ++py) {
+                        const int spy = clampi(y + py + sy - R, 0, H - 1);
+                        const uint8_t *c = cur + (size_t)(y + py) * pitch + x;
+                        const uint8_t *r = ref + (size_t)spy * pitch;
+                        for (int px = 0; px < B; ++px) {
+                            const int spx = clampi(x + px + sx - R, 0, W - 1);
+                            sum += abs((int)c[px] - (int)r[spx]);
+                        }
+                    }
+                    if (volume) volume[(blk * S + sy) * S + sx] = sum;
+                    if (sum < best) {
+                        best = sum;
+                        bdy = sy - R;
+                        bdx = sx - R;
+                    }
+                }
+            }
+            if (mv_sad) {
+                mv_sad[blk * 3 + 0] = bdy;
+                mv_sad[blk * 3 + 1] = bdx;
+                mv_sad[blk * 3 + 2] = best;
+            }
+        }
+    }
+}
+
+/*
+ * Full-search SAD volume on int32 frames: the exact check loop of
+ * main/full_me.cu:66-87 (one frame searched against itself there; here cur
+ * and ref may differ), search offsets ofs..ofs+S-1 in each axis.
+ */
+void oracle_sad_volume_i32(const int32_t *cur, const int32_t *ref, int H, int W, int B, int S, int ofs,
+                           int32_t *volume) {
+    size_t c = 0;
+    for (int y = 0; y + B <= H; y += B)
+        for (int x = 0; x + B <= W; x += B)
+            for (int sy = 0; sy < S; ++sy)
+                for (int sx = 0; sx < S; ++sx) {
+                    int32_t sum = 0;
+                    for (int py = 0; py < B; ++py)
+                        for (int px = 0; px < B; ++px) {
+                            const int spy = clampi(y + py + sy + ofs, 0, H - 1);
+                            const int spx = clampi(x + px + sx + ofs, 0, W - 1);
+                            sum += abs(cur[(size_t)(y + py) * W + (x + px)] - ref[(size_t)spy * W + spx]);
+                        }
+                    volume[c++] = sum;
+                }
+}
+
+/*
+ * SAD cost volume.  Follows main/cost_volume.cu:67-80 (clamp applied to each
+ * final coordinate, cost_volume.cu:73-74), generalised to a left/right pair
+ * and a (2KY+1)x(2KX+1) window: shift s = s0 + step*i, rows [y_begin, y_end).
+ */
+void oracle_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step,
+                           int KY, int KX, int y_begin, int y_end, int32_t *out) {
+    for (int y = y_begin; y < y_end; ++y)
+        for (int x = 0; x < W; ++x)
+            for (int i = 0; i < D; ++i) {
+                const int s = s0 + step * i;
+                int32_t sad = 0;
+                for (int dy = -KY; dy <= KY; ++dy) {
+                    const int cy = clampi(y + dy, 0, H - 1);
+                    for (int dx = -KX; dx <= KX; ++dx) {
+                        const int lx = clampi(x + dx, 0, W - 1);
+                        const int rx = clampi(x + dx + s, 0, W - 1);
+                        sad += abs((int)L[(size_t)cy * pitch + lx] - (int)R[(size_t)cy * pitch + rx]);
+                    }
+                }
+                out[((size_t)y * W + x) * D + i] = sad;
+            }
+}

@@ -1,0 +1,98 @@
+"""GPU parity of full-search motion estimation against the oracle (bit-exact)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from umi_amd import ops, workloads
+from umi_amd._lib import call
+
+GOLD = Path(__file__).resolve().parent / "golden"
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[1, 0], ids=["qsad", "sad"])
+def variant(request, gpu):
+    call("umi_me_set_variant", request.param)
+    yield request.param
+    call("umi_me_set_variant", 1)
+
+
+@pytest.mark.parametrize("name", ["me_ramp", "me_shift", "me_random8", "me_random16"])
+def test_me_golden(variant, name):
+    f = np.load(GOLD / f"{name}.npz")
+    print(f"[me] golden {name} variant={variant}", flush=True)
+    mv, vol = ops.me_full_search_np(f["cur"], f["ref"], int(f["B"]), int(f["R"]), volume=True)
+    np.testing.assert_array_equal(vol, f["vol"])
+    np.testing.assert_array_equal(mv, f["mv"])
+
+
+@pytest.mark.parametrize("H,W,B,R", [(64, 96, 8, 16), (72, 80, 16, 7), (100, 130, 8, 32), (37, 53, 8, 1),
+                                     (96, 160, 16, 16), (80, 80, 8, 64)])
+def test_me_random_vs_oracle(variant, H, W, B, R):
+    g = workloads.rng(H * 1000 + W + R)
+    cur = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    ref = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    print(f"[me] random H={H} W={W} B={B} R={R} variant={variant}", flush=True)
+    mv, vol = ops.me_full_search_np(cur, ref, B, R, volume=True)
+    omv, ovol = oracle.me_full_search(cur, ref, B, R, volume=True)
+    np.testing.assert_array_equal(vol, ovol)
+    np.testing.assert_array_equal(mv, omv)
+
+
+def test_me_ties_first_in_raster_order(gpu):
+    f = np.full((64, 64), 9, np.uint8)
+    mv, _ = ops.me_full_search_np(f, f, 8, 5)
+    assert (mv[..., 0] == -5).all() and (mv[..., 1] == -5).all() and (mv[..., 2] == 0).all()
+
+
+def test_me_full_hd_r16_vs_oracle(gpu):
+    # 1920x1080 pair, 8x8 blocks, +-16: every block bit-exact against the oracle
+    cur, ref, true_mv = workloads.me_pair(1080, 1920, 16, seed=1)
+    print("[me] 1080p R=16 full frame", flush=True)
+    mv, _ = ops.me_full_search_np(cur, ref, 8, 16)
+    omv, _ = oracle.me_full_search(cur, ref, 8, 16)
+    np.testing.assert_array_equal(mv, omv)
+    # most interior blocks recover the planted motion
+    t = true_mv[np.arange(135)[:, None] // 2, np.arange(240)[None, :] // 2]
+    hit = (mv[..., :2] == t).all(axis=2)[2:-2, 2:-2].mean()
+    assert hit > 0.9, hit
+
+
+def test_me_full_hd_r32_rows_vs_oracle(gpu):
+    # the +-32 judge search: block rows at both frame edges and in the middle
+    cur, ref, _ = workloads.me_pair(1080, 1920, 32, seed=5)
+    print("[me] 1080p R=32", flush=True)
+    mv, vol = ops.me_full_search_np(cur, ref, 8, 32)
+    for rows in [(0, 6), (64, 68), (129, 135)]:
+        omv, _ = oracle.me_full_search(cur, ref, 8, 32, by_range=rows)
+        np.testing.assert_array_equal(mv[rows[0]:rows[1]], omv[rows[0]:rows[1]])
+
+
+def test_me_volume_argmin_consistent(gpu):
+    # size-independent property: the MV output is the first minimum of the volume output
+    cur, ref, _ = workloads.me_pair(264, 480, 16, seed=3)
+    mv, vol = ops.me_full_search_np(cur, ref, 8, 16, volume=True)
+    S = 33
+    flat = vol.reshape(vol.shape[0], vol.shape[1], S * S)
+    idx = flat.argmin(axis=2)
+    np.testing.assert_array_equal(mv[..., 0], idx // S - 16)
+    np.testing.assert_array_equal(mv[..., 1], idx % S - 16)
+    np.testing.assert_array_equal(mv[..., 2], flat.min(axis=2))
+
+
+def test_me_pitch_and_offsets(gpu):
+    # frames inside a wider allocation (pitch > W)
+    from umi_amd import DeviceBuffer
+
+    H, W, P = 48, 56, 80
+    g = workloads.rng(11)
+    big_c = g.integers(0, 256, size=(H, P), dtype=np.int64).astype(np.uint8)
+    big_r = g.integers(0, 256, size=(H, P), dtype=np.int64).astype(np.uint8)
+    dc, dr = DeviceBuffer.from_array(big_c), DeviceBuffer.from_array(big_r)
+    mv = DeviceBuffer(6 * 7 * 3 * 4)
+    ops.me_full_search(dc, dr, H, W, P, 8, 4, mv)
+    got = mv.download((6, 7, 3), np.int32)
+    omv, _ = oracle.me_full_search(np.ascontiguousarray(big_c[:, :W]), np.ascontiguousarray(big_r[:, :W]), 8, 4)
+    np.testing.assert_array_equal(got, omv)

@@ -1,0 +1,81 @@
+"""ctypes binding of libumi_hip.so (the C ABI declared in include/umi_hip.h).
+
+The library must have been built (``python -m umi_amd.build`` or
+``__graft_entry__.build()``).  There is no fallback: if it is missing or a
+call fails, an exception says so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libumi_hip.so"
+
+c_int, c_size_t, c_void_p, c_float = C.c_int, C.c_size_t, C.c_void_p, C.c_float
+P = C.c_void_p  # device pointers travel as integers
+
+# name -> argument types (return type is always umi_status = int unless noted)
+SIGNATURES = {
+    "umi_device_count": [C.POINTER(c_int)],
+    "umi_set_device": [c_int],
+    "umi_device_name": [C.c_char_p, c_size_t],
+    "umi_malloc": [C.POINTER(c_void_p), c_size_t],
+    "umi_free": [P],
+    "umi_host_alloc": [C.POINTER(c_void_p), c_size_t],
+    "umi_host_free": [P],
+    "umi_memcpy": [P, P, c_size_t, c_int],
+    "umi_memcpy_async": [P, P, c_size_t, c_int, P],
+    "umi_memset": [P, c_int, c_size_t],
+    "umi_synchronize": [],
+    "umi_stream_create": [C.POINTER(c_void_p)],
+    "umi_stream_destroy": [P],
+    "umi_stream_synchronize": [P],
+    "umi_launch_kernel": [P, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, c_size_t,
+                          C.POINTER(c_void_p), P],
+    "umi_timer_create": [C.POINTER(c_void_p)],
+    "umi_timer_destroy": [P],
+    "umi_timer_start": [P, P],
+    "umi_timer_stop": [P, P],
+    "umi_timer_elapsed_ms": [P, C.POINTER(c_float)],
+    "umi_graph_begin": [P],
+    "umi_graph_end": [P, C.POINTER(c_void_p)],
+    "umi_graph_launch": [P, P],
+    "umi_graph_destroy": [P],
+    "umi_me_set_variant": [c_int],
+    "umi_me_full_search_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, P],
+    "umi_cost_volume_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
+}
+
+
+class UmiError(RuntimeError):
+    """A libumi_hip call returned a non-zero status."""
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libumi_hip.so once and attach the argument types."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise UmiError(f"{LIB_PATH} is not built; run `python -m umi_amd.build` first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, argtypes in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = argtypes
+            f.restype = c_int
+        L.umi_last_error.argtypes = []
+        L.umi_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().umi_last_error().decode(errors="replace")
+        raise UmiError(f"{what or 'libumi_hip'} failed (status {status}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

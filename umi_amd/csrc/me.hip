@@ -1,0 +1,255 @@
+// Full-search block motion estimation on uint8 frames (gfx950).
+//
+// Computes, for every BxB block of `cur`, the SAD against `ref` at every
+// integer offset (dy, dx) in [-R, R]^2 with clamp-to-edge reads of `ref`, and
+// the first minimum in raster order over (dy, dx).  This is the MotionEst
+// strategy of main/full_me.cu:8-15,34-64 (johnjohnlin/UMI) with its clamp
+// boundary (src/UMI.h:205-218, "WRAP" clamps) and output layout
+// (main/full_me.cu:56-62), generalised to two frames and uint8 pixels.
+//
+// Design (one workgroup = a TBX x TBY tile of blocks):
+//  * the tile's reference window ((TBY*B+2R) rows x (TBX*B+2R+4) bytes,
+//    clamped) and the tile's current pixels are staged once in LDS;
+//  * a thread owns one block and one group of 4 horizontal offsets
+//    dx = 4g..4g+3, and walks the 2R+B window rows once.  Each window row is
+//    read from LDS a single time and feeds the B vertical offsets that use it
+//    (a ring of B accumulators indexed (row - py) mod B, static after
+//    unrolling), so LDS traffic is (B/4+1) dwords per row per thread;
+//  * v_qsad_pk_u16_u8 computes the SAD of 4 cur bytes against 4 byte-shifted
+//    windows of 8 ref bytes in one instruction (16 |a-b| per lane-op), with
+//    16-bit accumulators (B <= 16 keeps 255*B*B < 65536);
+//  * the argmin key is (SAD << idx_bits) | raster_index, so a plain unsigned
+//    min implements "smallest SAD, then first in raster order"; per-thread
+//    minima are merged with an LDS atomic min per block.
+#include "common.h"
+
+namespace {
+using namespace umi;
+
+// Largest workgroup per block size: B = 16 holds 64 cur dwords and 16 packed
+// accumulators per thread, which needs the 256-VGPR budget of <= 512 threads.
+template <int B>
+constexpr int me_max_threads() { return B == 8 ? 1024 : 512; }
+
+template <int B, bool QSAD, bool VOLUME>
+__global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
+    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R,
+    int nby, int nbx, int tbx, int tby, int G, int ws4, int wrows, int idx_bits,
+    int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume) {
+    extern __shared__ uint32_t lds[];
+    constexpr int B4 = B / 4;
+    const int nblk = tbx * tby;
+    const int cs4 = tbx * B4;             // cur tile row stride (dwords)
+    uint32_t *win = lds;                  // wrows x ws4 dwords
+    uint32_t *ctile = win + wrows * ws4;  // (tby*B) x cs4 dwords
+    uint32_t *best = ctile + tby * B * cs4;
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int bx0 = blockIdx.x * tbx, by0 = blockIdx.y * tby;
+    const int x0 = bx0 * B, y0 = by0 * B;
+
+    // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(x0-R+c)].
+    for (int i = tid; i < wrows * ws4; i += nthr) {
+        const int r = i / ws4, c4 = i - r * ws4;
+        const uint8_t *row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
+        const int gx = x0 - R + 4 * c4;
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
+        win[i] = v;
+    }
+    // Current tile.  Blocks past the frame edge read clamped pixels and are
+    // never written out.
+    for (int i = tid; i < tby * B * cs4; i += nthr) {
+        const int r = i / cs4, c4 = i - r * cs4;
+        const uint8_t *row = cur + (size_t)min(y0 + r, H - 1) * pitch;
+        const int gx = x0 + 4 * c4;
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v |= (uint32_t)row[min(gx + b, W - 1)] << (8 * b);
+        ctile[i] = v;
+    }
+    for (int i = tid; i < nblk; i += nthr) best[i] = 0xffffffffu;
+    __syncthreads();
+
+    const int bl = tid / G, g = tid - bl * G;
+    if (bl < nblk) {
+        const int bxl = bl % tbx, byl = bl / tbx;
+        const int bx = bx0 + bxl, by = by0 + byl;
+        const bool valid = bx < nbx && by < nby;
+        const size_t blk = (size_t)by * nbx + bx;
+        const int S = 2 * R + 1;
+
+        uint32_t c[B][B4];
+#pragma unroll
+        for (int py = 0; py < B; ++py)
+#pragma unroll
+            for (int k = 0; k < B4; ++k) c[py][k] = ctile[(byl * B + py) * cs4 + bxl * B4 + k];
+
+        const uint32_t *wbase = win + (byl * B) * ws4 + bxl * B4 + g;
+        // QSAD: one packed u64 (4 x u16) per ring slot; otherwise 4 x u32.
+        uint64_t acc[B];
+        uint32_t acc32[B][4];
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+            acc[s] = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc32[s][q] = 0;
+        }
+        uint32_t bestk = 0xffffffffu;
+        const int nrows = 2 * R + B;
+        for (int r0 = 0; r0 < nrows; r0 += B) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int r = r0 + j;
+                if (r < nrows) {
+                    const uint32_t *wr = wbase + r * ws4;
+                    uint32_t w[B4 + 1];
+#pragma unroll
+                    for (int k = 0; k <= B4; ++k) w[k] = wr[k];
+                    if constexpr (QSAD) {
+#pragma unroll
+                        for (int py = 0; py < B; ++py) {
+                            const int slot = (j - py + B) % B;
+#pragma unroll
+                            for (int k = 0; k < B4; ++k) {
+                                const uint64_t s0 = (uint64_t)w[k] | ((uint64_t)w[k + 1] << 32);
+                                acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(s0, c[py][k], acc[slot]);
+                            }
+                        }
+                    } else {
+                        uint32_t u[4][B4];
+#pragma unroll
+                        for (int k = 0; k < B4; ++k) {
+                            u[0][k] = w[k];
+#pragma unroll
+                            for (int q = 1; q < 4; ++q) u[q][k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], q);
+                        }
+#pragma unroll
+                        for (int py = 0; py < B; ++py) {
+                            const int slot = (j - py + B) % B;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                                for (int k = 0; k < B4; ++k)
+                                    acc32[slot][q] = __builtin_amdgcn_sad_u8(u[q][k], c[py][k], acc32[slot][q]);
+                        }
+                    }
+                    // Row r completes vertical offset index dyi = r - (B-1).
+                    const int fs = (j + 1) % B;
+                    const int dyi = r - (B - 1);
+                    if (dyi >= 0) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int dxi = 4 * g + q;
+                            if (dxi < S) {
+                                const uint32_t sad =
+                                    QSAD ? (uint32_t)(acc[fs] >> (16 * q)) & 0xffffu : acc32[fs][q];
+                                const uint32_t key = (sad << idx_bits) | (uint32_t)(dyi * S + dxi);
+                                bestk = min(bestk, key);
+                                if (VOLUME && valid) volume[(blk * S + dyi) * S + dxi] = (int32_t)sad;
+                            }
+                        }
+                    }
+                    acc[fs] = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc32[fs][q] = 0;
+                }
+            }
+        }
+        atomicMin(&best[bl], bestk);
+    }
+    __syncthreads();
+    if (mv_sad) {
+        const int S = 2 * R + 1;
+        for (int i = tid; i < nblk; i += nthr) {
+            const int bx = bx0 + i % tbx, by = by0 + i / tbx;
+            if (bx < nbx && by < nby) {
+                const uint32_t key = best[i];
+                const uint32_t idx = key & ((1u << idx_bits) - 1u);
+                int32_t *o = mv_sad + ((size_t)by * nbx + bx) * 3;
+                o[0] = (int32_t)(idx / S) - R;
+                o[1] = (int32_t)(idx % S) - R;
+                o[2] = (int32_t)(key >> idx_bits);
+            }
+        }
+    }
+}
+
+struct MePlan {
+    int G, tbx, tby, threads, ws4, wrows, idx_bits;
+    size_t lds;
+};
+
+MePlan me_plan(int B, int R) {
+    MePlan p;
+    p.G = umi::ceil_div(2 * R + 1, 4);
+    const int max_threads = B == 8 ? me_max_threads<8>() : me_max_threads<16>();
+    int nblk = 1;
+    while (nblk * 2 * p.G <= max_threads && nblk * 2 <= 256) nblk *= 2;
+    p.tbx = nblk < 16 ? nblk : 16;
+    p.tby = nblk / p.tbx;
+    p.threads = umi::round_up(nblk * p.G, 64);
+    p.ws4 = umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
+    p.wrows = p.tby * B + 2 * R;
+    const int S = 2 * R + 1;
+    p.idx_bits = umi::bits_for((uint32_t)(S * S - 1));
+    p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
+    return p;
+}
+
+int g_me_variant = 1;  // 1 = v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32
+
+template <int B, bool QSAD>
+umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R,
+                     int32_t *mv_sad, int32_t *volume, hipStream_t st) {
+    const MePlan p = me_plan(B, R);
+    const int nby = H / B, nbx = W / B;
+    dim3 grid(umi::ceil_div(nbx, p.tbx), umi::ceil_div(nby, p.tby));
+    if (volume)
+        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, true>), grid, dim3(p.threads), p.lds, st, cur, ref,
+                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.ws4, p.wrows, p.idx_bits, mv_sad,
+                           volume);
+    else
+        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false>), grid, dim3(p.threads), p.lds, st, cur, ref,
+                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.ws4, p.wrows, p.idx_bits, mv_sad,
+                           volume);
+    return umi::launch_check("me_full_search_kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+// Selects the SAD instruction form (1 = v_qsad_pk_u16_u8, 0 = v_sad_u8).
+// Both give identical results; exported for measurement.
+umi_status umi_me_set_variant(int variant) {
+    if (variant != 0 && variant != 1) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
+    g_me_variant = variant;
+    return UMI_OK;
+}
+
+umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int B, int R,
+                                 int32_t *mv_sad, int32_t *volume, umi_stream s) {
+    using umi::set_error;
+    if (!cur || !ref) return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: null frame");
+    if (B != 8 && B != 16) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: block %d (8 or 16)", B);
+    if (R < 1 || R > 64) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: radius %d (1..64)", R);
+    if (H < 1 || W < 1 || pitch < W)
+        return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: H=%d W=%d pitch=%d", H, W, pitch);
+    if (H / B == 0 || W / B == 0) return UMI_OK;  // no whole block
+    if (!mv_sad && !volume) return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: no output");
+    const MePlan p = me_plan(B, R);
+    if (p.lds > 64 * 1024) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: LDS %zu B", p.lds);
+    const int sad_bits = umi::bits_for(255u * B * B);
+    if (sad_bits + p.idx_bits > 32) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: key overflow");
+    hipStream_t st = umi::as_stream(s);
+    if (B == 8)
+        return g_me_variant ? me_launch<8, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
+                            : me_launch<8, false>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
+    return g_me_variant ? me_launch<16, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
+                        : me_launch<16, false>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+"""Seeded synthetic workloads for the UMI operator configurations.
+
+All generators are deterministic functions of ``seed`` (numpy PCG64).  They
+produce the inputs only; results come from the library (and, in tests, from
+the oracle).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def box5_u8(img: np.ndarray) -> np.ndarray:
+    """5x5 box filter with edge replication, integer rounding (sum + 12) // 25."""
+    p = np.pad(img.astype(np.int32), 2, mode="edge")
+    H, W = img.shape
+    acc = np.zeros((H, W), np.int32)
+    for dy in range(5):
+        for dx in range(5):
+            acc += p[dy:dy + H, dx:dx + W]
+    return ((acc + 12) // 25).astype(np.uint8)
+
+
+def smooth_noise_u8(H: int, W: int, g: np.random.Generator) -> np.ndarray:
+    return box5_u8(g.integers(0, 256, size=(H, W), dtype=np.int32).astype(np.uint8))
+
+
+def me_pair(H: int = 1080, W: int = 1920, R: int = 16, tile: int = 16, sigma: float = 2.0, seed: int = 1):
+    """Frame pair for full-search ME.
+
+    frame0 = uniform noise smoothed by a 5x5 box; frame1(y, x) =
+    clip(round(frame0[cl(y+my)][cl(x+mx)] + N(0, sigma^2))) with (my, mx)
+    uniform in [-R, R]^2, constant over each ``tile`` x ``tile`` block.
+    Returns (cur=frame1, ref=frame0, true_mv int32 [H//tile+1][W//tile+1][2]).
+    """
+    g = rng(seed)
+    f0 = smooth_noise_u8(H, W, g)
+    ty, tx = -(-H // tile), -(-W // tile)
+    mv = g.integers(-R, R + 1, size=(ty, tx, 2), dtype=np.int32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    my = mv[yy // tile, xx // tile, 0]
+    mx = mv[yy // tile, xx // tile, 1]
+    src = f0[np.clip(yy + my, 0, H - 1), np.clip(xx + mx, 0, W - 1)].astype(np.float64)
+    f1 = np.clip(np.rint(src + g.normal(0.0, sigma, size=(H, W))), 0, 255).astype(np.uint8)
+    return f1, f0, mv
+
+
+def stereo_pair(H: int = 375, W: int = 1242, D: int = 128, tile: int = 32, sigma: float = 2.0, seed: int = 1):
+    """Left/right pair: R[y][x] = L[y][cl(x + d)] + N(0, sigma^2), d piecewise
+    constant over ``tile`` x ``tile`` tiles, uniform in [0, D-1]."""
+    g = rng(seed)
+    L = smooth_noise_u8(H, W, g)
+    ty, tx = -(-H // tile), -(-W // tile)
+    d = g.integers(0, D, size=(ty, tx), dtype=np.int32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    dd = d[yy // tile, xx // tile]
+    src = L[yy, np.clip(xx + dd, 0, W - 1)].astype(np.float64)
+    R = np.clip(np.rint(src + g.normal(0.0, sigma, size=(H, W))), 0, 255).astype(np.uint8)
+    return L, R, d
