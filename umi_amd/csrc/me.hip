@@ -186,16 +186,20 @@ MePlan me_plan(int B, int R) {
     MePlan p;
     p.G = umi::ceil_div(2 * R + 1, 4);
     const int max_threads = B == 8 ? me_max_threads<8>() : me_max_threads<16>();
-    int nblk = 1;
-    while (nblk * 2 * p.G <= max_threads && nblk * 2 <= 256) nblk *= 2;
-    p.tbx = nblk < 16 ? nblk : 16;
-    p.tby = nblk / p.tbx;
-    p.threads = umi::round_up(nblk * p.G, 64);
-    p.ws4 = umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
-    p.wrows = p.tby * B + 2 * R;
     const int S = 2 * R + 1;
     p.idx_bits = umi::bits_for((uint32_t)(S * S - 1));
-    p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
+    int nblk = 1;
+    while (nblk * 2 * p.G <= max_threads && nblk * 2 <= 256) nblk *= 2;
+    // shrink the tile until window + current tile + keys fit 64 KiB of LDS
+    for (;; nblk /= 2) {
+        p.tbx = nblk < 16 ? nblk : 16;
+        p.tby = nblk / p.tbx;
+        p.threads = umi::round_up(nblk * p.G, 64);
+        p.ws4 = umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
+        p.wrows = p.tby * B + 2 * R;
+        p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
+        if (p.lds <= 64 * 1024 || nblk == 1) break;
+    }
     return p;
 }
 

@@ -15,6 +15,10 @@ from ._lib import call, lib
 H2D, D2H, D2D = 1, 2, 3
 
 
+def _handle(stream) -> int | None:
+    return None if stream is None else stream.handle
+
+
 class DeviceBuffer:
     """One device allocation of ``nbytes`` bytes (freed on ``free()``/GC)."""
 
@@ -96,11 +100,11 @@ class Timer:
         call("umi_timer_create", C.byref(p))
         self.handle = p.value
 
-    def start(self, stream=None) -> None:
-        call("umi_timer_start", self.handle, stream)
+    def start(self, stream: "Stream | None" = None) -> None:
+        call("umi_timer_start", self.handle, _handle(stream))
 
-    def stop(self, stream=None) -> None:
-        call("umi_timer_stop", self.handle, stream)
+    def stop(self, stream: "Stream | None" = None) -> None:
+        call("umi_timer_stop", self.handle, _handle(stream))
 
     def ms(self) -> float:
         v = C.c_float()
