@@ -115,6 +115,59 @@ umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, 
 umi_status umi_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D,
                               int s0, int step, int KY, int KX, int32_t *out, umi_stream s);
 
+/* ---------------------------------------------------------------------------
+ * 3x3 filters.  Pitches are in elements.  mode: 0 = PAD (out-of-image taps
+ * read `pad`, the reference's PAD boundary), 1 = clamp to edge (its WRAP).
+ * Taps are accumulated dy outer, dx inner, starting from `init`; the weighted
+ * forms use fused multiply-adds (x * w + acc).
+ *
+ * umi_filter3x3_*  : out = mul * (init + sum x)
+ *   replaces the Filter3x3 launch of main/filter3x3.cu:10-17,36-58.
+ * umi_wfilter3x3_* : out = init + sum x * weights[3*(dy+1)+(dx+1)]
+ *   (weights: 9 values in device memory, like the reference's 2nd input)
+ *   replaces the WFilter3x3 launch of main/wfilter3x3.cu:8-15,39-67.
+ * Integer forms wrap on overflow (two's complement).
+ * ------------------------------------------------------------------------- */
+umi_status umi_filter3x3_f32(const float *in, int H, int W, int in_pitch, float init, float mul, float pad,
+                             int mode, float *out, int out_pitch, umi_stream s);
+umi_status umi_filter3x3_i32(const int32_t *in, int H, int W, int in_pitch, int32_t init, int32_t mul,
+                             int32_t pad, int mode, int32_t *out, int out_pitch, umi_stream s);
+umi_status umi_wfilter3x3_f32(const float *in, int H, int W, int in_pitch, const float *weights, float init,
+                              float pad, int mode, float *out, int out_pitch, umi_stream s);
+umi_status umi_wfilter3x3_i32(const int32_t *in, int H, int W, int in_pitch, const int32_t *weights,
+                              int32_t init, int32_t pad, int mode, int32_t *out, int out_pitch, umi_stream s);
+
+/* ---------------------------------------------------------------------------
+ * Bilateral filter, (2r+1)^2 window.  Replaces the Bilateral launch of
+ * main/bilateral.cu:9-24,54-99:
+ *   c = in[y][x]; for dy, dx in [-r, r] (dy outer):
+ *     n = in[y+dy][x+dx] (or `pad` outside, mode 0; edge pixel, mode 1)
+ *     w = expf((n-c)*(n-c)*norm) * spatial[dy+r] * spatial[dx+r]
+ *     wsum += w; wxsum += w*n
+ *   out = wxsum / wsum
+ * Out-of-image neighbours count in the normalisation (bilateral.cu:111-113).
+ * spatial: 2r+1 floats in device memory; 0 <= r <= 12.
+ * ------------------------------------------------------------------------- */
+umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int radius, const float *spatial,
+                             float norm, float pad, int mode, float *out, int out_pitch, umi_stream s);
+
+/* ---------------------------------------------------------------------------
+ * Integral image out[y][x] = sum_{y'<=y, x'<=x} in[y'][x'].  Replaces the two
+ * scan launches of main/iimg.cu:37-82 (row scan into a transposed
+ * temporary, then a scan of it).  Forms: uint8 -> uint32 and int32 -> int32
+ * (exact; int32 wraps), float -> float with an fp64 column pass.
+ * workspace: device memory of umi_iimg_workspace_size(H, W, kind) bytes
+ * (kind 0 = u8, 1 = i32, 2 = f32); NULL makes the call allocate and free it
+ * itself, which synchronises the stream.  W <= 16000.
+ * ------------------------------------------------------------------------- */
+umi_status umi_iimg_workspace_size(int H, int W, int kind, size_t *bytes);
+umi_status umi_iimg_u8_u32(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
+                           void *workspace, size_t workspace_bytes, umi_stream s);
+umi_status umi_iimg_i32(const int32_t *in, int H, int W, int in_pitch, int32_t *out, int out_pitch,
+                        void *workspace, size_t workspace_bytes, umi_stream s);
+umi_status umi_iimg_f32(const float *in, int H, int W, int in_pitch, float *out, int out_pitch, void *workspace,
+                        size_t workspace_bytes, umi_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,10 +13,21 @@ import numpy as np
 ORACLE_PATH = Path(__file__).resolve().parent / "lib" / "liboracle.so"
 _P = C.c_void_p
 _I = C.c_int
+_F = C.c_float
 _SIGS = {
     "oracle_me_full_search_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P],
     "oracle_sad_volume_i32": [_P, _P, _I, _I, _I, _I, _I, _P],
     "oracle_cost_volume_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "oracle_filter3x3_i32": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _I],
+    "oracle_filter3x3_f32": [_P, _I, _I, _I, _F, _F, _F, _I, _P, _I],
+    "oracle_wfilter3x3_i32": [_P, _I, _I, _I, _P, _I, _I, _I, _P, _I],
+    "oracle_wfilter3x3_f32": [_P, _I, _I, _I, _P, _F, _F, _I, _P, _I],
+    "oracle_wfilter3x3_f64": [_P, _I, _I, _I, _P, _F, _F, _I, _P, _P],
+    "oracle_bilateral_f32": [_P, _I, _I, _I, _I, _P, _F, _F, _I, _P, _I],
+    "oracle_bilateral_f64": [_P, _I, _I, _I, _I, _P, _F, _F, _I, _P],
+    "oracle_iimg_u8_u32": [_P, _I, _I, _I, _P],
+    "oracle_iimg_i32": [_P, _I, _I, _I, _P],
+    "oracle_iimg_f64": [_P, _I, _I, _I, _P],
 }
 _lib = None
 
@@ -68,4 +79,66 @@ def cost_volume(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     y0, y1 = y_range if y_range is not None else (0, H)
     out = np.zeros((H, W, D), np.int32)
     lib().oracle_cost_volume_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
+    return out
+
+
+def filter3x3(img, init, mul, pad, mode=0):
+    dt = np.int32 if img.dtype == np.int32 else np.float32
+    img = np.ascontiguousarray(img, dt)
+    H, W = img.shape
+    out = np.zeros_like(img)
+    f = lib().oracle_filter3x3_i32 if dt == np.int32 else lib().oracle_filter3x3_f32
+    f(_ptr(img), H, W, W, init, mul, pad, mode, _ptr(out), W)
+    return out
+
+
+def wfilter3x3(img, w, init, pad, mode=0):
+    dt = np.int32 if img.dtype == np.int32 else np.float32
+    img = np.ascontiguousarray(img, dt)
+    w = np.ascontiguousarray(w, dt).reshape(9)
+    H, W = img.shape
+    out = np.zeros_like(img)
+    f = lib().oracle_wfilter3x3_i32 if dt == np.int32 else lib().oracle_wfilter3x3_f32
+    f(_ptr(img), H, W, W, _ptr(w), init, pad, mode, _ptr(out), W)
+    return out
+
+
+def wfilter3x3_f64(img, w, init, pad, mode=0):
+    """(value, magnitude sum |w x| + |init|) in double."""
+    img = np.ascontiguousarray(img, np.float32)
+    w = np.ascontiguousarray(w, np.float32).reshape(9)
+    H, W = img.shape
+    out = np.zeros((H, W), np.float64)
+    mag = np.zeros((H, W), np.float64)
+    lib().oracle_wfilter3x3_f64(_ptr(img), H, W, W, _ptr(w), init, pad, mode, _ptr(out), _ptr(mag))
+    return out, mag
+
+
+def bilateral(img, r, ws, norm, pad=0.0, mode=0, f64=False):
+    img = np.ascontiguousarray(img, np.float32)
+    ws = np.ascontiguousarray(ws, np.float32)
+    H, W = img.shape
+    if f64:
+        out = np.zeros((H, W), np.float64)
+        lib().oracle_bilateral_f64(_ptr(img), H, W, W, r, _ptr(ws), norm, pad, mode, _ptr(out))
+    else:
+        out = np.zeros((H, W), np.float32)
+        lib().oracle_bilateral_f32(_ptr(img), H, W, W, r, _ptr(ws), norm, pad, mode, _ptr(out), W)
+    return out
+
+
+def iimg(img):
+    """u8 -> u32, i32 -> i32 (wrapping), f32 -> f64 reference."""
+    H, W = img.shape
+    img = np.ascontiguousarray(img)
+    if img.dtype == np.uint8:
+        out = np.zeros((H, W), np.uint32)
+        lib().oracle_iimg_u8_u32(_ptr(img), H, W, W, _ptr(out))
+    elif img.dtype == np.int32:
+        out = np.zeros((H, W), np.int32)
+        lib().oracle_iimg_i32(_ptr(img), H, W, W, _ptr(out))
+    else:
+        img = np.ascontiguousarray(img, np.float32)
+        out = np.zeros((H, W), np.float64)
+        lib().oracle_iimg_f64(_ptr(img), H, W, W, _ptr(out))
     return out

@@ -2,7 +2,7 @@
  * umi_oracle.c — CPU oracle for the UMI operator workloads.
  *
  * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the CPU check
- * loops that johnjohnlin/UMI uses to validate each example (main/*.cu), with
+ * loops that johnjohnlin/UMI uses to validate each example (the main/ programs), with
  * the generalisations the MI355X library implements (two frames, uint8
  * pixels, argmin, 2-D windows).  Only tests/, __graft_entry__.smoke() and
  * bench.py's CPU-baseline leg may load it, and only as the checker.  The
@@ -113,4 +113,163 @@ void oracle_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int
                 }
                 out[((size_t)y * W + x) * D + i] = sad;
             }
+}
+
+/* Reads in[y][x] or applies the boundary: mode 0 -> pad value, 1 -> clamp
+ * (the PAD / "WRAP"=clamp modes of src/UMI.h:205-218). */
+#define BOUNDED(T, in, H, W, ip, y, x, mode, pad) \
+    (((y) >= 0 && (y) < (H) && (x) >= 0 && (x) < (W)) ? (in)[(size_t)(y) * (ip) + (x)] \
+     : ((mode) == 0 ? (pad) : (in)[(size_t)clampi((y), 0, (H)-1) * (ip) + clampi((x), 0, (W)-1)]))
+
+/*
+ * 3x3 box filter: main/filter3x3.cu:60-78 (sum = init; sum += pixel or pad;
+ * sum *= mul).  Integer form wraps (unsigned arithmetic), float adds in order.
+ */
+void oracle_filter3x3_i32(const int32_t *in, int H, int W, int ip, int32_t init, int32_t mul, int32_t pad,
+                          int mode, int32_t *out, int op) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            uint32_t sum = (uint32_t)init;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) sum += (uint32_t)BOUNDED(int32_t, in, H, W, ip, y + dy, x + dx, mode, pad);
+            out[(size_t)y * op + x] = (int32_t)(sum * (uint32_t)mul);
+        }
+}
+
+void oracle_filter3x3_f32(const float *in, int H, int W, int ip, float init, float mul, float pad, int mode,
+                          float *out, int op) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float sum = init;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) sum += BOUNDED(float, in, H, W, ip, y + dy, x + dx, mode, pad);
+            out[(size_t)y * op + x] = sum * mul;
+        }
+}
+
+/*
+ * Weighted 3x3 filter: main/wfilter3x3.cu:71-89 (sum = init; sum += w * x,
+ * dy outer, dx inner).  The float form states the multiply-add as fmaf, the
+ * contraction the strategy's `accum += x * w` (wfilter3x3.cu:13) gets on the
+ * GPU.  _f64 evaluates the same sum in double (error reference).
+ */
+void oracle_wfilter3x3_i32(const int32_t *in, int H, int W, int ip, const int32_t *w, int32_t init, int32_t pad,
+                           int mode, int32_t *out, int op) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            uint32_t sum = (uint32_t)init;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    sum += (uint32_t)w[3 * (dy + 1) + dx + 1] *
+                           (uint32_t)BOUNDED(int32_t, in, H, W, ip, y + dy, x + dx, mode, pad);
+            out[(size_t)y * op + x] = (int32_t)sum;
+        }
+}
+
+void oracle_wfilter3x3_f32(const float *in, int H, int W, int ip, const float *w, float init, float pad, int mode,
+                           float *out, int op) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float sum = init;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    sum = fmaf(BOUNDED(float, in, H, W, ip, y + dy, x + dx, mode, pad), w[3 * (dy + 1) + dx + 1], sum);
+            out[(size_t)y * op + x] = sum;
+        }
+}
+
+/* FP64 value and magnitude sum_{taps} |w x| (+|init|) of the weighted filter. */
+void oracle_wfilter3x3_f64(const float *in, int H, int W, int ip, const float *w, float init, float pad, int mode,
+                           double *out, double *mag) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double sum = init, m = fabs((double)init);
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const double t = (double)BOUNDED(float, in, H, W, ip, y + dy, x + dx, mode, pad) *
+                                     (double)w[3 * (dy + 1) + dx + 1];
+                    sum += t;
+                    m += fabs(t);
+                }
+            out[(size_t)y * W + x] = sum;
+            mag[(size_t)y * W + x] = m;
+        }
+}
+
+/*
+ * Bilateral filter: main/bilateral.cu:103-124 with the strategy's order
+ * (bilateral.cu:12-23): for dy (outer), dx: n = pixel or pad,
+ * w = expf(d*d*norm) * ws[dy] * ws[dx], wsum += w, wxsum += w*n (as fmaf),
+ * out = wxsum / wsum.  _f64 is the same in double with exp().
+ */
+void oracle_bilateral_f32(const float *in, int H, int W, int ip, int r, const float *ws, float norm, float pad,
+                          int mode, float *out, int op) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const float c = in[(size_t)y * ip + x];
+            float wsum = 0.f, wxsum = 0.f;
+            for (int dy = -r; dy <= r; ++dy)
+                for (int dx = -r; dx <= r; ++dx) {
+                    const float n = BOUNDED(float, in, H, W, ip, y + dy, x + dx, mode, pad);
+                    const float d = n - c;
+                    const float w = expf(d * d * norm) * ws[dy + r] * ws[dx + r];
+                    wsum += w;
+                    wxsum = fmaf(w, n, wxsum);
+                }
+            out[(size_t)y * op + x] = wxsum / wsum;
+        }
+}
+
+void oracle_bilateral_f64(const float *in, int H, int W, int ip, int r, const float *ws, float norm, float pad,
+                          int mode, double *out) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const double c = in[(size_t)y * ip + x];
+            double wsum = 0, wxsum = 0;
+            for (int dy = -r; dy <= r; ++dy)
+                for (int dx = -r; dx <= r; ++dx) {
+                    const double n = BOUNDED(float, in, H, W, ip, y + dy, x + dx, mode, pad);
+                    const double d = n - c;
+                    const double w = exp(d * d * (double)norm) * (double)ws[dy + r] * (double)ws[dx + r];
+                    wsum += w;
+                    wxsum += w * n;
+                }
+            out[(size_t)y * W + x] = wxsum / wsum;
+        }
+}
+
+/*
+ * Integral image: main/iimg.cu:84-112 (first check: ans[y][x] = rowsum +
+ * ans[y-1][x]).  u8 -> u32 and i32 -> i32 in wrapping unsigned arithmetic;
+ * the f64 form accumulates float inputs exactly enough to serve as the error
+ * reference of the float path.
+ */
+void oracle_iimg_u8_u32(const uint8_t *in, int H, int W, int ip, uint32_t *out) {
+    for (int y = 0; y < H; ++y) {
+        uint32_t rowsum = 0;
+        for (int x = 0; x < W; ++x) {
+            rowsum += in[(size_t)y * ip + x];
+            out[(size_t)y * W + x] = rowsum + (y ? out[(size_t)(y - 1) * W + x] : 0u);
+        }
+    }
+}
+
+void oracle_iimg_i32(const int32_t *in, int H, int W, int ip, int32_t *out) {
+    for (int y = 0; y < H; ++y) {
+        uint32_t rowsum = 0;
+        for (int x = 0; x < W; ++x) {
+            rowsum += (uint32_t)in[(size_t)y * ip + x];
+            out[(size_t)y * W + x] = (int32_t)(rowsum + (y ? (uint32_t)out[(size_t)(y - 1) * W + x] : 0u));
+        }
+    }
+}
+
+void oracle_iimg_f64(const float *in, int H, int W, int ip, double *out) {
+    for (int y = 0; y < H; ++y) {
+        double rowsum = 0;
+        for (int x = 0; x < W; ++x) {
+            rowsum += in[(size_t)y * ip + x];
+            out[(size_t)y * W + x] = rowsum + (y ? out[(size_t)(y - 1) * W + x] : 0.0);
+        }
+    }
 }

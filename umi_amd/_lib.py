@@ -44,6 +44,15 @@ SIGNATURES = {
     "umi_me_set_variant": [c_int],
     "umi_me_full_search_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, P],
     "umi_cost_volume_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
+    "umi_filter3x3_f32": [P, c_int, c_int, c_int, c_float, c_float, c_float, c_int, P, c_int, P],
+    "umi_filter3x3_i32": [P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_int, P],
+    "umi_wfilter3x3_f32": [P, c_int, c_int, c_int, P, c_float, c_float, c_int, P, c_int, P],
+    "umi_wfilter3x3_i32": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, P, c_int, P],
+    "umi_bilateral_f32": [P, c_int, c_int, c_int, c_int, P, c_float, c_float, c_int, P, c_int, P],
+    "umi_iimg_workspace_size": [c_int, c_int, c_int, C.POINTER(c_size_t)],
+    "umi_iimg_u8_u32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
+    "umi_iimg_i32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
+    "umi_iimg_f32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
 }
 
 

@@ -1,0 +1,178 @@
+// 3x3 filters and the bilateral filter (gfx950).
+//
+// filter3x3  : out = mul * (init + sum_{3x3} x)           main/filter3x3.cu:10-17,36-58
+// wfilter3x3 : out = init + sum_{dy,dx} x * w[dy][dx]      main/wfilter3x3.cu:8-15,39-67
+// bilateral  : out = sum w*n / sum w, w = exp((n-c)^2 norm) ws[dy] ws[dx]
+//                                                          main/bilateral.cu:9-24,54-99
+// (johnjohnlin/UMI).  Out-of-image taps take the PAD value (the reference's
+// PAD boundary, src/UMI.h:205-218) or, with mode = clamp, the edge pixel.
+// Taps are accumulated in the strategy's order (dy outer, dx inner, starting
+// from init) with fused multiply-adds, so float results equal the oracle's
+// fmaf loop bit for bit (filters) or to a few ulp (bilateral: exp).
+//
+// Design: HBM-bound stencils.  A 256-thread workgroup stages a
+// (TY+2r) x (TX+2r) halo tile in LDS (coalesced loads, boundary applied once
+// at load time), then each thread computes a vertical strip of outputs,
+// reusing the tile rows it has in flight; stores are coalesced along x.
+#include "common.h"
+
+namespace {
+using namespace umi;
+
+constexpr int F_TX = 64, F_TY = 32, F_ROWS = 8;  // 64 x 4 threads, 8 rows each
+
+template <typename T>
+struct Arith;
+template <>
+struct Arith<float> {
+    static __device__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    static __device__ float add(float a, float b) { return a + b; }
+    static __device__ float mul(float a, float b) { return a * b; }
+};
+template <>
+struct Arith<int32_t> {  // two's-complement wrap, as the reference's int32 examples rely on
+    static __device__ int32_t fma(int32_t a, int32_t b, int32_t c) { return (int32_t)((uint32_t)a * (uint32_t)b + (uint32_t)c); }
+    static __device__ int32_t add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+    static __device__ int32_t mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+};
+
+template <typename T>
+__device__ inline T load_bounded(const T *in, int H, int W, int ip, int gy, int gx, int mode, T pad) {
+    if (gy >= 0 && gy < H && gx >= 0 && gx < W) return in[(size_t)gy * ip + gx];
+    if (mode == 0) return pad;
+    return in[(size_t)clampi(gy, 0, H - 1) * ip + clampi(gx, 0, W - 1)];
+}
+
+// WEIGHTED = false: Filter3x3 (box sum then * mul); true: WFilter3x3.
+template <typename T, bool WEIGHTED>
+__global__ __launch_bounds__(256) void filter3x3_kernel(const T *__restrict__ in, int H, int W, int ip,
+                                                        const T *__restrict__ wts, T init, T mul, T pad, int mode,
+                                                        T *__restrict__ out, int op) {
+    constexpr int LW = F_TX + 2, LH = F_TY + 2;
+    __shared__ T tile[LH * LW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * F_TX, y0 = blockIdx.y * F_TY;
+    for (int e = tid; e < LH * LW; e += 256) {
+        const int r = e / LW, c = e - r * LW;
+        tile[e] = load_bounded(in, H, W, ip, y0 - 1 + r, x0 - 1 + c, mode, pad);
+    }
+    T w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w[k] = WEIGHTED ? wts[k] : T(0);
+    __syncthreads();
+    const int tx = tid & (F_TX - 1), ty = tid / F_TX;
+    const int x = x0 + tx;
+    if (x >= W) return;
+#pragma unroll
+    for (int i = 0; i < F_ROWS; ++i) {
+        const int ly = ty * F_ROWS + i;
+        const int y = y0 + ly;
+        if (y >= H) break;
+        T acc = init;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const T v = tile[(ly + dy) * LW + tx + dx];
+                acc = WEIGHTED ? Arith<T>::fma(v, w[dy * 3 + dx], acc) : Arith<T>::add(acc, v);
+            }
+        out[(size_t)y * op + x] = WEIGHTED ? acc : Arith<T>::mul(acc, mul);
+    }
+}
+
+constexpr int BL_TX = 32, BL_TY = 8;  // one output per thread
+constexpr int BL_MAXR = 12;           // the reference example uses sigma 4 -> radius 12
+
+__global__ __launch_bounds__(256) void bilateral_kernel(const float *__restrict__ in, int H, int W, int ip, int r,
+                                                        const float *__restrict__ ws, float norm, float pad,
+                                                        int mode, float *__restrict__ out, int op) {
+    extern __shared__ float bl_lds[];
+    const int LW = BL_TX + 2 * r, LH = BL_TY + 2 * r;
+    float *tile = bl_lds;
+    float *wsh = bl_lds + LW * LH;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * BL_TX, y0 = blockIdx.y * BL_TY;
+    for (int e = tid; e < LH * LW; e += 256) {
+        const int rr = e / LW, c = e - rr * LW;
+        tile[e] = load_bounded(in, H, W, ip, y0 - r + rr, x0 - r + c, mode, pad);
+    }
+    if (tid < 2 * r + 1) wsh[tid] = ws[tid];
+    __syncthreads();
+    const int tx = tid % BL_TX, ty = tid / BL_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= W || y >= H) return;
+    const float center = tile[(ty + r) * LW + tx + r];
+    float wsum = 0.f, wxsum = 0.f;
+    for (int dy = 0; dy <= 2 * r; ++dy) {
+        const float wy = wsh[dy];
+        const float *row = tile + (ty + dy) * LW + tx;
+        for (int dx = 0; dx <= 2 * r; ++dx) {
+            const float n = row[dx];
+            const float d = n - center;
+            const float w = expf(d * d * norm) * wy * wsh[dx];
+            wsum += w;
+            wxsum = __builtin_fmaf(w, n, wxsum);
+        }
+    }
+    out[(size_t)y * op + x] = wxsum / wsum;
+}
+
+template <typename T, bool WEIGHTED>
+umi_status filter_launch(const T *in, int H, int W, int ip, const T *w, T init, T mul, T pad, int mode, T *out,
+                         int op, umi_stream s, const char *name) {
+    if (!in || !out || (WEIGHTED && !w)) return set_error(UMI_ERR_INVALID, "%s: null pointer", name);
+    if (H < 0 || W < 0 || ip < W || op < W)
+        return set_error(UMI_ERR_INVALID, "%s: H=%d W=%d in_pitch=%d out_pitch=%d", name, H, W, ip, op);
+    if (mode != 0 && mode != 1) return set_error(UMI_ERR_INVALID, "%s: mode %d (0 pad, 1 clamp)", name, mode);
+    if (H == 0 || W == 0) return UMI_OK;
+    dim3 grid(ceil_div(W, F_TX), ceil_div(H, F_TY));
+    hipLaunchKernelGGL((filter3x3_kernel<T, WEIGHTED>), grid, dim3(256), 0, as_stream(s), in, H, W, ip, w, init,
+                       mul, pad, mode, out, op);
+    return launch_check(name);
+}
+
+}  // namespace
+
+extern "C" {
+
+umi_status umi_filter3x3_f32(const float *in, int H, int W, int in_pitch, float init, float mul, float pad,
+                             int mode, float *out, int out_pitch, umi_stream s) {
+    return filter_launch<float, false>(in, H, W, in_pitch, nullptr, init, mul, pad, mode, out, out_pitch, s,
+                                       "umi_filter3x3_f32");
+}
+
+umi_status umi_filter3x3_i32(const int32_t *in, int H, int W, int in_pitch, int32_t init, int32_t mul,
+                             int32_t pad, int mode, int32_t *out, int out_pitch, umi_stream s) {
+    return filter_launch<int32_t, false>(in, H, W, in_pitch, nullptr, init, mul, pad, mode, out, out_pitch, s,
+                                         "umi_filter3x3_i32");
+}
+
+umi_status umi_wfilter3x3_f32(const float *in, int H, int W, int in_pitch, const float *weights, float init,
+                              float pad, int mode, float *out, int out_pitch, umi_stream s) {
+    return filter_launch<float, true>(in, H, W, in_pitch, weights, init, 0.f, pad, mode, out, out_pitch, s,
+                                      "umi_wfilter3x3_f32");
+}
+
+umi_status umi_wfilter3x3_i32(const int32_t *in, int H, int W, int in_pitch, const int32_t *weights,
+                              int32_t init, int32_t pad, int mode, int32_t *out, int out_pitch, umi_stream s) {
+    return filter_launch<int32_t, true>(in, H, W, in_pitch, weights, init, 0, pad, mode, out, out_pitch, s,
+                                        "umi_wfilter3x3_i32");
+}
+
+umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int radius, const float *spatial,
+                             float norm, float pad, int mode, float *out, int out_pitch, umi_stream s) {
+    if (!in || !out || !spatial) return set_error(UMI_ERR_INVALID, "umi_bilateral_f32: null pointer");
+    if (H < 0 || W < 0 || in_pitch < W || out_pitch < W)
+        return set_error(UMI_ERR_INVALID, "umi_bilateral_f32: H=%d W=%d pitches %d %d", H, W, in_pitch, out_pitch);
+    if (radius < 0 || radius > BL_MAXR)
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_bilateral_f32: radius %d (0..%d)", radius, BL_MAXR);
+    if (mode != 0 && mode != 1) return set_error(UMI_ERR_INVALID, "umi_bilateral_f32: mode %d", mode);
+    if (H == 0 || W == 0) return UMI_OK;
+    const size_t lds = sizeof(float) * ((size_t)(BL_TX + 2 * radius) * (BL_TY + 2 * radius) + 2 * radius + 1);
+    dim3 grid(ceil_div(W, BL_TX), ceil_div(H, BL_TY));
+    hipLaunchKernelGGL(bilateral_kernel, grid, dim3(256), lds, as_stream(s), in, H, W, in_pitch, radius, spatial,
+                       norm, pad, mode, out, out_pitch);
+    return launch_check("umi_bilateral_f32");
+}
+
+}  // extern "C"

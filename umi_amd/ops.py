@@ -68,3 +68,97 @@ def cost_volume_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int 
     out = DeviceBuffer(H * W * D * 4)
     cost_volume(dl, dr, H, W, W, D, s0, step, KY, KX, out)
     return out.download((H, W, D), np.int32)
+
+
+# ---------------------------------------------------------------------------
+# 3x3 filters (main/filter3x3.cu, main/wfilter3x3.cu) and bilateral
+# (main/bilateral.cu), johnjohnlin/UMI
+# ---------------------------------------------------------------------------
+PAD, CLAMP = 0, 1
+
+
+def filter3x3(inp: DeviceBuffer, H: int, W: int, in_pitch: int, init, mul, pad, mode: int, out: DeviceBuffer,
+              out_pitch: int, dtype=np.float32, stream=None) -> None:
+    name = "umi_filter3x3_f32" if dtype == np.float32 else "umi_filter3x3_i32"
+    call(name, inp.ptr, H, W, in_pitch, init, mul, pad, mode, out.ptr, out_pitch, _stream(stream))
+
+
+def wfilter3x3(inp: DeviceBuffer, H: int, W: int, in_pitch: int, weights: DeviceBuffer, init, pad, mode: int,
+               out: DeviceBuffer, out_pitch: int, dtype=np.float32, stream=None) -> None:
+    name = "umi_wfilter3x3_f32" if dtype == np.float32 else "umi_wfilter3x3_i32"
+    call(name, inp.ptr, H, W, in_pitch, weights.ptr, init, pad, mode, out.ptr, out_pitch, _stream(stream))
+
+
+def bilateral(inp: DeviceBuffer, H: int, W: int, in_pitch: int, radius: int, spatial: DeviceBuffer, norm: float,
+              pad: float, mode: int, out: DeviceBuffer, out_pitch: int, stream=None) -> None:
+    call("umi_bilateral_f32", inp.ptr, H, W, in_pitch, radius, spatial.ptr, norm, pad, mode, out.ptr, out_pitch,
+         _stream(stream))
+
+
+def filter3x3_np(img: np.ndarray, init, mul, pad, mode: int = PAD) -> np.ndarray:
+    dt = np.int32 if img.dtype == np.int32 else np.float32
+    img = np.ascontiguousarray(img, dt)
+    H, W = img.shape
+    d, o = DeviceBuffer.from_array(img), DeviceBuffer(max(img.nbytes, 4))
+    filter3x3(d, H, W, W, init, mul, pad, mode, o, W, dt)
+    return o.download((H, W), dt)
+
+
+def wfilter3x3_np(img: np.ndarray, w: np.ndarray, init, pad, mode: int = PAD) -> np.ndarray:
+    dt = np.int32 if img.dtype == np.int32 else np.float32
+    img = np.ascontiguousarray(img, dt)
+    H, W = img.shape
+    d, o = DeviceBuffer.from_array(img), DeviceBuffer(max(img.nbytes, 4))
+    dw = DeviceBuffer.from_array(np.ascontiguousarray(w, dt).reshape(9))
+    wfilter3x3(d, H, W, W, dw, init, pad, mode, o, W, dt)
+    return o.download((H, W), dt)
+
+
+def spatial_weights(radius: int, sigma_s: float) -> np.ndarray:
+    """ws[d] = exp(-d^2 / (2 sigma_s^2)), d = -r..r (main/bilateral.cu:45-48)."""
+    d = np.arange(-radius, radius + 1, dtype=np.float64)
+    return np.exp(-d * d / (2.0 * sigma_s * sigma_s)).astype(np.float32)
+
+
+def bilateral_np(img: np.ndarray, radius: int, sigma_s: float, sigma_r: float, pad: float = 0.0,
+                 mode: int = PAD) -> np.ndarray:
+    img = np.ascontiguousarray(img, np.float32)
+    H, W = img.shape
+    ws = spatial_weights(radius, sigma_s)
+    norm = np.float32(-1.0 / (2.0 * sigma_r * sigma_r))
+    d, o, dw = DeviceBuffer.from_array(img), DeviceBuffer(max(img.nbytes, 4)), DeviceBuffer.from_array(ws)
+    bilateral(d, H, W, W, radius, dw, float(norm), pad, mode, o, W)
+    return o.download((H, W), np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Integral image (main/iimg.cu), johnjohnlin/UMI
+# ---------------------------------------------------------------------------
+_IIMG = {np.dtype(np.uint8): (0, "umi_iimg_u8_u32", np.uint32), np.dtype(np.int32): (1, "umi_iimg_i32", np.int32),
+         np.dtype(np.float32): (2, "umi_iimg_f32", np.float32)}
+
+
+def iimg_workspace_size(H: int, W: int, dtype) -> int:
+    import ctypes
+
+    kind = _IIMG[np.dtype(dtype)][0]
+    n = ctypes.c_size_t(0)
+    call("umi_iimg_workspace_size", H, W, kind, ctypes.byref(n))
+    return n.value
+
+
+def iimg(inp: DeviceBuffer, H: int, W: int, in_pitch: int, out: DeviceBuffer, out_pitch: int, dtype,
+         workspace: DeviceBuffer | None = None, stream=None) -> None:
+    _, name, _ = _IIMG[np.dtype(dtype)]
+    call(name, inp.ptr, H, W, in_pitch, out.ptr, out_pitch, workspace.ptr if workspace is not None else None,
+         workspace.nbytes if workspace is not None else 0, _stream(stream))
+
+
+def iimg_np(img: np.ndarray, use_workspace: bool = True) -> np.ndarray:
+    img = np.ascontiguousarray(img)
+    H, W = img.shape
+    _, _, odt = _IIMG[img.dtype]
+    d, o = DeviceBuffer.from_array(img), DeviceBuffer(max(H * W * 4, 4))
+    ws = DeviceBuffer(max(iimg_workspace_size(H, W, img.dtype), 4)) if use_workspace else None
+    iimg(d, H, W, W, o, W, img.dtype, ws)
+    return o.download((H, W), odt)
