@@ -61,6 +61,19 @@ umi_status umi_launch_kernel(const void *func, unsigned gx, unsigned gy, unsigne
                              unsigned bx, unsigned by, unsigned bz, size_t shmem,
                              void **args, umi_stream s);
 
+/* Validates the descriptors of a generic UMI::Execute<> call (the checks of
+ * src/UMI.cpp:39-66 plus view/size/plane sanity) and returns a status.
+ *   p_dims[16] : per dim {total, block, warp, shuf} (shuf 0 read as 1)
+ *   a_dims[8]  : per dim {total, block}
+ *   views      : per array (n_in inputs then n_out outputs) 24 ints:
+ *                p view {id, ofs, stride} x4, then a view x4
+ *   phys       : per array 8 ints: {size, mode} x4 (mode 0 PAD, 1 WRAP=clamp)
+ *   planes     : per array plane count (the PtrNum nibble)
+ *   expand     : Expand template word (nibble per dim, dim 0 highest) */
+umi_status umi_validate_execute(const int32_t *p_dims, const int32_t *a_dims, int n_in, int n_out,
+                                const int32_t *views, const int32_t *phys, const int32_t *planes,
+                                uint64_t expand, uint64_t zero_stride_opt);
+
 /* Event timer on a stream (hipEvent pair).  No reference counterpart: the
  * reference times nothing (src/Timer.h is never included). */
 typedef struct umi_timer_s *umi_timer;

@@ -47,3 +47,63 @@ def test_errors_are_status_codes_not_aborts():
     assert b"null" in L.umi_last_error()
     assert L.umi_me_full_search_u8(1, 1, 64, 64, 64, 12, 4, 1, None, None) == 3
     assert L.umi_cost_volume_u8(1, 1, 8, 8, 8, 4, 0, 2, 1, 1, 1, None) == 1
+
+
+def _pack(pd, ad):
+    import numpy as np
+
+    return np.array(pd, np.int32).reshape(16), np.array(ad, np.int32).reshape(8)
+
+
+def _validate(pd, ad, expand=0x1111, views=None, phys=None, planes=None, n_in=0, n_out=0):
+    import numpy as np
+
+    from umi_amd._lib import lib
+
+    p, a = _pack(pd, ad)
+    v = np.zeros(max(1, 24 * (n_in + n_out)), np.int32) if views is None else np.asarray(views, np.int32)
+    ph = np.ones(max(1, 8 * (n_in + n_out)), np.int32) if phys is None else np.asarray(phys, np.int32)
+    if phys is None:
+        ph[1::2] = 0
+    pl = np.ones(max(1, n_in + n_out), np.int32) if planes is None else np.asarray(planes, np.int32)
+    return lib().umi_validate_execute(p.ctypes.data, a.ctypes.data, n_in, n_out, v.ctypes.data, ph.ctypes.data,
+                                      pl.ctypes.data, expand, 0)
+
+
+GOOD_P = [[1, 1, 1, 1], [1, 1, 1, 1], [300, 16, 4, 1], [500, 16, 8, 1]]
+GOOD_A = [[1, 1], [1, 1], [3, 3], [3, 3]]
+
+
+def test_validate_execute_accepts_reference_dims():
+    assert _validate(GOOD_P, GOOD_A, n_in=1, n_out=1) == 0
+    # main/iimg.cu:60 passes 3-element ParallelDims (warp_shuf 0): accepted as 1
+    assert _validate([[1, 1, 1, 0], [1, 1, 1, 0], [1, 1, 1, 0], [500, 128, 32, 0]], GOOD_A) == 0
+
+
+@pytest.mark.parametrize("pd,why", [
+    ([[1, 1, 1, 1], [1, 1, 1, 1], [300, 16, 2, 1], [500, 16, 8, 1]], b"not 32"),
+    ([[1, 1, 1, 1], [1, 1, 1, 1], [300, 16, 4, 3], [500, 16, 8, 1]], b"power of two"),
+    ([[1, 1, 1, 1], [1, 1, 1, 1], [300, 12, 8, 1], [500, 16, 4, 1]], b"multiple"),
+    ([[1, 1, 1, 1], [1, 1, 1, 1], [-3, 16, 4, 1], [500, 16, 8, 1]], b"total_size"),
+])
+def test_validate_execute_rejects(pd, why):
+    from umi_amd._lib import lib
+
+    assert _validate(pd, GOOD_A) == 1
+    assert why in lib().umi_last_error()
+
+
+def test_validate_execute_rejects_bad_views_and_sizes():
+    import numpy as np
+
+    from umi_amd._lib import lib
+
+    views = np.zeros(24, np.int32)
+    views[3] = 7  # p view 1 -> physical dim 7
+    assert _validate(GOOD_P, GOOD_A, views=views, n_in=1) == 1
+    phys = np.array([1, 0, 0, 0, 5, 0, 5, 0], np.int32)  # input with a zero-size dim
+    assert _validate(GOOD_P, GOOD_A, phys=phys, n_in=1) == 1
+    assert b"size" in lib().umi_last_error()
+    assert _validate(GOOD_P, GOOD_A, planes=[5], n_in=1) == 1
+    assert _validate(GOOD_P, GOOD_A, expand=0x1101) == 1
+    assert _validate(GOOD_P, [[1, 1], [1, 0], [3, 3], [3, 3]]) == 1

@@ -41,6 +41,7 @@ SIGNATURES = {
     "umi_graph_end": [P, C.POINTER(c_void_p)],
     "umi_graph_launch": [P, P],
     "umi_graph_destroy": [P],
+    "umi_validate_execute": [P, P, c_int, c_int, P, P, P, C.c_uint64, C.c_uint64],
     "umi_me_set_variant": [c_int],
     "umi_me_full_search_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, P],
     "umi_cost_volume_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],

@@ -69,6 +69,48 @@ def build_library(force: bool = True, jobs: int | None = None) -> Path:
     return LIB
 
 
+TEST_CPP = ROOT / "tests" / "cpp"
+REF_MAIN = Path("/root/reference/main")
+
+
+def _hipcc_program(src: Path, out: Path, extra=()) -> None:
+    out.parent.mkdir(parents=True, exist_ok=True)
+    rpath = os.path.relpath(LIB.parent, out.parent)
+    cmd = [HIPCC, "-x", "hip", *HIP_FLAGS[:2], "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", f"-I{PKG / 'src'}", f"-I{ROOT / 'include'}", f"-I{ROOT / 'oracle'}", *extra,
+           str(src), "-o", str(out), f"-L{LIB.parent}", "-lumi_hip", f"-Wl,-rpath,$ORIGIN/{rpath}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+
+
+def build_tests() -> list[Path]:
+    """C++ test programs of the generic engine (tests/cpp/*.cpp -> tests/cpp/bin/)."""
+    outs = []
+    srcs = sorted(TEST_CPP.glob("*.cpp"))
+    with cf.ThreadPoolExecutor(max_workers=max(1, len(srcs))) as ex:
+        for src, _ in zip(srcs, ex.map(lambda s: _hipcc_program(s, TEST_CPP / "bin" / s.stem), srcs)):
+            outs.append(TEST_CPP / "bin" / src.stem)
+    return outs
+
+
+def build_dropin() -> list[Path]:
+    """The reference's own example programs (main/*.cu, read where they lie in
+    /root/reference) compiled unchanged against this project's UMI headers into
+    oracle/_ref/<name>_gpu.  Each program checks its results with its own CPU
+    loop.  main/bilateral.cu calls two CUDA runtime functions outside the UMI
+    API (bilateral.cu:101); this recipe maps them to their HIP equivalents on the
+    command line.  Skipped when /root/reference is absent."""
+    if not REF_MAIN.is_dir():
+        return []
+    outdir = ROOT / "oracle" / "_ref"
+    srcs = sorted(REF_MAIN.glob("*.cu"))
+    extra = ["-DcudaGetLastError=hipGetLastError", "-DcudaGetErrorString=hipGetErrorString", "-UNDEBUG", "-w"]
+    with cf.ThreadPoolExecutor(max_workers=max(1, len(srcs))) as ex:
+        list(ex.map(lambda s: _hipcc_program(s, outdir / f"{s.stem}_gpu", extra), srcs))
+    return [outdir / f"{s.stem}_gpu" for s in srcs]
+
+
 def build_oracle() -> Path:
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
@@ -81,3 +123,5 @@ if __name__ == "__main__":
     lib = build_library(force="--force" in sys.argv)
     print(lib)
     print(build_oracle())
+    for p in build_tests() + build_dropin():
+        print(p)
