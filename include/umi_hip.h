@@ -181,6 +181,47 @@ umi_status umi_iimg_i32(const int32_t *in, int H, int W, int in_pitch, int32_t *
 umi_status umi_iimg_f32(const float *in, int H, int W, int in_pitch, float *out, int out_pitch, void *workspace,
                         size_t workspace_bytes, umi_stream s);
 
+/* ---------------------------------------------------------------------------
+ * Dense contractions on the matrix cores.  The reference runs GEMM and
+ * direct convolution as scalar UMI launches of its Gemm strategy
+ * (main/gemm.cu:8-15,37-67; main/fprop.cu:10-17,53-82); these entry points
+ * are their MFMA form.  Precision modes:
+ *   0 F32     fp32 operands, v_mfma_f32_32x32x2_f32
+ *   1 BF16    bf16 operands (RNE), fp32 accumulation
+ *   2 BF16X3  bf16 hi/lo split of both operands, Ah*Bh + Al*Bh + Ah*Bl
+ *   3 FP8     OCP e4m3 operands scaled per tensor by q = 448/amax
+ *   4 FP8X3   FP8 plus residual terms with e4m3((x*q - hi) * 16) at weight 2^-4
+ * Operands are prepared ("packed") from fp32 by umi_gemm_pack /
+ * umi_conv_pack_*; split modes produce a hi and a lo operand.  For the FP8
+ * modes the per-tensor amax comes from umi_amax_f32 and the packer writes
+ * the scale q it used to q_out (device float), which the contraction reads
+ * back to scale its result by 1/(qa*qb).  All scalars stay in device memory,
+ * so a whole pack+contract sequence can be captured in a graph.
+ * ------------------------------------------------------------------------- */
+umi_status umi_gemm_elem_bytes(int mode, int *bytes);
+umi_status umi_amax_f32(const float *src, size_t n, float *amax, umi_stream s);
+/* dst[r][k] (K-contiguous, ld_dst elements per row) from fp32 src:
+ * transpose = 0: src[r][k] (ld_src per row); 1: src[k][r] (ld_src per row). */
+umi_status umi_gemm_pack(const float *src, int R, int K, int ld_src, int transpose, int mode, const float *amax,
+                         void *dst_hi, void *dst_lo, int ld_dst, float *q_out, umi_stream s);
+/* C[m][n] = sum_k A[m][k] * Bt[n][k] (/(qa*qb) in the FP8 modes); A, Bt
+ * K-contiguous (lda, ldb elements), C fp32 row-major.  K * elem_bytes must
+ * be a multiple of 64 and lda, ldb * elem_bytes of 16. */
+umi_status umi_gemm(int mode, int M, int N, int K, const void *a_hi, const void *a_lo, int lda, const void *b_hi,
+                    const void *b_lo, int ldb, float *C, int ldc, const float *qa, const float *qb, umi_stream s);
+/* Convolution forward: out[n][k][oy][ox] = sum_{c,r,s} w[k][c][r][s] *
+ * in[n][c][oy*stride+r-pad][ox*stride+s-pad] (zero padding), NCHW / KCRS
+ * fp32 in the caller's layout (main/fprop.cu:84-108).  The input is packed
+ * to NHWC and the filter to KRSC; out is NCHW fp32.  C * elem_bytes must be
+ * a multiple of 16 and R*S*C * elem_bytes of 64. */
+umi_status umi_conv_pack_input(const float *src_nchw, int N, int C, int H, int W, int mode, const float *amax,
+                               void *hi, void *lo, float *q_out, umi_stream s);
+umi_status umi_conv_pack_filter(const float *src_kcrs, int K, int C, int R, int S, int mode, const float *amax,
+                                void *hi, void *lo, float *q_out, umi_stream s);
+umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, int S, int stride, int pad,
+                          const void *in_hi, const void *in_lo, const void *w_hi, const void *w_lo, float *out,
+                          const float *q_in, const float *q_w, umi_stream s);
+
 #ifdef __cplusplus
 }
 #endif

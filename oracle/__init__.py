@@ -142,3 +142,88 @@ def iimg(img):
         out = np.zeros((H, W), np.float64)
         lib().oracle_iimg_f64(_ptr(img), H, W, W, _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# FP64 oracles of the dense contractions (main/gemm.cu:69-78 and
+# main/fprop.cu:84-108 restated in float64 numpy) and decoders of the packed
+# operand formats, for error statistics.
+# ---------------------------------------------------------------------------
+def decode_bf16(u16: np.ndarray) -> np.ndarray:
+    return (u16.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _e4m3_table() -> np.ndarray:
+    t = np.zeros(256, np.float64)
+    for b in range(256):
+        s = -1.0 if b & 0x80 else 1.0
+        e = (b >> 3) & 0xF
+        m = b & 7
+        if e == 15 and m == 7:
+            t[b] = np.nan
+        elif e == 0:
+            t[b] = s * (m / 8.0) * 2.0 ** -6
+        else:
+            t[b] = s * (1 + m / 8.0) * 2.0 ** (e - 7)
+    return t
+
+
+E4M3 = _e4m3_table()
+
+
+def decode_e4m3(u8: np.ndarray) -> np.ndarray:
+    return E4M3[u8]
+
+
+def gemm_f64(A, B):
+    """C = A @ B in float64 (the row-major A[m][k], B[k][n] of main/gemm.cu)."""
+    return np.asarray(A, np.float64) @ np.asarray(B, np.float64)
+
+
+def im2col_f64(x, R, S, stride, pad):
+    N, C, H, W = x.shape
+    OH, OW = (H + 2 * pad - R) // stride + 1, (W + 2 * pad - S) // stride + 1
+    xp = np.pad(np.asarray(x, np.float64), ((0, 0), (0, 0), (pad, pad), (pad, pad)))
+    cols = np.empty((N, OH, OW, C, R, S), np.float64)
+    for r in range(R):
+        for s in range(S):
+            cols[:, :, :, :, r, s] = xp[:, :, r:r + stride * OH:stride, s:s + stride * OW:stride].transpose(0, 2, 3, 1)
+    return cols.reshape(N * OH * OW, C * R * S), (N, OH, OW)
+
+
+def conv_f64(x, w, stride, pad):
+    """out[n][k][oy][ox] = sum_{c,r,s} w[k][c][r][s] x[n][c][oy*s+r-p][ox*s+s-p], zero padding, float64."""
+    K, C, R, S = w.shape
+    cols, (N, OH, OW) = im2col_f64(x, R, S, stride, pad)
+    out = cols @ np.asarray(w, np.float64).reshape(K, C * R * S).T
+    return out.reshape(N, OH, OW, K).transpose(0, 3, 1, 2)
+
+
+def conv_direct_i32(x, w, stride, pad):
+    """The loop of main/fprop.cu:84-108 (int32 wrap), for small shapes."""
+    N, C, H, W = x.shape
+    K, _, R, S = w.shape
+    OH, OW = (H + 2 * pad - R) // stride + 1, (W + 2 * pad - S) // stride + 1
+    out = np.zeros((N, K, OH, OW), np.int64)
+    for n in range(N):
+        for k in range(K):
+            for oy in range(OH):
+                for ox in range(OW):
+                    acc = 0
+                    for c in range(C):
+                        for r in range(R):
+                            for s in range(S):
+                                yy, xx = oy * stride + r - pad, ox * stride + s - pad
+                                if 0 <= yy < H and 0 <= xx < W:
+                                    acc += int(w[k, c, r, s]) * int(x[n, c, yy, xx])
+                    out[n, k, oy, ox] = acc
+    return (out & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+def contraction_error(C, ref, A_rows_norm, B_cols_norm):
+    """max / mean of |C - ref| / (||a_m|| ||b_n||), and max |C - ref| / max |ref|."""
+    err = np.abs(np.asarray(C, np.float64) - ref)
+    scale = np.outer(A_rows_norm, B_cols_norm)
+    rel = err / np.maximum(scale, 1e-300)
+    return {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+            "max_abs_over_max_ref": float(err.max() / np.abs(ref).max())}

@@ -54,6 +54,14 @@ SIGNATURES = {
     "umi_iimg_u8_u32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_iimg_i32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_iimg_f32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
+    "umi_gemm_elem_bytes": [c_int, C.POINTER(c_int)],
+    "umi_amax_f32": [P, c_size_t, P, P],
+    "umi_gemm_pack": [P, c_int, c_int, c_int, c_int, c_int, P, P, P, c_int, P, P],
+    "umi_gemm": [c_int, c_int, c_int, c_int, P, P, c_int, P, P, c_int, P, c_int, P, P, P],
+    "umi_conv_pack_input": [P, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P],
+    "umi_conv_pack_filter": [P, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P],
+    "umi_conv_fprop": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P,
+                       P],
 }
 
 

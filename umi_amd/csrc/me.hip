@@ -23,6 +23,8 @@
 //    minima are merged with an LDS atomic min per block.
 #include "common.h"
 
+#include <type_traits>
+
 namespace {
 using namespace umi;
 
@@ -30,6 +32,17 @@ using namespace umi;
 // accumulators per thread, which needs the 256-VGPR budget of <= 512 threads.
 template <int B>
 constexpr int me_max_threads() { return B == 8 ? 1024 : 512; }
+
+// Four clamped bytes of row `row` starting at column gx, as one dword.
+// Interior, 4-byte-aligned columns take a single dword load.
+__device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
+    if (gx >= 0 && gx + 3 < W && ((reinterpret_cast<uintptr_t>(row + gx) & 3) == 0))
+        return *reinterpret_cast<const uint32_t *>(row + gx);
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
+    return v;
+}
 
 template <int B, bool QSAD, bool VOLUME>
 __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
@@ -52,23 +65,13 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
     // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(x0-R+c)].
     for (int i = tid; i < wrows * ws4; i += nthr) {
         const int r = i / ws4, c4 = i - r * ws4;
-        const uint8_t *row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
-        const int gx = x0 - R + 4 * c4;
-        uint32_t v = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
-        win[i] = v;
+        win[i] = load4_clamped(ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch, x0 - R + 4 * c4, W);
     }
     // Current tile.  Blocks past the frame edge read clamped pixels and are
     // never written out.
     for (int i = tid; i < tby * B * cs4; i += nthr) {
         const int r = i / cs4, c4 = i - r * cs4;
-        const uint8_t *row = cur + (size_t)min(y0 + r, H - 1) * pitch;
-        const int gx = x0 + 4 * c4;
-        uint32_t v = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) v |= (uint32_t)row[min(gx + b, W - 1)] << (8 * b);
-        ctile[i] = v;
+        ctile[i] = load4_clamped(cur + (size_t)min(y0 + r, H - 1) * pitch, x0 + 4 * c4, W);
     }
     for (int i = tid; i < nblk; i += nthr) best[i] = 0xffffffffu;
     __syncthreads();
@@ -99,65 +102,80 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
         }
         uint32_t bestk = 0xffffffffu;
         const int nrows = 2 * R + B;
-        for (int r0 = 0; r0 < nrows; r0 += B) {
+
+        // One window row r (= r0 + j): add its contribution to the open
+        // vertical offsets dyi = r - py, then close dyi = r - (B-1).
+        // FIRST (r0 == 0) skips py > j (dyi < 0); LAST skips dyi > 2R and
+        // rows past the window.  Skipped terms only feed discarded slots.
+        auto row = [&](const int r0, const int j, auto first_tag, auto last_tag) {
+            constexpr bool FIRST = decltype(first_tag)::value;
+            constexpr bool LAST = decltype(last_tag)::value;
+            const int r = r0 + j;
+            if (LAST && r >= nrows) return;
+            const uint32_t *wr = wbase + r * ws4;
+            uint32_t w[B4 + 1];
 #pragma unroll
-            for (int j = 0; j < B; ++j) {
-                const int r = r0 + j;
-                if (r < nrows) {
-                    const uint32_t *wr = wbase + r * ws4;
-                    uint32_t w[B4 + 1];
+            for (int k = 0; k <= B4; ++k) w[k] = wr[k];
+            if constexpr (!QSAD) {
+                uint32_t u[4][B4];
 #pragma unroll
-                    for (int k = 0; k <= B4; ++k) w[k] = wr[k];
-                    if constexpr (QSAD) {
+                for (int k = 0; k < B4; ++k) {
+                    u[0][k] = w[k];
 #pragma unroll
-                        for (int py = 0; py < B; ++py) {
-                            const int slot = (j - py + B) % B;
+                    for (int q = 1; q < 4; ++q) u[q][k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], q);
+                }
 #pragma unroll
-                            for (int k = 0; k < B4; ++k) {
-                                const uint64_t s0 = (uint64_t)w[k] | ((uint64_t)w[k + 1] << 32);
-                                acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(s0, c[py][k], acc[slot]);
-                            }
-                        }
-                    } else {
-                        uint32_t u[4][B4];
+                for (int py = 0; py < B; ++py) {
+                    if (FIRST && py > j) continue;
+                    if (LAST && r - py > 2 * R) continue;
+                    const int slot = (j - py + B) % B;
 #pragma unroll
-                        for (int k = 0; k < B4; ++k) {
-                            u[0][k] = w[k];
+                    for (int q = 0; q < 4; ++q)
 #pragma unroll
-                            for (int q = 1; q < 4; ++q) u[q][k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], q);
-                        }
+                        for (int k = 0; k < B4; ++k)
+                            acc32[slot][q] = __builtin_amdgcn_sad_u8(u[q][k], c[py][k], acc32[slot][q]);
+                }
+            } else {
 #pragma unroll
-                        for (int py = 0; py < B; ++py) {
-                            const int slot = (j - py + B) % B;
+                for (int py = 0; py < B; ++py) {
+                    if (FIRST && py > j) continue;
+                    if (LAST && r - py > 2 * R) continue;
+                    const int slot = (j - py + B) % B;
 #pragma unroll
-                            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                                for (int k = 0; k < B4; ++k)
-                                    acc32[slot][q] = __builtin_amdgcn_sad_u8(u[q][k], c[py][k], acc32[slot][q]);
-                        }
+                    for (int k = 0; k < B4; ++k) {
+                        const uint64_t s0 = (uint64_t)w[k] | ((uint64_t)w[k + 1] << 32);
+                        acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(s0, c[py][k], acc[slot]);
                     }
-                    // Row r completes vertical offset index dyi = r - (B-1).
-                    const int fs = (j + 1) % B;
-                    const int dyi = r - (B - 1);
-                    if (dyi >= 0) {
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const int dxi = 4 * g + q;
-                            if (dxi < S) {
-                                const uint32_t sad =
-                                    QSAD ? (uint32_t)(acc[fs] >> (16 * q)) & 0xffffu : acc32[fs][q];
-                                const uint32_t key = (sad << idx_bits) | (uint32_t)(dyi * S + dxi);
-                                bestk = min(bestk, key);
-                                if (VOLUME && valid) volume[(blk * S + dyi) * S + dxi] = (int32_t)sad;
-                            }
-                        }
-                    }
-                    acc[fs] = 0;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc32[fs][q] = 0;
                 }
             }
+            const int fs = (j + 1) % B;
+            const int dyi = r - (B - 1);
+            if ((!FIRST || j == B - 1) && dyi >= 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dxi = 4 * g + q;
+                    if (dxi < S) {
+                        const uint32_t sad = QSAD ? (uint32_t)(acc[fs] >> (16 * q)) & 0xffffu : acc32[fs][q];
+                        bestk = min(bestk, (sad << idx_bits) | (uint32_t)(dyi * S + dxi));
+                        if (VOLUME && valid) volume[(blk * S + dyi) * S + dxi] = (int32_t)sad;
+                    }
+                }
+            }
+            acc[fs] = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc32[fs][q] = 0;
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        const int nchunks = (nrows + B - 1) / B;  // >= 2 since R >= 1
+#pragma unroll
+        for (int j = 0; j < B; ++j) row(0, j, T{}, F{});
+        for (int ch = 1; ch < nchunks - 1; ++ch) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) row(ch * B, j, F{}, F{});
         }
+#pragma unroll
+        for (int j = 0; j < B; ++j) row((nchunks - 1) * B, j, F{}, T{});
         atomicMin(&best[bl], bestk);
     }
     __syncthreads();
@@ -226,8 +244,9 @@ umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int p
 
 extern "C" {
 
-// Selects the SAD instruction form (1 = v_qsad_pk_u16_u8, 0 = v_sad_u8).
-// Both give identical results; exported for measurement.
+// Selects the SAD instruction form for 8x8 blocks (1 = v_qsad_pk_u16_u8,
+// 0 = v_sad_u8 + v_alignbyte_b32).  Both give identical results; exported
+// for measurement.
 umi_status umi_me_set_variant(int variant) {
     if (variant != 0 && variant != 1) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
     g_me_variant = variant;
@@ -252,8 +271,9 @@ umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, 
     if (B == 8)
         return g_me_variant ? me_launch<8, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
                             : me_launch<8, false>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
-    return g_me_variant ? me_launch<16, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
-                        : me_launch<16, false>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
+    // B = 16 always uses v_qsad_pk_u16_u8: the v_sad_u8 form needs 64 more
+    // accumulator registers than fit without spilling.
+    return me_launch<16, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
 }
 
 }  // extern "C"

@@ -162,3 +162,124 @@ def iimg_np(img: np.ndarray, use_workspace: bool = True) -> np.ndarray:
     ws = DeviceBuffer(max(iimg_workspace_size(H, W, img.dtype), 4)) if use_workspace else None
     iimg(d, H, W, W, o, W, img.dtype, ws)
     return o.download((H, W), odt)
+
+
+# ---------------------------------------------------------------------------
+# MFMA GEMM / convolution (main/gemm.cu, main/fprop.cu), johnjohnlin/UMI
+# ---------------------------------------------------------------------------
+F32, BF16, BF16X3, FP8, FP8X3 = 0, 1, 2, 3, 4
+MODE_NAMES = {F32: "f32", BF16: "bf16", BF16X3: "bf16x3", FP8: "fp8", FP8X3: "fp8x3"}
+SPLIT = {BF16X3, FP8X3}
+FP8_MODES = {FP8, FP8X3}
+
+
+def elem_bytes(mode: int) -> int:
+    import ctypes
+
+    n = ctypes.c_int(0)
+    call("umi_gemm_elem_bytes", mode, ctypes.byref(n))
+    return n.value
+
+
+class Operand:
+    """A packed contraction operand: hi (and lo for split modes) plus its scale q."""
+
+    def __init__(self, rows: int, k: int, mode: int):
+        self.rows, self.k, self.mode = rows, k, mode
+        eb = elem_bytes(mode)
+        self.hi = DeviceBuffer(max(rows * k * eb, 16))
+        self.lo = DeviceBuffer(max(rows * k * eb, 16)) if mode in SPLIT else None
+        self.amax = DeviceBuffer(4)
+        self.q = DeviceBuffer(4) if mode in FP8_MODES else None
+
+    @property
+    def lo_ptr(self):
+        return self.lo.ptr if self.lo is not None else None
+
+    @property
+    def q_ptr(self):
+        return self.q.ptr if self.q is not None else None
+
+
+def gemm_pack(src: DeviceBuffer, R: int, K: int, ld_src: int, transpose: bool, mode: int, stream=None) -> Operand:
+    op = Operand(R, K, mode)
+    if mode in FP8_MODES:
+        call("umi_amax_f32", src.ptr, R * K, op.amax.ptr, _stream(stream))
+    call("umi_gemm_pack", src.ptr, R, K, ld_src, int(transpose), mode, op.amax.ptr, op.hi.ptr, op.lo_ptr, K,
+         op.q_ptr, _stream(stream))
+    return op
+
+
+def gemm(mode: int, M: int, N: int, K: int, A: Operand, Bt: Operand, C: DeviceBuffer, stream=None) -> None:
+    call("umi_gemm", mode, M, N, K, A.hi.ptr, A.lo_ptr, K, Bt.hi.ptr, Bt.lo_ptr, K, C.ptr, N, A.q_ptr, Bt.q_ptr,
+         _stream(stream))
+
+
+def gemm_np(A: np.ndarray, B: np.ndarray, mode: int, return_operands: bool = False):
+    """C = A @ B for fp32 A (M x K) and B (K x N, the reference's row-major layout)."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = A.shape
+    N = B.shape[1]
+    dA, dB = DeviceBuffer.from_array(A), DeviceBuffer.from_array(B)
+    oa = gemm_pack(dA, M, K, K, False, mode)
+    ob = gemm_pack(dB, N, K, N, True, mode)
+    dC = DeviceBuffer(M * N * 4)
+    gemm(mode, M, N, K, oa, ob, dC)
+    C = dC.download((M, N), np.float32)
+    if return_operands:
+        return C, download_operand(oa), download_operand(ob)
+    return C
+
+
+def download_operand(op: Operand):
+    """(hi bytes, lo bytes or None, q or None) of a packed operand."""
+    eb = elem_bytes(op.mode)
+    dt = {4: np.float32, 2: np.uint16, 1: np.uint8}[eb]
+    hi = op.hi.download((op.rows, op.k), dt)
+    lo = op.lo.download((op.rows, op.k), dt) if op.lo is not None else None
+    q = float(op.q.download((1,), np.float32)[0]) if op.q is not None else None
+    return hi, lo, q
+
+
+def conv_pack_input(src: DeviceBuffer, N: int, C: int, H: int, W: int, mode: int, stream=None) -> Operand:
+    op = Operand(N * H * W, C, mode)
+    if mode in FP8_MODES:
+        call("umi_amax_f32", src.ptr, N * C * H * W, op.amax.ptr, _stream(stream))
+    call("umi_conv_pack_input", src.ptr, N, C, H, W, mode, op.amax.ptr, op.hi.ptr, op.lo_ptr, op.q_ptr,
+         _stream(stream))
+    return op
+
+
+def conv_pack_filter(src: DeviceBuffer, K: int, C: int, R: int, S: int, mode: int, stream=None) -> Operand:
+    op = Operand(K, R * S * C, mode)
+    if mode in FP8_MODES:
+        call("umi_amax_f32", src.ptr, K * C * R * S, op.amax.ptr, _stream(stream))
+    call("umi_conv_pack_filter", src.ptr, K, C, R, S, mode, op.amax.ptr, op.hi.ptr, op.lo_ptr, op.q_ptr,
+         _stream(stream))
+    return op
+
+
+def conv_fprop(mode: int, N: int, C: int, H: int, W: int, K: int, R: int, S: int, stride: int, pad: int,
+               x: Operand, w: Operand, out: DeviceBuffer, stream=None) -> None:
+    call("umi_conv_fprop", mode, N, C, H, W, K, R, S, stride, pad, x.hi.ptr, x.lo_ptr, w.hi.ptr, w.lo_ptr, out.ptr,
+         x.q_ptr, w.q_ptr, _stream(stream))
+
+
+def conv_out_shape(N, K, H, W, R, S, stride, pad):
+    return N, K, (H + 2 * pad - R) // stride + 1, (W + 2 * pad - S) // stride + 1
+
+
+def conv_fprop_np(x: np.ndarray, w: np.ndarray, stride: int, pad: int, mode: int):
+    """NCHW fp32 input, KCRS fp32 filter -> NCHW fp32 output."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    N, C, H, W = x.shape
+    K, _, R, S = w.shape
+    shp = conv_out_shape(N, K, H, W, R, S, stride, pad)
+    dx, dw = DeviceBuffer.from_array(x), DeviceBuffer.from_array(w)
+    ox = conv_pack_input(dx, N, C, H, W, mode)
+    ow = conv_pack_filter(dw, K, C, R, S, mode)
+    out = DeviceBuffer(int(np.prod(shp)) * 4)
+    conv_fprop(mode, N, C, H, W, K, R, S, stride, pad, ox, ow, out)
+    return out.download(shp, np.float32)
