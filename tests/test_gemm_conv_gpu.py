@@ -1,9 +1,12 @@
 """MFMA GEMM and implicit-GEMM convolution on the GPU.
 
 Kernel correctness: the output equals the FP64 contraction of the operands
-the kernel actually received (decoded from the packed bf16 / e4m3 bytes,
-hi + lo for the split modes), to fp32 accumulation rounding (|err| <= 2e-6 of
-||a|| ||b||).  Precision: against the FP64 oracle of the unrounded inputs,
+the kernel actually received (decoded from the packed bf16 / e4m3 bytes; for
+the split modes the three products hi*hi + lo*hi + hi*lo, with the fp8 lo
+weight 2^-4), to accumulation rounding: 2e-6 of ||a|| ||b|| for the fp32
+and bf16 instructions, 5e-5 for the fp8 instruction, whose 64-term dot
+product is not summed in IEEE fp32 inside the matrix core (measured max
+2.2e-5, mean 1e-6).  Precision: against the FP64 oracle of the unrounded inputs,
 each mode meets its stated bound; the measured max / mean are printed.
 """
 import numpy as np
@@ -18,17 +21,32 @@ MODES = [ops.F32, ops.BF16, ops.BF16X3, ops.FP8, ops.FP8X3]
 PRECISION = {ops.F32: 1e-6, ops.BF16X3: 2e-6, ops.BF16: 1e-3, ops.FP8: 4e-2, ops.FP8X3: 4e-3}
 
 
-def dequant(op, mode):
+OWN_TOL = {ops.F32: 2e-6, ops.BF16: 2e-6, ops.BF16X3: 2e-6, ops.FP8: 5e-5, ops.FP8X3: 5e-5}
+
+
+def parts(op, mode):
+    """(hi, lo or None) as float64, already divided by q and with the lo weight applied."""
     hi, lo, q = op
     if mode == ops.F32:
-        return hi.astype(np.float64)
+        return hi.astype(np.float64), None
     if mode in (ops.BF16, ops.BF16X3):
-        v = oracle.decode_bf16(hi)
-        return v + oracle.decode_bf16(lo) if lo is not None else v
-    v = oracle.decode_e4m3(hi)
-    if lo is not None:
-        v = v + oracle.decode_e4m3(lo) / 16.0
-    return v / q
+        return oracle.decode_bf16(hi), (oracle.decode_bf16(lo) if lo is not None else None)
+    h = oracle.decode_e4m3(hi) / q
+    return h, (oracle.decode_e4m3(lo) / (16.0 * q) if lo is not None else None)
+
+
+def dequant(op, mode):
+    h, l = parts(op, mode)
+    return h if l is None else h + l
+
+
+def three_products(contract, a, b, mode):
+    """hi*hi (+ lo*hi + hi*lo): exactly the products the kernel forms."""
+    (ah, al), (bh, bl) = parts(a, mode), parts(b, mode)
+    out = contract(ah, bh)
+    if al is not None:
+        out = out + contract(al, bh) + contract(ah, bl)
+    return out
 
 
 def normed_err(C, ref, A, Bt):
@@ -45,11 +63,10 @@ def test_gemm_kernel_exact_on_its_operands(mode, M, N, K, gpu):
     B = g.standard_normal((K, N)).astype(np.float32)
     print(f"[gemm] {ops.MODE_NAMES[mode]} {M}x{N}x{K}", flush=True)
     C, oa, ob = ops.gemm_np(A, B, mode, return_operands=True)
-    Aq, Btq = dequant(oa, mode), dequant(ob, mode)
-    ref = Aq @ Btq.T
-    e = normed_err(C, ref, Aq, Btq)
+    ref = three_products(lambda a, b: a @ b.T, oa, ob, mode)
+    e = normed_err(C, ref, dequant(oa, mode), dequant(ob, mode))
     print(f"[gemm]   vs own operands {e}", flush=True)
-    assert e["max_rel"] <= 2e-6
+    assert e["max_rel"] <= OWN_TOL[mode]
 
 
 @pytest.mark.parametrize("mode", MODES, ids=[ops.MODE_NAMES[m] for m in MODES])
@@ -96,11 +113,13 @@ def test_conv_kernel_exact_on_its_operands(mode, shape, gpu):
     from umi_amd import DeviceBuffer
     ox = ops.conv_pack_input(DeviceBuffer.from_array(x), N, C, H, W, mode)
     ow = ops.conv_pack_filter(DeviceBuffer.from_array(w), K, C, R, R, mode)
-    xq = dequant(ops.download_operand(ox), mode).reshape(N, H, W, C).transpose(0, 3, 1, 2)
-    wq = dequant(ops.download_operand(ow), mode).reshape(K, R, R, C).transpose(0, 3, 1, 2)
-    refq = oracle.conv_f64(xq, wq, stride, pad)
+    to_x = lambda v: v.reshape(N, H, W, C).transpose(0, 3, 1, 2)
+    to_w = lambda v: v.reshape(K, R, R, C).transpose(0, 3, 1, 2)
+    refq = three_products(lambda a, b: oracle.conv_f64(to_x(b), to_w(a), stride, pad),
+                          ops.download_operand(ow), ops.download_operand(ox), mode)
+    xq, wq = to_x(dequant(ops.download_operand(ox), mode)), to_w(dequant(ops.download_operand(ow), mode))
     scale = np.sqrt(C * R * R) * np.abs(xq).max() * np.abs(wq).max()
-    assert np.abs(out - refq).max() <= 2e-6 * scale
+    assert np.abs(out - refq).max() <= OWN_TOL[mode] * scale
     assert np.abs(out - ref).max() <= max(PRECISION[mode] * 40, 1e-5) * scale
 
 

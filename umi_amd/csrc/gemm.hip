@@ -17,7 +17,8 @@
 //            instruction, so all three products share one accumulator.
 //
 // Design: 256 x 256 output tile per 256-thread workgroup (2 x 2 waves, each
-// 128 x 128 = 4 x 4 MFMA 32x32 blocks held in the accumulator registers);
+// 128 x 128 = 4 x 4 MFMA 32x32 blocks held in the accumulator registers;
+// 128 x 128 and 64 x 128 tiles for smaller or narrow problems);
 // K is walked in stages of 64 bytes per operand row (32 bf16, 64 fp8,
 // 16 f32), double-buffered in LDS (row stride 80 B so ds_read_b128 of 32
 // consecutive rows spreads over the banks), global loads for stage s+1
@@ -40,10 +41,8 @@ typedef __attribute__((ext_vector_type(4))) int i32x4;
 
 enum GemmMode { GM_F32 = 0, GM_BF16 = 1, GM_BF16X3 = 2, GM_FP8 = 3, GM_FP8X3 = 4 };
 
-constexpr int BM = 256, BN = 256;
 constexpr int ROWB = 64;          // bytes of one operand row per stage
 constexpr int LDSROW = ROWB + 16; // padded LDS row stride
-constexpr int PANEL = BM * LDSROW;  // bytes of one operand panel per stage (BM == BN)
 constexpr int NTHREADS = 256;
 
 template <int MODE>
@@ -135,14 +134,30 @@ __device__ inline i32x4 fetch16(const uint8_t *p) {
     return p ? *(const i32x4 *)p : i32x4{0, 0, 0, 0};
 }
 
-template <int MODE, class ASrc, class BSrc, class Dst>
+// Workgroup tile: WGM x WGN waves, each wave TM x TN MFMA blocks of 32 x 32.
+template <int WGM_, int WGN_, int TM_, int TN_>
+struct Tile {
+    static constexpr int WGM = WGM_, WGN = WGN_, TM = TM_, TN = TN_;
+    static constexpr int BM = WGM * TM * 32, BN = WGN * TN * 32;
+    static_assert(WGM * WGN * 64 == NTHREADS, "four waves");
+};
+using TileGemm = Tile<2, 2, 4, 4>;   // 256 x 256: large GEMMs
+using TileMid = Tile<2, 2, 2, 2>;    // 128 x 128
+using TileNarrow = Tile<1, 4, 2, 1>; // 64 x 128: few output rows (conv with K <= 64)
+
+template <class TL, int MODE, class ASrc, class BSrc, class Dst>
 __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BSrc b_hi, BSrc b_lo, Dst dst, int K,
                                                         int tiles_m, int tiles_n, const float *qa, const float *qb) {
     using OP = Mfma<MODE>;
-    constexpr int NOP = OP::NOP;  // operand panels per stage: A, B (+ A lo, B lo)
+    constexpr bool SPLIT = OP::NOP == 4;
+    constexpr int BMt = TL::BM, BNt = TL::BN, TM = TL::TM, TN = TL::TN;
+    constexpr int RA = BMt / 64, RB = BNt / 64;          // 16-B chunks per thread per panel
+    constexpr int OFF_B = BMt * LDSROW;                   // panel offsets inside one stage
+    constexpr int OFF_LO = (BMt + BNt) * LDSROW;
+    constexpr int STAGE = (SPLIT ? 2 : 1) * (BMt + BNt) * LDSROW;
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = wv >> 1, wn = wv & 1;
+    const int wm = wv / TL::WGN, wn = wv % TL::WGN;
 
     // XCD-aware tile order: ids on one XCD (id % 8) take consecutive tiles.
     const int ntiles = tiles_m * tiles_n;
@@ -154,40 +169,43 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
     const int gsz = min(GROUP, tiles_m - first_m);
     const int tm = first_m + (pid % (GROUP * tiles_n)) % gsz;
     const int tn = (pid % (GROUP * tiles_n)) / gsz;
-    const int m0 = tm * BM, n0 = tn * BN;
+    const int m0 = tm * BMt, n0 = tn * BNt;
 
-    const int kbytes = K * OP::ESZ;
-    const int nstages = kbytes / ROWB;
+    const int nstages = K * OP::ESZ / ROWB;
     // global -> LDS assignment: thread t copies 16 B chunk (t & 3) of rows t/4 + 64 i
     const int lrow = tid >> 2, lchunk = (tid & 3) * 16;
-    i32x4 stage_regs[NOP][4];
+    i32x4 ra[SPLIT ? 2 : 1][RA], rb[SPLIT ? 2 : 1][RB];
     auto gload = [&](int st) {
         const int kb = st * ROWB + lchunk;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = lrow + 64 * i;
-            stage_regs[0][i] = fetch16(a_hi.row(m0 + r, kb, OP::ESZ));
-            stage_regs[1][i] = fetch16(b_hi.row(n0 + r, kb, OP::ESZ));
-            if constexpr (NOP == 4) {
-                stage_regs[2][i] = fetch16(a_lo.row(m0 + r, kb, OP::ESZ));
-                stage_regs[3][i] = fetch16(b_lo.row(n0 + r, kb, OP::ESZ));
-            }
+        for (int i = 0; i < RA; ++i) {
+            ra[0][i] = fetch16(a_hi.row(m0 + lrow + 64 * i, kb, OP::ESZ));
+            if constexpr (SPLIT) ra[1][i] = fetch16(a_lo.row(m0 + lrow + 64 * i, kb, OP::ESZ));
+        }
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            rb[0][i] = fetch16(b_hi.row(n0 + lrow + 64 * i, kb, OP::ESZ));
+            if constexpr (SPLIT) rb[1][i] = fetch16(b_lo.row(n0 + lrow + 64 * i, kb, OP::ESZ));
         }
     };
     auto sstore = [&](int buf) {
-        uint8_t *base = smem + buf * NOP * PANEL;
+        uint8_t *base = smem + buf * STAGE;
 #pragma unroll
-        for (int o = 0; o < NOP; ++o)
+        for (int h = 0; h < (SPLIT ? 2 : 1); ++h) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                *(i32x4 *)(base + o * PANEL + (lrow + 64 * i) * LDSROW + lchunk) = stage_regs[o][i];
+            for (int i = 0; i < RA; ++i)
+                *(i32x4 *)(base + h * OFF_LO + (lrow + 64 * i) * LDSROW + lchunk) = ra[h][i];
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+                *(i32x4 *)(base + h * OFF_LO + OFF_B + (lrow + 64 * i) * LDSROW + lchunk) = rb[h][i];
+        }
     };
 
-    f32x16 acc[4][4];
+    f32x16 acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
@@ -202,23 +220,23 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
     for (int st = 0; st < nstages; ++st) {
         const int buf = st & 1;
         if (st + 1 < nstages) gload(st + 1);
-        const uint8_t *base = smem + buf * NOP * PANEL;
-        const uint8_t *pa = base + (wm * 128 + lr) * LDSROW + lk;
-        const uint8_t *pb = base + PANEL + (wn * 128 + lr) * LDSROW + lk;
+        const uint8_t *base = smem + buf * STAGE;
+        const uint8_t *pa = base + (wm * TM * 32 + lr) * LDSROW + lk;
+        const uint8_t *pb = base + OFF_B + (wn * TN * 32 + lr) * LDSROW + lk;
 #pragma unroll
         for (int kc = 0; kc < OP::CH; ++kc) {
             const int ko = kc * 2 * OP::LANEB;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < TN; ++j) {
                     const uint8_t *ai = pa + i * 32 * LDSROW + ko;
                     const uint8_t *bj = pb + j * 32 * LDSROW + ko;
                     acc[i][j] = OP::run(ai, bj, acc[i][j], UNIT, UNIT);
-                    if constexpr (NOP == 4) {
+                    if constexpr (SPLIT) {
                         constexpr int S1 = (MODE == GM_FP8X3) ? LO : UNIT;
-                        acc[i][j] = OP::run(ai + 2 * PANEL, bj, acc[i][j], S1, UNIT);      // A lo * B hi
-                        acc[i][j] = OP::run(ai, bj + 2 * PANEL, acc[i][j], UNIT, S1);      // A hi * B lo
+                        acc[i][j] = OP::run(ai + OFF_LO, bj, acc[i][j], S1, UNIT);  // A lo * B hi
+                        acc[i][j] = OP::run(ai, bj + OFF_LO, acc[i][j], UNIT, S1);  // A hi * B lo
                     }
                 }
         }
@@ -231,24 +249,24 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
 
     const float alpha = (qa ? 1.0f / *qa : 1.0f) * (qb ? 1.0f / *qb : 1.0f);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int n = n0 + wn * 128 + j * 32 + lr;
+                const int m = m0 + wm * TM * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = n0 + wn * TN * 32 + j * 32 + lr;
                 dst.put(m, n, acc[i][j][r] * alpha);
             }
 }
 
-template <int MODE, class ASrc, class BSrc, class Dst>
-umi_status gemm_launch(ASrc ah, ASrc al, BSrc bh, BSrc bl, Dst d, int M, int N, int K, const float *qa,
-                       const float *qb, hipStream_t st) {
+template <class TL, int MODE, class ASrc, class BSrc, class Dst>
+umi_status gemm_launch_t(ASrc ah, ASrc al, BSrc bh, BSrc bl, Dst d, int M, int N, int K, const float *qa,
+                         const float *qb, hipStream_t st) {
     using OP = Mfma<MODE>;
-    const int tm = ceil_div(M, BM), tn = ceil_div(N, BN);
-    const size_t lds = (size_t)2 * OP::NOP * PANEL;
-    auto kern = gemm_kernel<MODE, ASrc, BSrc, Dst>;
+    const int tm = ceil_div(M, TL::BM), tn = ceil_div(N, TL::BN);
+    const size_t lds = (size_t)2 * (OP::NOP / 2) * (TL::BM + TL::BN) * LDSROW;
+    auto kern = gemm_kernel<TL, MODE, ASrc, BSrc, Dst>;
     static bool attr_set = false;
     if (!attr_set) {
         UMI_TRY(hip_check(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -257,6 +275,17 @@ umi_status gemm_launch(ASrc ah, ASrc al, BSrc bh, BSrc bl, Dst d, int M, int N, 
     }
     hipLaunchKernelGGL(kern, dim3(tm * tn), dim3(NTHREADS), lds, st, ah, al, bh, bl, d, K, tm, tn, qa, qb);
     return launch_check("gemm_kernel");
+}
+
+// Tile choice: the 256 x 256 tile when it yields at least one workgroup per
+// CU, otherwise the 128 x 128 one; the 64 x 128 tile when M <= 64.
+template <int MODE, class ASrc, class BSrc, class Dst>
+umi_status gemm_launch(ASrc ah, ASrc al, BSrc bh, BSrc bl, Dst d, int M, int N, int K, const float *qa,
+                       const float *qb, hipStream_t st) {
+    if (M <= 64) return gemm_launch_t<TileNarrow, MODE>(ah, al, bh, bl, d, M, N, K, qa, qb, st);
+    if ((long)ceil_div(M, 256) * ceil_div(N, 256) >= 256)
+        return gemm_launch_t<TileGemm, MODE>(ah, al, bh, bl, d, M, N, K, qa, qb, st);
+    return gemm_launch_t<TileMid, MODE>(ah, al, bh, bl, d, M, N, K, qa, qb, st);
 }
 
 int esize(int mode) { return mode == GM_F32 ? 4 : (mode <= GM_BF16X3 ? 2 : 1); }
