@@ -222,6 +222,34 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
                           const void *in_hi, const void *in_lo, const void *w_hi, const void *w_lo, float *out,
                           const float *q_in, const float *q_w, umi_stream s);
 
+/* ---------------------------------------------------------------------------
+ * Batches of images of uneven sizes.  The reference runs one Execute per
+ * image (src/UMI_gpu.h:409-623, e.g. main/wfilter3x3.cu:39,
+ * main/bilateral.cu:54, main/iimg.cu:37,60); here a batch object describes
+ * n images inside one input and one output buffer (element offsets and
+ * pitches), and each batched operator is one persistent launch over the
+ * tiles of all images (the integral image: three launches, band / carry
+ * scan / carry add).  Creation uploads the work tables once; the operator
+ * calls allocate nothing and can be captured in a graph.  `config` selects
+ * a tile shape (0 .. umi_batch_configs(op)-1, -1 = default); op 0 = 3x3
+ * filter, 1 = bilateral.
+ * ------------------------------------------------------------------------- */
+typedef struct umi_batch_s *umi_batch;
+umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in_off, const int *in_pitch,
+                            const int64_t *out_off, const int *out_pitch, umi_batch *b);
+umi_status umi_batch_destroy(umi_batch b);
+umi_status umi_batch_configs(int op, int *count);
+umi_status umi_batch_wfilter3x3_f32(umi_batch b, const float *in, const float *weights, float init, float pad,
+                                    int mode, float *out, int config, umi_stream s);
+umi_status umi_batch_bilateral_f32(umi_batch b, const float *in, int radius, const float *spatial, float norm,
+                                   float pad, int mode, float *out, int config, umi_stream s);
+/* kind 0: uint8 -> uint32, kind 2: float -> float (fp64 column pass) */
+umi_status umi_batch_iimg_workspace_size(umi_batch b, int kind, size_t *bytes);
+umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, void *workspace,
+                                 size_t workspace_bytes, umi_stream s);
+umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
+                              umi_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,3 +107,22 @@ def test_validate_execute_rejects_bad_views_and_sizes():
     assert _validate(GOOD_P, GOOD_A, planes=[5], n_in=1) == 1
     assert _validate(GOOD_P, GOOD_A, expand=0x1101) == 1
     assert _validate(GOOD_P, [[1, 1], [1, 0], [3, 3], [3, 3]]) == 1
+
+
+def test_batch_create_validates_on_host():
+    import ctypes
+
+    import numpy as np
+
+    from umi_amd._lib import lib
+
+    H = np.array([10, 0], np.int32)
+    W = np.array([10, 10], np.int32)
+    off = np.array([0, 100], np.int64)
+    h = ctypes.c_void_p()
+    assert lib().umi_batch_create(2, H.ctypes.data, W.ctypes.data, off.ctypes.data, W.ctypes.data, off.ctypes.data,
+                                  W.ctypes.data, ctypes.byref(h)) == 1
+    assert b"image 1" in lib().umi_last_error()
+    n = ctypes.c_int()
+    assert lib().umi_batch_configs(0, ctypes.byref(n)) == 0 and n.value >= 1
+    assert lib().umi_batch_configs(7, ctypes.byref(n)) == 1

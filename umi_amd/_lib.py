@@ -60,6 +60,14 @@ SIGNATURES = {
     "umi_gemm": [c_int, c_int, c_int, c_int, P, P, c_int, P, P, c_int, P, c_int, P, P, P],
     "umi_conv_pack_input": [P, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P],
     "umi_conv_pack_filter": [P, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P],
+    "umi_batch_create": [c_int, P, P, P, P, P, P, C.POINTER(c_void_p)],
+    "umi_batch_destroy": [P],
+    "umi_batch_configs": [c_int, C.POINTER(c_int)],
+    "umi_batch_wfilter3x3_f32": [P, P, P, c_float, c_float, c_int, P, c_int, P],
+    "umi_batch_bilateral_f32": [P, P, c_int, P, c_float, c_float, c_int, P, c_int, P],
+    "umi_batch_iimg_workspace_size": [P, c_int, C.POINTER(c_size_t)],
+    "umi_batch_iimg_u8_u32": [P, P, P, P, c_size_t, P],
+    "umi_batch_iimg_f32": [P, P, P, P, c_size_t, P],
     "umi_conv_fprop": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P,
                        P],
 }

@@ -1,0 +1,339 @@
+// Batches of images of uneven sizes, one persistent launch per operator.
+//
+// The reference launches one UMI kernel per image (one Execute per call,
+// src/UMI_gpu.h:409-623; main/wfilter3x3.cu:39, main/bilateral.cu:54,
+// main/iimg.cu:37,60, johnjohnlin/UMI).  Here a batch object holds, per
+// image, its size and its offsets into one input and one output buffer, plus
+// prefix sums of the work items (tiles, bands, columns, rows) of every image
+// for every tile configuration, all resident in device memory.  A batched
+// operator is then one persistent kernel whose workgroups stride over the
+// concatenated work list, find their image by binary search over the
+// prefix array and run the same tile body as the one-image kernel.  No
+// launch allocates or copies, so the launches can be captured in a hipGraph.
+// Tile configurations are selectable so a tuner can pick the fastest.
+#include "common.h"
+#include "iimg_dev.h"
+#include "stencil_dev.h"
+
+#include <vector>
+
+namespace {
+using namespace umi;
+
+struct BatchImage {
+    long long in_off, out_off;  // element offsets from the base pointers
+    int H, W, in_pitch, out_pitch;
+};
+
+// 3x3 filter tile shapes (TX, TY) and bilateral tile shapes (TX x TY = 256).
+constexpr int NF = 4, NB = 4;
+constexpr int FTX[NF] = {64, 64, 128, 32}, FTY[NF] = {32, 16, 16, 32};
+constexpr int BTX[NB] = {32, 16, 64, 128}, BTY[NB] = {8, 16, 4, 2};
+
+constexpr int PERSIST_WG = 256 * 8;  // workgroups of a persistent launch (8 per CU)
+
+// index i with prefix[i] <= t < prefix[i+1]
+__device__ inline int find_item(const int *__restrict__ prefix, int n, int t) {
+    int lo = 0, hi = n;  // prefix[lo] <= t < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void batch_wfilter_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                            const int *__restrict__ prefix, const float *__restrict__ in,
+                                                            const float *__restrict__ wts, float init, float pad,
+                                                            int mode, float *__restrict__ out) {
+    __shared__ float tile[(TY + 2) * (TX + 2)];
+    float w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w[k] = wts[k];
+    const int total = prefix[n];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(prefix, n, t);
+        const BatchImage im = imgs[i];
+        const int local = t - prefix[i], tiles_x = ceil_div(im.W, TX);
+        const int ty = local / tiles_x, tx = local - ty * tiles_x;
+        filter3x3_tile<float, true, TX, TY>(tile, in + im.in_off, im.H, im.W, im.in_pitch, w, init, 0.f, pad, mode,
+                                            out + im.out_off, im.out_pitch, tx * TX, ty * TY);
+    }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                              const int *__restrict__ prefix,
+                                                              const float *__restrict__ in, int r,
+                                                              const float *__restrict__ ws, float norm, float pad,
+                                                              int mode, float *__restrict__ out) {
+    extern __shared__ float bl_lds[];
+    float *wsh = bl_lds;
+    if ((int)threadIdx.x < 2 * r + 1) wsh[threadIdx.x] = ws[threadIdx.x];
+    __syncthreads();
+    const int total = prefix[n];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(prefix, n, t);
+        const BatchImage im = imgs[i];
+        const int local = t - prefix[i], tiles_x = ceil_div(im.W, TX);
+        const int ty = local / tiles_x, tx = local - ty * tiles_x;
+        bilateral_tile<TX, TY>(bl_lds + 32, wsh, in + im.in_off, im.H, im.W, im.in_pitch, r, norm, pad, mode,
+                               out + im.out_off, im.out_pitch, tx * TX, ty * TY);
+    }
+}
+
+// Integral image, phase 1: one workgroup per band of one image.
+template <class T>
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_band_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                                     const int *__restrict__ band_prefix,
+                                                                     const long long *__restrict__ ws_off,
+                                                                     const typename T::In *__restrict__ in,
+                                                                     typename T::Out *__restrict__ out,
+                                                                     typename T::Col *__restrict__ ws) {
+    extern __shared__ uint32_t im_lds[];
+    const int total = band_prefix[n];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(band_prefix, n, t);
+        const BatchImage im = imgs[i];
+        const IimgPlan p = iimg_plan(im.H, im.W);
+        iimg_band<T>(reinterpret_cast<typename T::Row *>(im_lds), in + im.in_off, im.H, im.W, im.in_pitch, p,
+                     t - band_prefix[i], out + im.out_off, im.out_pitch, ws + ws_off[i]);
+    }
+}
+
+// Phase 2: one thread per column of every image with more than one band.
+template <class T>
+__global__ void batch_iimg_scan_kernel(const BatchImage *__restrict__ imgs, int n, const int *__restrict__ col_prefix,
+                                       const long long *__restrict__ ws_off, typename T::Col *__restrict__ ws) {
+    const int total = col_prefix[n];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int i = find_item(col_prefix, n, t);
+        const BatchImage im = imgs[i];
+        const IimgPlan p = iimg_plan(im.H, im.W);
+        iimg_carry_scan<T>(im.W, p.nb, t - col_prefix[i], ws + ws_off[i]);
+    }
+}
+
+// Phase 3: one workgroup per row (at or below the second band) of every image.
+template <class T>
+__global__ __launch_bounds__(256) void batch_iimg_add_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                             const int *__restrict__ row_prefix,
+                                                             const long long *__restrict__ ws_off,
+                                                             typename T::Out *__restrict__ out,
+                                                             const typename T::Col *__restrict__ ws) {
+    const int total = row_prefix[n];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = find_item(row_prefix, n, t);
+        const BatchImage im = imgs[i];
+        const IimgPlan p = iimg_plan(im.H, im.W);
+        const int y = p.TB + (t - row_prefix[i]);
+        for (int x = threadIdx.x; x < im.W; x += blockDim.x)
+            iimg_carry_add<T>(im.W, p.TB, y, x, ws + ws_off[i], out + im.out_off, im.out_pitch);
+    }
+}
+
+}  // namespace
+
+struct umi_batch_s {
+    int n = 0;
+    int max_lds_iimg = 0;
+    std::vector<BatchImage> host;
+    BatchImage *imgs = nullptr;
+    // device prefix arrays (n + 1 ints each) and their totals
+    int *ftiles[NF] = {}, *btiles[NB] = {};
+    int ftotal[NF] = {}, btotal[NB] = {};
+    int *bands = nullptr, *cols = nullptr, *rows = nullptr;
+    int nbands = 0, ncols = 0, nrows = 0;
+    long long *ws_off = nullptr;
+    long long ws_elems = 0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+template <class V>
+umi_status upload(umi_batch_s *b, const std::vector<V> &v, V **dst) {
+    void *p = nullptr;
+    UMI_TRY(hip_check(hipMalloc(&p, sizeof(V) * v.size()), "hipMalloc (batch tables)"));
+    b->allocs.push_back(p);
+    UMI_TRY(hip_check(hipMemcpy(p, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice), "hipMemcpy (batch tables)"));
+    *dst = static_cast<V *>(p);
+    return UMI_OK;
+}
+
+umi_status batch_check(umi_batch b, const void *in, const void *out, const char *who) {
+    if (!b) return set_error(UMI_ERR_INVALID, "%s: null batch", who);
+    if (!in || !out) return set_error(UMI_ERR_INVALID, "%s: null buffer", who);
+    return UMI_OK;
+}
+
+int grid_for(int items) { return items < PERSIST_WG ? (items > 0 ? items : 1) : PERSIST_WG; }
+
+template <int C>
+umi_status launch_wfilter(umi_batch b, const float *in, const float *w, float init, float pad, int mode, float *out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL((batch_wfilter_kernel<FTX[C], FTY[C]>), dim3(grid_for(b->ftotal[C])), dim3(256), 0, st, b->imgs,
+                       b->n, b->ftiles[C], in, w, init, pad, mode, out);
+    return launch_check("batch_wfilter_kernel");
+}
+
+template <int C>
+umi_status launch_bilateral(umi_batch b, const float *in, int r, const float *ws, float norm, float pad, int mode,
+                            float *out, hipStream_t st) {
+    const size_t lds = sizeof(float) * ((size_t)(BTX[C] + 2 * r) * (BTY[C] + 2 * r) + 32);
+    hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C]>), dim3(grid_for(b->btotal[C])), dim3(256), lds, st,
+                       b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
+    return launch_check("batch_bilateral_kernel");
+}
+
+template <class T>
+umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *out, void *ws, size_t ws_bytes,
+                       hipStream_t st, const char *who) {
+    UMI_TRY(batch_check(b, in, out, who));
+    const size_t need = (size_t)b->ws_elems * sizeof(typename T::Col);
+    if (need && (!ws || ws_bytes < need)) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
+    auto *w = static_cast<typename T::Col *>(ws);
+    hipLaunchKernelGGL(batch_iimg_band_kernel<T>, dim3(grid_for(b->nbands)), dim3(IM_THREADS), (size_t)b->max_lds_iimg,
+                       st, b->imgs, b->n, b->bands, b->ws_off, in, out, w);
+    UMI_TRY(launch_check("batch_iimg_band_kernel"));
+    if (b->ncols > 0) {
+        hipLaunchKernelGGL(batch_iimg_scan_kernel<T>, dim3(grid_for(ceil_div(b->ncols, 256))), dim3(256), 0, st,
+                           b->imgs, b->n, b->cols, b->ws_off, w);
+        UMI_TRY(launch_check("batch_iimg_scan_kernel"));
+        hipLaunchKernelGGL(batch_iimg_add_kernel<T>, dim3(grid_for(b->nrows)), dim3(256), 0, st, b->imgs, b->n,
+                           b->rows, b->ws_off, out, w);
+        UMI_TRY(launch_check("batch_iimg_add_kernel"));
+    }
+    return UMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in_off, const int *in_pitch,
+                            const int64_t *out_off, const int *out_pitch, umi_batch *out) {
+    if (!out) return set_error(UMI_ERR_INVALID, "umi_batch_create: null result pointer");
+    *out = nullptr;
+    if (n < 1 || !H || !W || !in_off || !in_pitch || !out_off || !out_pitch)
+        return set_error(UMI_ERR_INVALID, "umi_batch_create: n=%d or null arrays", n);
+    auto *b = new umi_batch_s;
+    b->n = n;
+    std::vector<int> fp[NF], bp[NB], bands(n + 1, 0), cols(n + 1, 0), rows(n + 1, 0);
+    for (int c = 0; c < NF; ++c) fp[c].assign(n + 1, 0);
+    for (int c = 0; c < NB; ++c) bp[c].assign(n + 1, 0);
+    std::vector<long long> wso(n, 0);
+    long long ws = 0;
+    for (int i = 0; i < n; ++i) {
+        if (H[i] < 1 || W[i] < 1 || in_pitch[i] < W[i] || out_pitch[i] < W[i] || in_off[i] < 0 || out_off[i] < 0) {
+            delete b;
+            return set_error(UMI_ERR_INVALID, "umi_batch_create: image %d: H=%d W=%d pitches %d %d", i, H[i], W[i],
+                             in_pitch[i], out_pitch[i]);
+        }
+        const IimgPlan p = iimg_plan(H[i], W[i]);
+        if (p.TB < 1) {
+            delete b;
+            return set_error(UMI_ERR_UNSUPPORTED, "umi_batch_create: image %d width %d", i, W[i]);
+        }
+        b->host.push_back({in_off[i], out_off[i], H[i], W[i], in_pitch[i], out_pitch[i]});
+        for (int c = 0; c < NF; ++c) fp[c][i + 1] = fp[c][i] + ceil_div(W[i], FTX[c]) * ceil_div(H[i], FTY[c]);
+        for (int c = 0; c < NB; ++c) bp[c][i + 1] = bp[c][i] + ceil_div(W[i], BTX[c]) * ceil_div(H[i], BTY[c]);
+        bands[i + 1] = bands[i] + p.nb;
+        cols[i + 1] = cols[i] + (p.nb > 1 ? W[i] : 0);
+        rows[i + 1] = rows[i] + (p.nb > 1 ? H[i] - p.TB : 0);
+        wso[i] = ws;
+        ws += (long long)p.nb * W[i];
+        b->max_lds_iimg = std::max(b->max_lds_iimg, 4 * p.LS * p.TB);
+    }
+    umi_status st = upload(b, b->host, &b->imgs);
+    for (int c = 0; c < NF && st == UMI_OK; ++c) {
+        st = upload(b, fp[c], &b->ftiles[c]);
+        b->ftotal[c] = fp[c][n];
+    }
+    for (int c = 0; c < NB && st == UMI_OK; ++c) {
+        st = upload(b, bp[c], &b->btiles[c]);
+        b->btotal[c] = bp[c][n];
+    }
+    if (st == UMI_OK) st = upload(b, bands, &b->bands);
+    if (st == UMI_OK) st = upload(b, cols, &b->cols);
+    if (st == UMI_OK) st = upload(b, rows, &b->rows);
+    if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
+    b->nbands = bands[n];
+    b->ncols = cols[n];
+    b->nrows = rows[n];
+    b->ws_elems = ws;
+    if (st != UMI_OK) {
+        for (void *p : b->allocs) hipFree(p);
+        delete b;
+        return st;
+    }
+    *out = b;
+    return UMI_OK;
+}
+
+umi_status umi_batch_destroy(umi_batch b) {
+    if (!b) return UMI_OK;
+    for (void *p : b->allocs) hipFree(p);
+    delete b;
+    return UMI_OK;
+}
+
+umi_status umi_batch_configs(int op, int *count) {
+    if (!count) return set_error(UMI_ERR_INVALID, "umi_batch_configs: null pointer");
+    if (op == 0) *count = NF;
+    else if (op == 1) *count = NB;
+    else return set_error(UMI_ERR_INVALID, "umi_batch_configs: op %d", op);
+    return UMI_OK;
+}
+
+umi_status umi_batch_wfilter3x3_f32(umi_batch b, const float *in, const float *weights, float init, float pad,
+                                    int mode, float *out, int config, umi_stream s) {
+    UMI_TRY(batch_check(b, in, out, "umi_batch_wfilter3x3_f32"));
+    if (!weights) return set_error(UMI_ERR_INVALID, "umi_batch_wfilter3x3_f32: null weights");
+    if (mode != 0 && mode != 1) return set_error(UMI_ERR_INVALID, "umi_batch_wfilter3x3_f32: mode %d", mode);
+    hipStream_t st = as_stream(s);
+    switch (config < 0 ? 0 : config) {
+        case 0: return launch_wfilter<0>(b, in, weights, init, pad, mode, out, st);
+        case 1: return launch_wfilter<1>(b, in, weights, init, pad, mode, out, st);
+        case 2: return launch_wfilter<2>(b, in, weights, init, pad, mode, out, st);
+        case 3: return launch_wfilter<3>(b, in, weights, init, pad, mode, out, st);
+        default: return set_error(UMI_ERR_INVALID, "umi_batch_wfilter3x3_f32: config %d", config);
+    }
+}
+
+umi_status umi_batch_bilateral_f32(umi_batch b, const float *in, int radius, const float *spatial, float norm,
+                                   float pad, int mode, float *out, int config, umi_stream s) {
+    UMI_TRY(batch_check(b, in, out, "umi_batch_bilateral_f32"));
+    if (!spatial) return set_error(UMI_ERR_INVALID, "umi_batch_bilateral_f32: null weights");
+    if (radius < 0 || radius > BL_MAXR) return set_error(UMI_ERR_UNSUPPORTED, "umi_batch_bilateral_f32: radius %d", radius);
+    if (mode != 0 && mode != 1) return set_error(UMI_ERR_INVALID, "umi_batch_bilateral_f32: mode %d", mode);
+    hipStream_t st = as_stream(s);
+    switch (config < 0 ? 0 : config) {
+        case 0: return launch_bilateral<0>(b, in, radius, spatial, norm, pad, mode, out, st);
+        case 1: return launch_bilateral<1>(b, in, radius, spatial, norm, pad, mode, out, st);
+        case 2: return launch_bilateral<2>(b, in, radius, spatial, norm, pad, mode, out, st);
+        case 3: return launch_bilateral<3>(b, in, radius, spatial, norm, pad, mode, out, st);
+        default: return set_error(UMI_ERR_INVALID, "umi_batch_bilateral_f32: config %d", config);
+    }
+}
+
+umi_status umi_batch_iimg_workspace_size(umi_batch b, int kind, size_t *bytes) {
+    if (!b || !bytes) return set_error(UMI_ERR_INVALID, "umi_batch_iimg_workspace_size: null pointer");
+    if (kind != 0 && kind != 2) return set_error(UMI_ERR_INVALID, "umi_batch_iimg_workspace_size: kind %d", kind);
+    *bytes = (size_t)b->ws_elems * (kind == 0 ? sizeof(uint32_t) : sizeof(double));
+    return UMI_OK;
+}
+
+umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, void *workspace,
+                                 size_t workspace_bytes, umi_stream s) {
+    return launch_iimg<IU8>(b, in, out, workspace, workspace_bytes, as_stream(s), "umi_batch_iimg_u8_u32");
+}
+
+umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
+                              umi_stream s) {
+    return launch_iimg<IF32>(b, in, out, workspace, workspace_bytes, as_stream(s), "umi_batch_iimg_f32");
+}
+
+}  // extern "C"

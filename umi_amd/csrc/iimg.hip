@@ -19,92 +19,23 @@
 // LDS rows use the padded index e + e / C (C = per-lane chunk) so the
 // chunk-strided accesses of a wave fall in distinct banks.
 #include "common.h"
-#include "wave.h"
+#include "iimg_dev.h"
 
 namespace {
 using namespace umi;
 
-template <typename Tin, typename Trow, typename Tcol, typename Tout>
-struct IimgTypes {
-    using In = Tin;
-    using Row = Trow;
-    using Col = Tcol;
-    using Out = Tout;
-};
-using IU8 = IimgTypes<uint8_t, uint32_t, uint32_t, uint32_t>;
-using II32 = IimgTypes<int32_t, uint32_t, uint32_t, int32_t>;
-using IF32 = IimgTypes<float, float, double, float>;
-
-constexpr int IM_THREADS = 256;
-constexpr int IM_LDS = 64 * 1024;
-
-struct IimgPlan {
-    int C, LS, TB, nb;
-};
-inline IimgPlan iimg_plan(int H, int W) {
-    IimgPlan p;
-    p.C = ceil_div(W, 64);
-    p.LS = W + ceil_div(W, p.C) + 1;
-    p.TB = IM_LDS / (4 * p.LS);
-    if (p.TB > 32) p.TB = 32;
-    p.nb = p.TB > 0 ? ceil_div(H, p.TB) : 0;
-    return p;
-}
-
 template <class T>
 __global__ __launch_bounds__(IM_THREADS) void iimg_band_kernel(const typename T::In *__restrict__ in, int H, int W,
-                                                               int ip, int C, int LS, int TB,
-                                                               typename T::Out *__restrict__ out, int op,
-                                                               typename T::Col *__restrict__ totals) {
-    using Row = typename T::Row;
-    using Col = typename T::Col;
+                                                               int ip, IimgPlan p, typename T::Out *__restrict__ out,
+                                                               int op, typename T::Col *__restrict__ totals) {
     extern __shared__ uint32_t im_lds[];
-    Row *rows = reinterpret_cast<Row *>(im_lds);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int b = blockIdx.x;
-    const int y0 = b * TB;
-    const int nr = min(TB, H - y0);
-    // coalesced load of the band, converted to the row type
-    for (int r = 0; r < nr; ++r) {
-        const typename T::In *src = in + (size_t)(y0 + r) * ip;
-        for (int x = tid; x < W; x += IM_THREADS) rows[r * LS + x + x / C] = (Row)src[x];
-    }
-    __syncthreads();
-    // row scans: one wave per row
-    for (int r = wv; r < nr; r += IM_THREADS / 64) {
-        Row *row = rows + r * LS + lane * (C + 1);
-        const int n = min(C, W - lane * C);
-        Row run = 0;
-        for (int k = 0; k < n; ++k) {
-            run = run + row[k];
-            row[k] = run;
-        }
-        const Row incl = wave_inclusive_scan(run, [](Row a, Row c) { return a + c; });
-        const Row excl = incl - run;
-        for (int k = 0; k < n; ++k) row[k] = row[k] + excl;
-    }
-    __syncthreads();
-    // column pass within the band
-    for (int x = tid; x < W; x += IM_THREADS) {
-        Col acc = 0;
-        for (int r = 0; r < nr; ++r) {
-            acc = acc + (Col)rows[r * LS + x + x / C];
-            out[(size_t)(y0 + r) * op + x] = (typename T::Out)acc;
-        }
-        totals[(size_t)b * W + x] = acc;
-    }
+    iimg_band<T>(reinterpret_cast<typename T::Row *>(im_lds), in, H, W, ip, p, blockIdx.x, out, op, totals);
 }
 
 template <class T>
 __global__ void iimg_carry_scan_kernel(int W, int nb, typename T::Col *__restrict__ totals) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= W) return;
-    typename T::Col run = 0;
-    for (int b = 0; b < nb; ++b) {
-        const typename T::Col t = totals[(size_t)b * W + x];
-        totals[(size_t)b * W + x] = run;
-        run = run + t;
-    }
+    if (x < W) iimg_carry_scan<T>(W, nb, x, totals);
 }
 
 template <class T>
@@ -112,10 +43,7 @@ __global__ void iimg_carry_add_kernel(int H, int W, int TB, const typename T::Co
                                       typename T::Out *__restrict__ out, int op) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = TB + blockIdx.y;  // band 0 has no carry
-    if (x >= W || y >= H) return;
-    const typename T::Col c = carry[(size_t)(y / TB) * W + x];
-    typename T::Out *o = out + (size_t)y * op + x;
-    *o = (typename T::Out)((typename T::Col)*o + c);
+    if (x < W && y < H) iimg_carry_add<T>(W, TB, y, x, carry, out, op);
 }
 
 template <class T>
@@ -145,7 +73,7 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     }
     auto *tot = static_cast<typename T::Col *>(tmp);
     hipLaunchKernelGGL(iimg_band_kernel<T>, dim3(p.nb), dim3(IM_THREADS), (size_t)4 * p.LS * p.TB, st, in, H, W,
-                       ip, p.C, p.LS, p.TB, out, op, tot);
+                       ip, p, out, op, tot);
     umi_status rc = launch_check(name);
     if (rc == UMI_OK && p.nb > 1) {
         hipLaunchKernelGGL(iimg_carry_scan_kernel<T>, dim3(ceil_div(W, 256)), dim3(256), 0, st, W, p.nb, tot);

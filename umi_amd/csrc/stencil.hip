@@ -15,106 +15,34 @@
 // at load time), then each thread computes a vertical strip of outputs,
 // reusing the tile rows it has in flight; stores are coalesced along x.
 #include "common.h"
+#include "stencil_dev.h"
 
 namespace {
 using namespace umi;
 
-constexpr int F_TX = 64, F_TY = 32, F_ROWS = 8;  // 64 x 4 threads, 8 rows each
+constexpr int F_TX = 64, F_TY = 32;
+constexpr int BL_TX = 32, BL_TY = 8;  // one output per thread
 
-template <typename T>
-struct Arith;
-template <>
-struct Arith<float> {
-    static __device__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-    static __device__ float add(float a, float b) { return a + b; }
-    static __device__ float mul(float a, float b) { return a * b; }
-};
-template <>
-struct Arith<int32_t> {  // two's-complement wrap, as the reference's int32 examples rely on
-    static __device__ int32_t fma(int32_t a, int32_t b, int32_t c) { return (int32_t)((uint32_t)a * (uint32_t)b + (uint32_t)c); }
-    static __device__ int32_t add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
-    static __device__ int32_t mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
-};
-
-template <typename T>
-__device__ inline T load_bounded(const T *in, int H, int W, int ip, int gy, int gx, int mode, T pad) {
-    if (gy >= 0 && gy < H && gx >= 0 && gx < W) return in[(size_t)gy * ip + gx];
-    if (mode == 0) return pad;
-    return in[(size_t)clampi(gy, 0, H - 1) * ip + clampi(gx, 0, W - 1)];
-}
-
-// WEIGHTED = false: Filter3x3 (box sum then * mul); true: WFilter3x3.
 template <typename T, bool WEIGHTED>
 __global__ __launch_bounds__(256) void filter3x3_kernel(const T *__restrict__ in, int H, int W, int ip,
                                                         const T *__restrict__ wts, T init, T mul, T pad, int mode,
                                                         T *__restrict__ out, int op) {
-    constexpr int LW = F_TX + 2, LH = F_TY + 2;
-    __shared__ T tile[LH * LW];
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * F_TX, y0 = blockIdx.y * F_TY;
-    for (int e = tid; e < LH * LW; e += 256) {
-        const int r = e / LW, c = e - r * LW;
-        tile[e] = load_bounded(in, H, W, ip, y0 - 1 + r, x0 - 1 + c, mode, pad);
-    }
+    __shared__ T tile[(F_TY + 2) * (F_TX + 2)];
     T w[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) w[k] = WEIGHTED ? wts[k] : T(0);
-    __syncthreads();
-    const int tx = tid & (F_TX - 1), ty = tid / F_TX;
-    const int x = x0 + tx;
-    if (x >= W) return;
-#pragma unroll
-    for (int i = 0; i < F_ROWS; ++i) {
-        const int ly = ty * F_ROWS + i;
-        const int y = y0 + ly;
-        if (y >= H) break;
-        T acc = init;
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const T v = tile[(ly + dy) * LW + tx + dx];
-                acc = WEIGHTED ? Arith<T>::fma(v, w[dy * 3 + dx], acc) : Arith<T>::add(acc, v);
-            }
-        out[(size_t)y * op + x] = WEIGHTED ? acc : Arith<T>::mul(acc, mul);
-    }
+    filter3x3_tile<T, WEIGHTED, F_TX, F_TY>(tile, in, H, W, ip, w, init, mul, pad, mode, out, op, blockIdx.x * F_TX,
+                                            blockIdx.y * F_TY);
 }
-
-constexpr int BL_TX = 32, BL_TY = 8;  // one output per thread
-constexpr int BL_MAXR = 12;           // the reference example uses sigma 4 -> radius 12
 
 __global__ __launch_bounds__(256) void bilateral_kernel(const float *__restrict__ in, int H, int W, int ip, int r,
                                                         const float *__restrict__ ws, float norm, float pad,
                                                         int mode, float *__restrict__ out, int op) {
     extern __shared__ float bl_lds[];
-    const int LW = BL_TX + 2 * r, LH = BL_TY + 2 * r;
-    float *tile = bl_lds;
-    float *wsh = bl_lds + LW * LH;
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * BL_TX, y0 = blockIdx.y * BL_TY;
-    for (int e = tid; e < LH * LW; e += 256) {
-        const int rr = e / LW, c = e - rr * LW;
-        tile[e] = load_bounded(in, H, W, ip, y0 - r + rr, x0 - r + c, mode, pad);
-    }
-    if (tid < 2 * r + 1) wsh[tid] = ws[tid];
-    __syncthreads();
-    const int tx = tid % BL_TX, ty = tid / BL_TX;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= W || y >= H) return;
-    const float center = tile[(ty + r) * LW + tx + r];
-    float wsum = 0.f, wxsum = 0.f;
-    for (int dy = 0; dy <= 2 * r; ++dy) {
-        const float wy = wsh[dy];
-        const float *row = tile + (ty + dy) * LW + tx;
-        for (int dx = 0; dx <= 2 * r; ++dx) {
-            const float n = row[dx];
-            const float d = n - center;
-            const float w = expf(d * d * norm) * wy * wsh[dx];
-            wsum += w;
-            wxsum = __builtin_fmaf(w, n, wxsum);
-        }
-    }
-    out[(size_t)y * op + x] = wxsum / wsum;
+    float *wsh = bl_lds;
+    if ((int)threadIdx.x < 2 * r + 1) wsh[threadIdx.x] = ws[threadIdx.x];
+    bilateral_tile<BL_TX, BL_TY>(bl_lds + 32, wsh, in, H, W, ip, r, norm, pad, mode, out, op, blockIdx.x * BL_TX,
+                                 blockIdx.y * BL_TY);
 }
 
 template <typename T, bool WEIGHTED>
@@ -168,7 +96,7 @@ umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int ra
         return set_error(UMI_ERR_UNSUPPORTED, "umi_bilateral_f32: radius %d (0..%d)", radius, BL_MAXR);
     if (mode != 0 && mode != 1) return set_error(UMI_ERR_INVALID, "umi_bilateral_f32: mode %d", mode);
     if (H == 0 || W == 0) return UMI_OK;
-    const size_t lds = sizeof(float) * ((size_t)(BL_TX + 2 * radius) * (BL_TY + 2 * radius) + 2 * radius + 1);
+    const size_t lds = sizeof(float) * ((size_t)(BL_TX + 2 * radius) * (BL_TY + 2 * radius) + 32);
     dim3 grid(ceil_div(W, BL_TX), ceil_div(H, BL_TY));
     hipLaunchKernelGGL(bilateral_kernel, grid, dim3(256), lds, as_stream(s), in, H, W, in_pitch, radius, spatial,
                        norm, pad, mode, out, out_pitch);

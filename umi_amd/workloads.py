@@ -60,3 +60,19 @@ def stereo_pair(H: int = 375, W: int = 1242, D: int = 128, tile: int = 32, sigma
     src = L[yy, np.clip(xx + dd, 0, W - 1)].astype(np.float64)
     R = np.clip(np.rint(src + g.normal(0.0, sigma, size=(H, W))), 0, 255).astype(np.uint8)
     return L, R, d
+
+
+def uneven_sizes(n: int = 256, lo: int = 64, hi: int = 1024, seed: int = 1):
+    """n (H, W) pairs, each side log-uniform in [lo, hi] independently."""
+    g = rng(seed)
+    e = g.uniform(np.log(lo), np.log(hi), size=(n, 2))
+    return [(int(h), int(w)) for h, w in np.clip(np.rint(np.exp(e)), lo, hi).astype(int)]
+
+
+def uneven_batch(n: int = 256, lo: int = 64, hi: int = 1024, seed: int = 1):
+    """float32 images, values uniform [0, 1), with uneven log-uniform sizes; plus
+    their uint8 copies floor(256 x)."""
+    g = rng(seed + 1000)
+    imgs = [g.random(s, dtype=np.float32) for s in uneven_sizes(n, lo, hi, seed)]
+    u8 = [np.minimum(np.floor(im * 256.0), 255).astype(np.uint8) for im in imgs]
+    return imgs, u8
