@@ -35,7 +35,118 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
     return ap.parse_args()
+
+
+def timed(fn, st, n, w=3):
+    from umi_amd import Timer
+
+    for _ in range(w):
+        fn()
+    st.synchronize()
+    t = Timer()
+    t.start(st)
+    for _ in range(n):
+        fn()
+    t.stop(st)
+    st.synchronize()
+    return t.ms() / n
+
+
+def hbm_copy_probe(st, mb=256):
+    """Streaming copy kernel of known size: the measured HBM roof, and the byte
+    reference for the FETCH_SIZE / WRITE_SIZE counters of profiling runs."""
+    from umi_amd import DeviceBuffer
+    from umi_amd._lib import call
+
+    n = mb << 20
+    a, b = DeviceBuffer(n), DeviceBuffer(n)
+    ms = timed(lambda: call("umi_copy_d2d", b.ptr, a.ptr, n, st.handle), st, 20)
+    return {"bytes": 2 * n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
+
+
+def extra_configs(st):
+    """Secondary configurations, each on its own workload (not part of `value`):
+    C3 GEMM 4096^3 and conv 16x64x56x56 K64 3x3 in five precision forms with
+    FP64-oracle error; C2 cost volume; C1 filter; C4 uneven batch."""
+    import oracle
+    from umi_amd import DeviceBuffer, batch, ops, workloads
+
+    out = {}
+    g = workloads.rng(3)
+    # --- C3 GEMM (error on 64 sampled rows against FP64)
+    M = N = K = 4096
+    A64, B64 = g.standard_normal((M, K)), g.standard_normal((K, N))
+    dA, dB = DeviceBuffer.from_array(A64.astype(np.float32)), DeviceBuffer.from_array(B64.astype(np.float32))
+    rows = np.arange(0, M, M // 64)
+    ref = A64[rows] @ B64
+    gemm = {}
+    for mode in (ops.F32, ops.BF16, ops.BF16X3, ops.FP8, ops.FP8X3):
+        oa = ops.gemm_pack(dA, M, K, K, False, mode, stream=st)
+        ob = ops.gemm_pack(dB, N, K, N, True, mode, stream=st)
+        dC = DeviceBuffer(M * N * 4)
+        ms = timed(lambda: ops.gemm(mode, M, N, K, oa, ob, dC, stream=st), st, 10)
+        C = dC.download((M, N), np.float32)[rows]
+        e = oracle.contraction_error(C, ref, np.linalg.norm(A64[rows], axis=1), np.linalg.norm(B64, axis=0))
+        gemm[ops.MODE_NAMES[mode]] = {"ms": ms, "tflops": 2 * M * N * K / (ms * 1e-3) / 1e12, **e}
+    out["gemm_4096"] = gemm
+    # --- C3 conv (full FP64 oracle)
+    x64, w64 = g.standard_normal((16, 64, 56, 56)), g.standard_normal((64, 64, 3, 3))
+    ref = oracle.conv_f64(x64, w64, 1, 1)
+    cols, _ = oracle.im2col_f64(x64, 3, 3, 1, 1)
+    scale = np.linalg.norm(w64.reshape(64, -1), axis=1)[None, :, None, None] * \
+        np.linalg.norm(cols, axis=1).reshape(16, 56, 56)[:, None]
+    dx, dw = DeviceBuffer.from_array(x64.astype(np.float32)), DeviceBuffer.from_array(w64.astype(np.float32))
+    conv = {}
+    flops = 2 * 16 * 64 * 56 * 56 * 64 * 9
+    for mode in (ops.F32, ops.BF16, ops.BF16X3, ops.FP8, ops.FP8X3):
+        ox = ops.conv_pack_input(dx, 16, 64, 56, 56, mode, stream=st)
+        ow = ops.conv_pack_filter(dw, 64, 64, 3, 3, mode, stream=st)
+        o = DeviceBuffer(16 * 64 * 56 * 56 * 4)
+        ms = timed(lambda: ops.conv_fprop(mode, 16, 64, 56, 56, 64, 3, 3, 1, 1, ox, ow, o, stream=st), st, 20)
+        rel = np.abs(o.download((16, 64, 56, 56), np.float32) - ref) / scale
+        conv[ops.MODE_NAMES[mode]] = {"us": ms * 1e3, "tflops": flops / (ms * 1e-3) / 1e12,
+                                      "max_rel": float(rel.max()), "mean_rel": float(rel.mean())}
+    out["conv_16x64x56x56_k64_3x3"] = conv
+    # --- C2 cost volume
+    L, R, _ = workloads.stereo_pair(375, 1242, 128, seed=1)
+    dl, dr, cv = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R), DeviceBuffer(375 * 1242 * 128 * 4)
+    ms = timed(lambda: ops.cost_volume(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, cv, stream=st), st, 20)
+    out["cost_volume_1242x375_d128_5x5"] = {"ms": ms, "gbs": (375 * 1242 * 128 * 4 + 2 * 375 * 1242) / (ms * 1e-3) / 1e9}
+    # --- C1 filter
+    img = g.random((1024, 1024), dtype=np.float32)
+    wt = g.normal(size=9)
+    di, do = DeviceBuffer.from_array(img), DeviceBuffer(img.nbytes)
+    dwt = DeviceBuffer.from_array((wt / wt.sum()).astype(np.float32))
+    ms = timed(lambda: ops.wfilter3x3(di, 1024, 1024, 1024, dwt, 0.0, 0.0, 0, do, 1024, stream=st), st, 100)
+    out["wfilter3x3_1024_f32"] = {"us": ms * 1e3, "gbs": 8 * 1024 * 1024 / (ms * 1e-3) / 1e9}
+    # --- C4 uneven batch, graph-captured launches
+    imgs, u8 = workloads.uneven_batch(256, 64, 1024, seed=1)
+    bf, bu = batch.ImageBatch(imgs), batch.ImageBatch(u8, out_dtype=np.uint32)
+    px = bf.total
+    wsd = DeviceBuffer.from_array(ops.spatial_weights(2, 2.0))
+    wsu, wsf = bu.iimg_workspace(), bf.iimg_workspace()
+    t_wf = batch.time_graph(lambda s: bf.wfilter3x3(dwt, 0.0, 0.0, 0, 0, s), st)
+    t_bl = batch.time_graph(lambda s: bf.bilateral(2, wsd, -50.0, 0.0, 0, 1, s), st)
+    t_iu = batch.time_graph(lambda s: bu.iimg(wsu, s), st)
+    t_if = batch.time_graph(lambda s: bf.iimg(wsf, s), st)
+    out["batch_256_uneven"] = {"pixels": px, "wfilter_ms": t_wf, "wfilter_gbs": 8 * px / (t_wf * 1e-3) / 1e9,
+                               "bilateral_ms": t_bl, "bilateral_gpx_s": px / (t_bl * 1e-3) / 1e9,
+                               "iimg_u8_ms": t_iu, "iimg_u8_gbs": 5 * px / (t_iu * 1e-3) / 1e9,
+                               "iimg_f32_ms": t_if, "iimg_f32_gbs": 8 * px / (t_if * 1e-3) / 1e9}
+    return out
+
+
+def measured_traffic():
+    """HBM bytes per launch of the ME kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel reference)."""
+    p = os.path.join(ROOT, "profiles", "me_fullsearch_1080p_r16_pmc.json")
+    try:
+        with open(p) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(cur, ref, seconds):
@@ -76,6 +187,7 @@ def main():
     dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
     mv = DeviceBuffer(nblocks * 3 * 4)
     st = Stream()
+    copy = hbm_copy_probe(st)
     for _ in range(a.warmup):
         ops.me_full_search(dc, dr, H, W, W, BLK, RAD, mv, stream=st)
     st.synchronize()
@@ -115,11 +227,15 @@ def main():
         "config": {"workload": "full-search ME 1920x1080 u8, 8x8 blocks, +-16 (33x33), int32 SAD, MV+SAD out",
                    "blocks": nblocks, "candidates_per_block": (2 * RAD + 1) ** 2, "seed": a.seed},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "T abs-diff/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "hbm_gbs": (in_bytes + out_bytes) / (ms * 1e-3) / 1e9, "hbm_peak_gbs": HBM_PEAK_GBS},
+                     "frac": achieved / peak, "traffic": measured_traffic(),
+                     "algorithmic_bytes": in_bytes + out_bytes,
+                     "hbm_gbs": (in_bytes + out_bytes) / (ms * 1e-3) / 1e9, "hbm_peak_gbs": HBM_PEAK_GBS,
+                     "hbm_copy_probe_gbs": copy["gbs"]},
         "wall_s_timed_region": wall,
         "device": umi_amd.device_name(),
     }
+    if not a.no_extra:
+        line["extra"] = extra_configs(st)
     if not a.no_cpu:
         cb, cpu_mv = cpu_baseline(cur, ref, a.cpu_seconds)
         line["cpu_baseline"] = cb

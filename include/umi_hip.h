@@ -51,6 +51,10 @@ umi_status umi_memcpy(void *dst, const void *src, size_t bytes, int kind);
 umi_status umi_memcpy_async(void *dst, const void *src, size_t bytes, int kind, umi_stream s);
 umi_status umi_memset(void *ptr, int value, size_t bytes);
 umi_status umi_synchronize(void);
+/* Device-to-device copy by a streaming kernel (16-byte aligned, size a
+ * multiple of 16): graph-capturable, and the byte-count reference for HBM
+ * traffic counters.  No reference counterpart. */
+umi_status umi_copy_d2d(void *dst, const void *src, size_t bytes, umi_stream s);
 umi_status umi_stream_create(umi_stream *s);
 umi_status umi_stream_destroy(umi_stream s);
 umi_status umi_stream_synchronize(umi_stream s);

@@ -27,6 +27,7 @@ SIGNATURES = {
     "umi_memcpy_async": [P, P, c_size_t, c_int, P],
     "umi_memset": [P, c_int, c_size_t],
     "umi_synchronize": [],
+    "umi_copy_d2d": [P, P, c_size_t, P],
     "umi_stream_create": [C.POINTER(c_void_p)],
     "umi_stream_destroy": [P],
     "umi_stream_synchronize": [P],

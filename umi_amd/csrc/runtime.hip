@@ -5,6 +5,8 @@
 #include "common.h"
 #include <string.h>
 
+#include <algorithm>
+
 namespace umi {
 
 static thread_local char g_last_error[512] = "no error";
@@ -20,6 +22,15 @@ umi_status set_error(umi_status code, const char *fmt, ...) {
 }  // namespace umi
 
 using namespace umi;
+
+namespace {
+// Plain streaming copy, 16 B per lane; used as the HBM byte-count reference
+// for the profiler's FETCH_SIZE / WRITE_SIZE counters.
+__global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
 
 struct umi_timer_s {
     hipEvent_t start, stop;
@@ -98,6 +109,16 @@ umi_status umi_memset(void *ptr, int value, size_t bytes) {
     if (bytes == 0) return UMI_OK;
     if (!ptr) return set_error(UMI_ERR_INVALID, "umi_memset: null pointer");
     return hip_check(hipMemset(ptr, value, bytes), "hipMemset");
+}
+
+umi_status umi_copy_d2d(void *dst, const void *src, size_t bytes, umi_stream s) {
+    if (bytes == 0) return UMI_OK;
+    if (!dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return set_error(UMI_ERR_INVALID, "umi_copy_d2d: pointers and size must be 16-byte multiples");
+    const size_t n16 = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, as_stream(s), (const uint4 *)src, (uint4 *)dst, n16);
+    return launch_check("copy16_kernel");
 }
 
 umi_status umi_synchronize(void) { return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
