@@ -254,6 +254,67 @@ umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, 
 umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
                               umi_stream s);
 
+/* ---------------------------------------------------------------------------
+ * Learned motion proposals (no reference counterpart: johnjohnlin/UMI has
+ * only the exhaustive search of main/full_me.cu).  Building blocks of the
+ * proposal pipeline and of its on-GPU training; umi_amd/proposer.py strings
+ * them together.
+ * ------------------------------------------------------------------------- */
+/* Full search of the listed 8x8/16x16 blocks only (block index by*nbx+bx in
+ * list[0 .. *count)), writing their (dy, dx, SAD) into mv_sad; semantics of
+ * umi_me_full_search_u8.  max_count bounds *count. */
+umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int B, int R,
+                                        const int32_t *list, const int32_t *count, int max_count, int32_t *mv_sad,
+                                        umi_stream s);
+/* Synthetic pair: f0 = 5x5 box of hashed uniform noise; f1 = f0 warped by the
+ * smooth field of 4 bumps (9 floats each: cy, cx, 1/(2 s^2), ty, tx, a00,
+ * a01, a10, a11; components clamped to +-32) plus N(0, sigma^2) noise. */
+umi_status umi_synth_frame0(uint8_t *f0, int H, int W, int pitch, uint64_t seed, umi_stream s);
+umi_status umi_synth_warp(const uint8_t *f0, uint8_t *f1, int H, int W, int pitch, const float *bumps, float sigma,
+                          uint64_t seed, umi_stream s);
+/* Network input: fp32 [n][2][H/4][W/4], 4x4 box means scaled to [-1, 1]. */
+umi_status umi_cnn_downsample_pair(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, float *out, int n,
+                                   umi_stream s);
+/* Training kernels, fp32 NCHW, 3x3 convolution with dilation dil and zero
+ * padding dil (same size).  w: [Cout][Cin][3][3].  relu: the layer applies
+ * ReLU; backward passes mask with its output act > 0.  bwd_weight writes
+ * dw (Cout*Cin*9) followed by db (Cout). */
+umi_status umi_cnn_conv_fwd(const float *in, int N, int Cin, int Cout, int H, int W, int dil, const float *w,
+                            const float *bias, int relu, float *out, umi_stream s);
+umi_status umi_cnn_conv_bwd_data(const float *dout, const float *act, int N, int Cin, int Cout, int H, int W, int dil,
+                                 const float *w, int relu, float *din, umi_stream s);
+umi_status umi_cnn_conv_bwd_weight_workspace(int N, int Cin, int Cout, int H, int W, size_t *bytes);
+umi_status umi_cnn_conv_bwd_weight(const float *dout, const float *act, const float *in, int N, int Cin, int Cout,
+                                   int H, int W, int dil, int relu, float *dw_db, void *workspace,
+                                   size_t workspace_bytes, umi_stream s);
+/* Huber loss (delta 1) of the flow [N][2][H4][W4] against block labels
+ * [N][H4/2][W4/2][3] / 4; writes dloss/dflow (mean) and per-workgroup loss
+ * partial sums (count in *nparts). */
+umi_status umi_cnn_huber(const float *pred, const int32_t *labels, int N, int H4, int W4, float *grad,
+                         float *loss_parts, int *nparts, umi_stream s);
+umi_status umi_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
+                    int step, umi_stream s);
+/* Inference around the bf16 MFMA layers (umi_conv_nhwc_bf16): first layer
+ * (2 -> 32, fp32 NCHW in, bf16 NHWC out, ReLU), last layer (32 -> 2, bf16
+ * NHWC in, fp32 flow out); weights are rounded to bf16. */
+umi_status umi_cnn_first_layer(const float *in, int N, int H, int W, int dil, const float *w, const float *bias,
+                               uint16_t *out_nhwc, umi_stream s);
+umi_status umi_cnn_last_layer(const uint16_t *in_nhwc, int N, int H, int W, int dil, const float *w,
+                              const float *bias, float *flow, umi_stream s);
+/* Dilated 3x3 convolution on bf16 NHWC activations with a packed bf16 KRSC
+ * filter (umi_conv_pack_filter), + bias, optional ReLU, bf16 NHWC out. */
+umi_status umi_conv_nhwc_bf16(int N, int C, int H, int W, int K, int R, int S, int pad, int dil, const void *in,
+                              const void *w, const float *bias, int relu, void *out, umi_stream s);
+/* Block proposals from the quarter-resolution flow: round(4 x mean of the
+ * block's 2x2 outputs), clamped to [-R, R]; prop [H4/2][W4/2][2]. */
+umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s);
+/* Exact +-2 refinement around each 8x8 block's proposal (first minimum in
+ * raster order) into mv_sad; blocks whose minimum lies on the window border
+ * or whose SAD at a neighbour's proposal is below refined / slack are
+ * appended to fb_list (*fb_count, cleared first). */
+umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R, const int32_t *prop,
+                         float slack, int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count, umi_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,21 @@ SIGNATURES = {
     "umi_batch_iimg_workspace_size": [P, c_int, C.POINTER(c_size_t)],
     "umi_batch_iimg_u8_u32": [P, P, P, P, c_size_t, P],
     "umi_batch_iimg_f32": [P, P, P, P, c_size_t, P],
+    "umi_me_full_search_listed_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
+    "umi_synth_frame0": [P, c_int, c_int, c_int, C.c_uint64, P],
+    "umi_synth_warp": [P, P, c_int, c_int, c_int, P, c_float, C.c_uint64, P],
+    "umi_cnn_downsample_pair": [P, P, c_int, c_int, c_int, P, c_int, P],
+    "umi_cnn_conv_fwd": [P, c_int, c_int, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
+    "umi_cnn_conv_bwd_data": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, P, c_int, P, P],
+    "umi_cnn_conv_bwd_weight_workspace": [c_int, c_int, c_int, c_int, c_int, C.POINTER(c_size_t)],
+    "umi_cnn_conv_bwd_weight": [P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, c_size_t, P],
+    "umi_cnn_huber": [P, P, c_int, c_int, c_int, P, P, C.POINTER(c_int), P],
+    "umi_adam": [P, P, P, P, c_int, c_float, c_float, c_float, c_float, c_int, P],
+    "umi_cnn_first_layer": [P, c_int, c_int, c_int, c_int, P, P, P, P],
+    "umi_cnn_last_layer": [P, c_int, c_int, c_int, c_int, P, P, P, P],
+    "umi_conv_nhwc_bf16": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, c_int, P, P],
+    "umi_me_proposals": [P, c_int, c_int, c_int, P, P],
+    "umi_me_refine": [P, P, c_int, c_int, c_int, c_int, P, c_float, P, P, P, P],
     "umi_conv_fprop": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P,
                        P],
 }

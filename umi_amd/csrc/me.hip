@@ -44,12 +44,13 @@ __device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
     return v;
 }
 
+// One TBX x TBY tile of blocks whose first block is (bx0, by0); all threads
+// of the workgroup call it (it contains barriers).
 template <int B, bool QSAD, bool VOLUME>
-__global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
-    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R,
-    int nby, int nbx, int tbx, int tby, int G, int ws4, int wrows, int idx_bits,
-    int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume) {
-    extern __shared__ uint32_t lds[];
+__device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H,
+                               int W, int pitch, int R, int nby, int nbx, int tbx, int tby, int G, int ws4, int wrows,
+                               int idx_bits, int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume, int bx0,
+                               int by0) {
     constexpr int B4 = B / 4;
     const int nblk = tbx * tby;
     const int cs4 = tbx * B4;             // cur tile row stride (dwords)
@@ -59,7 +60,6 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int bx0 = blockIdx.x * tbx, by0 = blockIdx.y * tby;
     const int x0 = bx0 * B, y0 = by0 * B;
 
     // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(x0-R+c)].
@@ -195,6 +195,33 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
     }
 }
 
+template <int B, bool QSAD, bool VOLUME>
+__global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
+    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R, int nby, int nbx,
+    int tbx, int tby, int G, int ws4, int wrows, int idx_bits, int32_t *__restrict__ mv_sad,
+    int32_t *__restrict__ volume) {
+    extern __shared__ uint32_t lds[];
+    me_tile<B, QSAD, VOLUME>(lds, cur, ref, H, W, pitch, R, nby, nbx, tbx, tby, G, ws4, wrows, idx_bits, mv_sad,
+                             volume, blockIdx.x * tbx, blockIdx.y * tby);
+}
+
+// Listed blocks: a persistent grid walks list[0 .. *count) (block indices
+// by * nbx + bx), one block per iteration (tile 1 x 1).
+template <int B, bool QSAD>
+__global__ __launch_bounds__(me_max_threads<B>()) void me_listed_kernel(
+    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R, int nby, int nbx,
+    int G, int ws4, int wrows, int idx_bits, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int32_t *__restrict__ mv_sad) {
+    extern __shared__ uint32_t lds[];
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int blk = list[i];
+        me_tile<B, QSAD, false>(lds, cur, ref, H, W, pitch, R, nby, nbx, 1, 1, G, ws4, wrows, idx_bits, mv_sad,
+                                nullptr, blk % nbx, blk / nbx);
+        __syncthreads();
+    }
+}
+
 struct MePlan {
     int G, tbx, tby, threads, ws4, wrows, idx_bits;
     size_t lds;
@@ -277,3 +304,30 @@ umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, 
 }
 
 }  // extern "C"
+
+extern "C" umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                                   int B, int R, const int32_t *list, const int32_t *count,
+                                                   int max_count, int32_t *mv_sad, umi_stream s) {
+    using umi::set_error;
+    if (!cur || !ref || !list || !count || !mv_sad)
+        return set_error(UMI_ERR_INVALID, "umi_me_full_search_listed_u8: null pointer");
+    if (B != 8 && B != 16) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_listed_u8: block %d", B);
+    if (R < 1 || R > 64) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_listed_u8: radius %d", R);
+    if (H < 1 || W < 1 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_me_full_search_listed_u8: frame");
+    if (max_count <= 0 || H / B == 0 || W / B == 0) return UMI_OK;
+    const int G = umi::ceil_div(2 * R + 1, 4);
+    const int S = 2 * R + 1;
+    const int idx_bits = umi::bits_for((uint32_t)(S * S - 1));
+    const int ws4 = umi::ceil_div(B + 2 * R + 4, 4), wrows = B + 2 * R;
+    const size_t lds = 4 * ((size_t)wrows * ws4 + (size_t)B * (B / 4) + 1);
+    const int threads = umi::round_up(G, 64);
+    const int grid = max_count < 2048 ? max_count : 2048;
+    hipStream_t st = umi::as_stream(s);
+    if (B == 8)
+        hipLaunchKernelGGL((me_listed_kernel<8, true>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
+                           H / B, W / B, G, ws4, wrows, idx_bits, list, count, mv_sad);
+    else
+        hipLaunchKernelGGL((me_listed_kernel<16, true>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
+                           H / B, W / B, G, ws4, wrows, idx_bits, list, count, mv_sad);
+    return umi::launch_check("me_listed_kernel");
+}

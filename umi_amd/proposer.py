@@ -1,0 +1,361 @@
+"""Learned motion proposals for full-search block motion estimation.
+
+A 4-layer 3x3 convolution network (2 -> 32 -> 32 -> 32 -> 2 channels,
+dilations 1, 2, 4, 8, ReLU between layers; 19,682 parameters) reads the frame
+pair at quarter resolution and predicts the motion field.  The mean of each
+8x8 block's 2x2 quarter-resolution outputs, times 4, is the block's proposal.
+An exact SAD search over +-2 around the proposal refines it.  A block falls
+back to the full +-R search when the refined minimum lies on the window
+border, or when the block's SAD at one of its 8 neighbours' proposals is
+lower than the refined SAD by more than `slack` (1.1).
+
+Training runs entirely on the GPU with this project's kernels (fp32 forward,
+backward and Adam).  The labels are the plain full search's motion vectors.
+Inference runs the 32 -> 32 layers as bf16 MFMA convolutions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import struct
+from pathlib import Path
+
+import numpy as np
+
+from ._lib import call
+from .device import DeviceBuffer, Stream, Timer
+from . import ops
+
+LAYERS = [(2, 32, 1, True), (32, 32, 2, True), (32, 32, 4, True), (32, 2, 8, False)]  # (cin, cout, dilation, relu)
+MAGIC = b"UMIPROP1"
+CHECKPOINT = Path(__file__).resolve().parent / "data" / "proposer_1080p_r32.bin"
+
+
+def _s(st):
+    return None if st is None else st.handle
+
+
+def n_params() -> int:
+    return sum(co * ci * 9 + co for ci, co, _, _ in LAYERS)
+
+
+# ---------------------------------------------------------------- synthesis
+def bump_params(seed: int, H: int, W: int) -> np.ndarray:
+    """4 Gaussian-weighted affine motion bumps: (cy, cx, 1/(2 s^2), ty, tx, a00, a01, a10, a11)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    p = np.zeros((4, 9), np.float32)
+    for k in range(4):
+        s = g.uniform(200.0, 700.0)
+        p[k] = [g.uniform(0, H), g.uniform(0, W), 1.0 / (2 * s * s), g.uniform(-24, 24), g.uniform(-24, 24),
+                *g.uniform(-0.03, 0.03, size=4)]
+    return p
+
+
+class PairMaker:
+    """Synthetic 1920x1080 pairs on the GPU: ref = smoothed noise, cur = ref
+    warped by a smooth field (|mv| <= 32) plus N(0, sigma^2) noise."""
+
+    def __init__(self, H=1080, W=1920, sigma=2.0):
+        self.H, self.W, self.sigma = H, W, sigma
+        self.bumps = DeviceBuffer(4 * 9 * 4)
+
+    def make(self, seed: int, ref: DeviceBuffer, cur: DeviceBuffer, stream=None):
+        call("umi_synth_frame0", ref.ptr, self.H, self.W, self.W, C.c_uint64(2 * seed + 1), _s(stream))
+        self.bumps.upload(bump_params(seed, self.H, self.W))
+        call("umi_synth_warp", ref.ptr, cur.ptr, self.H, self.W, self.W, self.bumps.ptr, C.c_float(self.sigma),
+             C.c_uint64(2 * seed + 2), _s(stream))
+
+
+# ---------------------------------------------------------------- network
+class Network:
+    """fp32 master parameters (device), Adam state, forward/backward buffers."""
+
+    def __init__(self, seed: int = 0):
+        g = np.random.Generator(np.random.PCG64(seed))
+        self.host = []
+        for ci, co, _, _ in LAYERS:
+            w = (g.standard_normal((co, ci, 3, 3)) * np.sqrt(2.0 / (ci * 9))).astype(np.float32)
+            self.host.append((w, np.zeros(co, np.float32)))
+        self._upload()
+        self.step = 0
+
+    def _upload(self):
+        self.w = [DeviceBuffer.from_array(w) for w, _ in self.host]
+        self.b = [DeviceBuffer.from_array(b) for _, b in self.host]
+        self.m = [(DeviceBuffer.zeros(w.nbytes), DeviceBuffer.zeros(b.nbytes)) for w, b in self.host]
+        self.v = [(DeviceBuffer.zeros(w.nbytes), DeviceBuffer.zeros(b.nbytes)) for w, b in self.host]
+        # gradient buffers: weights then bias, contiguous (the backward kernel's output layout)
+        self.gw = [DeviceBuffer(w.nbytes + b.nbytes) for w, b in self.host]
+
+    def params(self):
+        return [(w.download(hw.shape, np.float32), b.download(hb.shape, np.float32))
+                for (w, b), (hw, hb) in zip(zip(self.w, self.b), self.host)]
+
+    # -- checkpoint: magic, version, step, per layer (cin, cout, dil, relu) + w + b
+    def save(self, path):
+        with open(path, "wb") as f:
+            f.write(MAGIC)
+            f.write(struct.pack("<iii", 1, self.step, len(LAYERS)))
+            for (ci, co, d, r), (w, b) in zip(LAYERS, self.params()):
+                f.write(struct.pack("<iiii", ci, co, d, int(r)))
+                f.write(w.astype("<f4").tobytes())
+                f.write(b.astype("<f4").tobytes())
+
+    @classmethod
+    def load(cls, path):
+        net = cls.__new__(cls)
+        data = Path(path).read_bytes()
+        if data[:8] != MAGIC:
+            raise ValueError(f"{path}: not a proposer checkpoint")
+        ver, step, nl = struct.unpack_from("<iii", data, 8)
+        if ver != 1 or nl != len(LAYERS):
+            raise ValueError(f"{path}: version {ver}, {nl} layers")
+        off = 20
+        net.host = []
+        for ci, co, d, r in LAYERS:
+            hdr = struct.unpack_from("<iiii", data, off)
+            if hdr != (ci, co, d, int(r)):
+                raise ValueError(f"{path}: layer {hdr} != {(ci, co, d, r)}")
+            off += 16
+            w = np.frombuffer(data, "<f4", co * ci * 9, off).reshape(co, ci, 3, 3).astype(np.float32)
+            off += w.nbytes
+            b = np.frombuffer(data, "<f4", co, off).astype(np.float32)
+            off += b.nbytes
+            net.host.append((w, b))
+        net._upload()
+        net.step = step
+        return net
+
+
+class Trainer:
+    """Forward + backward + Adam on batches of N quarter-resolution pairs."""
+
+    def __init__(self, net: Network, N: int, H4: int, W4: int):
+        self.net, self.N, self.H4, self.W4 = net, N, H4, W4
+        px = N * H4 * W4
+        self.act = [DeviceBuffer(px * co * 4) for _, co, _, _ in LAYERS]
+        self.grad = [DeviceBuffer(px * co * 4) for _, co, _, _ in LAYERS]
+        self.loss_parts = DeviceBuffer(((px + 255) // 256) * 4)
+        ws = 0
+        for ci, co, _, _ in LAYERS:
+            n = C.c_size_t()
+            call("umi_cnn_conv_bwd_weight_workspace", N, ci, co, H4, W4, C.byref(n))
+            ws = max(ws, n.value)
+        self.ws = DeviceBuffer(ws)
+
+    def forward(self, x: DeviceBuffer, stream=None):
+        src = x
+        for (ci, co, d, relu), w, b, a in zip(LAYERS, self.net.w, self.net.b, self.act):
+            call("umi_cnn_conv_fwd", src.ptr, self.N, ci, co, self.H4, self.W4, d, w.ptr, b.ptr, int(relu), a.ptr,
+                 _s(stream))
+            src = a
+        return self.act[-1]
+
+    def backward(self, x: DeviceBuffer, labels: DeviceBuffer, stream=None) -> None:
+        """Writes the loss partial sums and every layer's (dW, db) into net.gw."""
+        nparts = C.c_int()
+        call("umi_cnn_huber", self.act[-1].ptr, labels.ptr, self.N, self.H4, self.W4, self.grad[-1].ptr,
+             self.loss_parts.ptr, C.byref(nparts), _s(stream))
+        self.nparts = nparts.value
+        for li in range(len(LAYERS) - 1, -1, -1):
+            ci, co, d, relu = LAYERS[li]
+            inp = x if li == 0 else self.act[li - 1]
+            call("umi_cnn_conv_bwd_weight", self.grad[li].ptr, self.act[li].ptr, inp.ptr, self.N, ci, co, self.H4,
+                 self.W4, d, int(relu), self.net.gw[li].ptr, self.ws.ptr, self.ws.nbytes, _s(stream))
+            if li > 0:
+                call("umi_cnn_conv_bwd_data", self.grad[li].ptr, self.act[li].ptr, self.N, ci, co, self.H4, self.W4,
+                     d, self.net.w[li].ptr, int(relu), self.grad[li - 1].ptr, _s(stream))
+
+    def adam(self, lr=1e-3, stream=None):
+        self.net.step += 1
+        for li, (ci, co, _, _) in enumerate(LAYERS):
+            nw = co * ci * 9
+            gw = self.net.gw[li]
+            call("umi_adam", self.net.w[li].ptr, gw.ptr, self.net.m[li][0].ptr, self.net.v[li][0].ptr, nw,
+                 C.c_float(lr), C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), self.net.step, _s(stream))
+            call("umi_adam", self.net.b[li].ptr, gw.ptr + 4 * nw, self.net.m[li][1].ptr, self.net.v[li][1].ptr, co,
+                 C.c_float(lr), C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), self.net.step, _s(stream))
+
+    def loss(self) -> float:
+        parts = self.loss_parts.download((self.nparts,), np.float32)
+        return float(parts.astype(np.float64).sum() / (2 * self.N * self.H4 * self.W4))
+
+
+class Dataset:
+    """Quarter-resolution inputs and full-search labels of `n` seeded pairs, on the GPU."""
+
+    def __init__(self, n: int, seed0: int, H=1080, W=1920, R=32, stream=None):
+        self.n, self.H, self.W, self.R = n, H, W, R
+        self.H4, self.W4 = H // 4, W // 4
+        self.nby, self.nbx = H // 8, W // 8
+        self.x = DeviceBuffer(n * 2 * self.H4 * self.W4 * 4)
+        self.labels = DeviceBuffer(n * self.nby * self.nbx * 3 * 4)
+        pm = PairMaker(H, W)
+        ref, cur = DeviceBuffer(H * W), DeviceBuffer(H * W)
+        plane4 = 2 * self.H4 * self.W4 * 4
+        lab = self.nby * self.nbx * 3 * 4
+        for i in range(n):
+            pm.make(seed0 + i, ref, cur, stream)
+            call("umi_cnn_downsample_pair", cur.ptr, ref.ptr, H, W, W, self.x.ptr + i * plane4, 0, _s(stream))
+            call("umi_me_full_search_u8", cur.ptr, ref.ptr, H, W, W, 8, R, self.labels.ptr + i * lab, None, _s(stream))
+
+    def batch(self, idx):
+        """Device pointers of a contiguous batch [idx, idx + N)."""
+        return (self.x.ptr + idx * 2 * self.H4 * self.W4 * 4, self.labels.ptr + idx * self.nby * self.nbx * 12)
+
+
+class _View:
+    """A device pointer with the DeviceBuffer interface the kernels' wrappers use."""
+
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
+
+def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, log=print, stream=None):
+    tr = Trainer(net, batch, data.H4, data.W4)
+    g = np.random.Generator(np.random.PCG64(1234))
+    history = []
+    nb = data.n // batch
+    for ep in range(epochs):
+        order = g.permutation(nb)
+        tot = 0.0
+        for k, bi in enumerate(order):
+            xp, lp = data.batch(bi * batch)
+            x = _View(xp, 0)
+            tr.forward(x, stream)
+            tr.backward(x, _View(lp, 0), stream)
+            tr.adam(lr * (0.5 ** (ep // max(1, epochs // 3))), stream)
+            tot += tr.loss()
+        history.append(tot / nb)
+        log(f"[proposer] epoch {ep}: mean Huber loss {history[-1]:.4f}")
+    return history
+
+
+# ---------------------------------------------------------------- inference
+class Proposer:
+    """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
+
+    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1):
+        self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
+        self.H4, self.W4 = H // 4, W // 4
+        px = self.H4 * self.W4
+        self.nblk = (H // 8) * (W // 8)
+        self.x = DeviceBuffer(2 * px * 4)
+        self.a = [DeviceBuffer(px * 32 * 2) for _ in range(3)]
+        self.flow = DeviceBuffer(2 * px * 4)
+        self.prop = DeviceBuffer(self.nblk * 8)
+        self.fb_list = DeviceBuffer(self.nblk * 4)
+        self.fb_count = DeviceBuffer(4)
+        # bf16 KRSC filters of the two MFMA layers (rounded from the fp32 masters)
+        self.wk = []
+        for li in (1, 2):
+            op = ops.Operand(32, 9 * 32, ops.BF16)
+            call("umi_conv_pack_filter", net.w[li].ptr, 32, 32, 3, 3, ops.BF16, None, op.hi.ptr, None, None, None)
+            self.wk.append(op)
+
+    def propose(self, cur: DeviceBuffer, ref: DeviceBuffer, stream=None):
+        s = _s(stream)
+        call("umi_cnn_downsample_pair", cur.ptr, ref.ptr, self.H, self.W, self.W, self.x.ptr, 0, s)
+        call("umi_cnn_first_layer", self.x.ptr, 1, self.H4, self.W4, 1, self.net.w[0].ptr, self.net.b[0].ptr,
+             self.a[0].ptr, s)
+        call("umi_conv_nhwc_bf16", 1, 32, self.H4, self.W4, 32, 3, 3, 2, 2, self.a[0].ptr, self.wk[0].hi.ptr,
+             self.net.b[1].ptr, 1, self.a[1].ptr, s)
+        call("umi_conv_nhwc_bf16", 1, 32, self.H4, self.W4, 32, 3, 3, 4, 4, self.a[1].ptr, self.wk[1].hi.ptr,
+             self.net.b[2].ptr, 1, self.a[2].ptr, s)
+        call("umi_cnn_last_layer", self.a[2].ptr, 1, self.H4, self.W4, 8, self.net.w[3].ptr, self.net.b[3].ptr,
+             self.flow.ptr, s)
+        call("umi_me_proposals", self.flow.ptr, self.H4, self.W4, self.R, self.prop.ptr, s)
+
+    def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
+        s = _s(stream)
+        call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, C.c_float(self.slack),
+             mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr, s)
+        call("umi_me_full_search_listed_u8", cur.ptr, ref.ptr, self.H, self.W, self.W, 8, self.R, self.fb_list.ptr,
+             self.fb_count.ptr, self.nblk, mv_sad.ptr, s)
+
+    def run(self, cur, ref, mv_sad, stream=None, proposals=True):
+        """proposals=False: the same refinement and fallback around zero motion
+        (the method without the network)."""
+        if proposals:
+            self.propose(cur, ref, stream)
+        else:
+            self.prop.fill(0)
+        self.refine(cur, ref, mv_sad, stream)
+
+    def fallback_count(self) -> int:
+        return int(self.fb_count.download((1,), np.int32)[0])
+
+
+def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10):
+    """Fraction of blocks at the full search's minimum SAD, fallback fraction,
+    and mean times (ms) of the proposal pipeline and of the plain full search."""
+    H, W, R = prop.H, prop.W, prop.R
+    pm = PairMaker(H, W)
+    ref, cur = DeviceBuffer(H * W), DeviceBuffer(H * W)
+    mv, judge = DeviceBuffer(prop.nblk * 12), DeviceBuffer(prop.nblk * 12)
+    st = Stream()
+    at_min, fb, t_prop, t_full = [], [], [], []
+    for i in range(pairs):
+        pm.make(seed0 + i, ref, cur, st)
+        ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)
+        prop.run(cur, ref, mv, st, proposals)
+        st.synchronize()
+        a = mv.download((prop.nblk, 3), np.int32)
+        j = judge.download((prop.nblk, 3), np.int32)
+        at_min.append(float(np.mean(a[:, 2] == j[:, 2])))
+        fb.append(prop.fallback_count() / prop.nblk)
+        for name, fn, acc in (("prop", lambda: prop.run(cur, ref, mv, st, proposals), t_prop),
+                              ("full", lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st), t_full)):
+            fn()
+            st.synchronize()
+            t = Timer()
+            t.start(st)
+            for _ in range(reps):
+                fn()
+            t.stop(st)
+            acc.append(t.ms() / reps)
+    return {"at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
+            "fallback_frac": float(np.mean(fb)), "ms_proposal": float(np.mean(t_prop)),
+            "ms_full_search": float(np.mean(t_full)),
+            "speedup": float(np.mean(t_full) / np.mean(t_prop))}
+
+
+def main(argv=None):
+    """python -m umi_amd.proposer train --pairs 2000 --epochs 12 --out ckpt.bin
+       python -m umi_amd.proposer eval --ckpt ckpt.bin --pairs 4"""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(prog="python -m umi_amd.proposer")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed0", type=int, default=0, help="first training pair seed")
+    ap.add_argument("--eval-seed0", type=int, default=1_000_000, help="first held-out pair seed")
+    ap.add_argument("--ckpt", default=str(CHECKPOINT))
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    if a.cmd == "train":
+        t0 = time.time()
+        data = Dataset(a.pairs, a.seed0)
+        print(f"[proposer] {a.pairs} pairs synthesised and labelled in {time.time() - t0:.1f} s", flush=True)
+        net = Network(seed=1)
+        t0 = time.time()
+        hist = train(net, data, a.epochs, a.batch, a.lr, log=lambda m: print(m, flush=True))
+        print(f"[proposer] trained {a.epochs} epochs in {time.time() - t0:.1f} s", flush=True)
+        out = a.out or a.ckpt
+        Path(out).parent.mkdir(parents=True, exist_ok=True)
+        net.save(out)
+        print(json.dumps({"loss_history": hist, "checkpoint": out}), flush=True)
+    else:
+        net = Network.load(a.ckpt)
+        prop = Proposer(net)
+        with_net = evaluate(prop, a.pairs, a.eval_seed0, proposals=True)
+        without = evaluate(prop, a.pairs, a.eval_seed0, proposals=False)
+        print(json.dumps({"with_network": with_net, "without_network": without}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
