@@ -82,29 +82,52 @@ struct Mfma<GM_FP8X3> : Mfma<GM_FP8> {
 };
 
 // Row sources for a panel: dense K-contiguous rows, or implicit im2col rows
-// of an NHWC tensor (row = output pixel, k = (r, s, c)).
+// of an NHWC tensor (row = output pixel, k = (r, s, c)).  prep(row) is
+// evaluated once per tile, kprep(kbyte) once per stage, at() per chunk.
 struct DenseSrc {
     const uint8_t *base;
     long ld;      // bytes between rows
     int rows;
-    __device__ const uint8_t *row(int r, int kbyte, int /*esz*/) const {
-        return r < rows ? base + (long)r * ld + kbyte : nullptr;
-    }
+    struct P {
+        const uint8_t *p;
+    };
+    struct Kp {
+        int kb;
+    };
+    __device__ P prep(int r, int /*esz*/) const { return {r < rows ? base + (long)r * ld : nullptr}; }
+    __device__ Kp kprep(int kb, int /*esz*/) const { return {kb}; }
+    __device__ const uint8_t *at(const P &p, const Kp &k) const { return p.p ? p.p + k.kb : nullptr; }
 };
 struct ConvSrc {
     const uint8_t *base;  // NHWC input
     int N, H, W, C;       // C in elements
     int OH, OW, R, S, stride, pad, dil;
-    __device__ const uint8_t *row(int j, int kbyte, int esz) const {
+    struct P {
+        const uint8_t *img;  // start of image n, or null past the last pixel
+        int iy0, ix0;        // oy * stride - pad, ox * stride - pad
+    };
+    struct Kp {
+        int dy, dx;  // tap offset (r * dil, s * dil)
+        int cb;      // channel offset in bytes
+        int pixb;    // bytes per pixel (C * element size)
+    };
+    __device__ P prep(int j, int esz) const {
         const int pix = OH * OW;
-        if (j >= N * pix) return nullptr;
+        if (j >= N * pix) return {nullptr, 0, 0};
         const int n = j / pix, p = j - n * pix, oy = p / OW, ox = p - oy * OW;
+        return {base + (long)n * H * W * C * esz, oy * stride - pad, ox * stride - pad};
+    }
+    __device__ Kp kprep(int kbyte, int esz) const {
         const int k = kbyte / esz;
         const int rs = k / C, c = k - rs * C;
         const int r = rs / S, s = rs - r * S;
-        const int iy = oy * stride + r * dil - pad, ix = ox * stride + s * dil - pad;
+        return {r * dil, s * dil, c * esz, C * esz};
+    }
+    __device__ const uint8_t *at(const P &p, const Kp &k) const {
+        if (!p.img) return nullptr;
+        const int iy = p.iy0 + k.dy, ix = p.ix0 + k.dx;
         if (iy < 0 || iy >= H || ix < 0 || ix >= W) return nullptr;
-        return base + ((((long)n * H + iy) * W + ix) * C + c) * esz;
+        return p.img + ((long)iy * W + ix) * k.pixb + k.cb;
     }
 };
 
@@ -192,17 +215,31 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
     // global -> LDS assignment: thread t copies 16 B chunk (t & 3) of rows t/4 + 64 i
     const int lrow = tid >> 2, lchunk = (tid & 3) * 16;
     i32x4 ra[SPLIT ? 2 : 1][RA], rb[SPLIT ? 2 : 1][RB];
+    typename ASrc::P pa_[RA], pal_[RA];
+    typename BSrc::P pb_[RB], pbl_[RB];
+#pragma unroll
+    for (int i = 0; i < RA; ++i) {
+        pa_[i] = a_hi.prep(m0 + lrow + 64 * i, OP::ESZ);
+        if constexpr (SPLIT) pal_[i] = a_lo.prep(m0 + lrow + 64 * i, OP::ESZ);
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        pb_[i] = b_hi.prep(n0 + lrow + 64 * i, OP::ESZ);
+        if constexpr (SPLIT) pbl_[i] = b_lo.prep(n0 + lrow + 64 * i, OP::ESZ);
+    }
     auto gload = [&](int st) {
         const int kb = st * ROWB + lchunk;
+        const auto ka = a_hi.kprep(kb, OP::ESZ);
+        const auto kbp = b_hi.kprep(kb, OP::ESZ);
 #pragma unroll
         for (int i = 0; i < RA; ++i) {
-            ra[0][i] = fetch16(a_hi.row(m0 + lrow + 64 * i, kb, OP::ESZ));
-            if constexpr (SPLIT) ra[1][i] = fetch16(a_lo.row(m0 + lrow + 64 * i, kb, OP::ESZ));
+            ra[0][i] = fetch16(a_hi.at(pa_[i], ka));
+            if constexpr (SPLIT) ra[1][i] = fetch16(a_lo.at(pal_[i], ka));
         }
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
-            rb[0][i] = fetch16(b_hi.row(n0 + lrow + 64 * i, kb, OP::ESZ));
-            if constexpr (SPLIT) rb[1][i] = fetch16(b_lo.row(n0 + lrow + 64 * i, kb, OP::ESZ));
+            rb[0][i] = fetch16(b_hi.at(pb_[i], kbp));
+            if constexpr (SPLIT) rb[1][i] = fetch16(b_lo.at(pbl_[i], kbp));
         }
     };
     auto sstore = [&](int buf) {

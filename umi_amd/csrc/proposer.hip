@@ -76,8 +76,9 @@ __global__ void synth_warp_kernel(const uint8_t *f0, uint8_t *f1, int H, int W, 
 }
 
 // ---------------------------------------------------------------- network
-// Quarter-resolution input: out[n][c][y][x] = (mean of the 4x4 box / 255 - 0.5) * 2,
-// c = 0 current frame, 1 reference frame.
+// Quarter-resolution input: out[n][c][y][x] = (mean of the 4x4 box - 128) / 8,
+// c = 0 current frame, 1 reference frame.  (The box mean of smoothed noise
+// has a standard deviation of about 8 grey levels, so inputs are O(1).)
 __global__ void downsample_pair_kernel(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, float *out,
                                        int n) {
     const int H4 = H / 4, W4 = W / 4;
@@ -90,8 +91,8 @@ __global__ void downsample_pair_kernel(const uint8_t *cur, const uint8_t *ref, i
             b += ref[(size_t)(4 * y + dy) * pitch + 4 * x + dx];
         }
     const size_t plane = (size_t)H4 * W4;
-    out[(size_t)n * 2 * plane + (size_t)y * W4 + x] = (a * (1.f / (16.f * 255.f)) - 0.5f) * 2.f;
-    out[(size_t)n * 2 * plane + plane + (size_t)y * W4 + x] = (b * (1.f / (16.f * 255.f)) - 0.5f) * 2.f;
+    out[(size_t)n * 2 * plane + (size_t)y * W4 + x] = (a * (1.f / 16.f) - 128.f) * 0.125f;
+    out[(size_t)n * 2 * plane + plane + (size_t)y * W4 + x] = (b * (1.f / 16.f) - 128.f) * 0.125f;
 }
 
 // Forward 3x3 conv, dilation d, zero padding d (same size), + bias, optional

@@ -200,6 +200,7 @@ class Dataset:
 
     def batch(self, idx):
         """Device pointers of a contiguous batch [idx, idx + N)."""
+        idx = int(idx)
         return (self.x.ptr + idx * 2 * self.H4 * self.W4 * 4, self.labels.ptr + idx * self.nby * self.nbx * 12)
 
 
@@ -207,7 +208,7 @@ class _View:
     """A device pointer with the DeviceBuffer interface the kernels' wrappers use."""
 
     def __init__(self, ptr, nbytes):
-        self.ptr, self.nbytes = ptr, nbytes
+        self.ptr, self.nbytes = int(ptr), int(nbytes)
 
 
 def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, log=print, stream=None):
@@ -287,13 +288,26 @@ class Proposer:
 
 def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10):
     """Fraction of blocks at the full search's minimum SAD, fallback fraction,
-    and mean times (ms) of the proposal pipeline and of the plain full search."""
+    and mean times (ms) of the proposal pipeline (with its stages) and of the
+    plain full search."""
     H, W, R = prop.H, prop.W, prop.R
     pm = PairMaker(H, W)
     ref, cur = DeviceBuffer(H * W), DeviceBuffer(H * W)
     mv, judge = DeviceBuffer(prop.nblk * 12), DeviceBuffer(prop.nblk * 12)
     st = Stream()
-    at_min, fb, t_prop, t_full = [], [], [], []
+    at_min, fb = [], []
+    times = {"pipeline": [], "network": [], "refine": [], "full_search": []}
+
+    def timed(fn):
+        fn()
+        st.synchronize()
+        t = Timer()
+        t.start(st)
+        for _ in range(reps):
+            fn()
+        t.stop(st)
+        return t.ms() / reps
+
     for i in range(pairs):
         pm.make(seed0 + i, ref, cur, st)
         ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)
@@ -303,20 +317,18 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
         j = judge.download((prop.nblk, 3), np.int32)
         at_min.append(float(np.mean(a[:, 2] == j[:, 2])))
         fb.append(prop.fallback_count() / prop.nblk)
-        for name, fn, acc in (("prop", lambda: prop.run(cur, ref, mv, st, proposals), t_prop),
-                              ("full", lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st), t_full)):
-            fn()
-            st.synchronize()
-            t = Timer()
-            t.start(st)
-            for _ in range(reps):
-                fn()
-            t.stop(st)
-            acc.append(t.ms() / reps)
-    return {"at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
-            "fallback_frac": float(np.mean(fb)), "ms_proposal": float(np.mean(t_prop)),
-            "ms_full_search": float(np.mean(t_full)),
-            "speedup": float(np.mean(t_full) / np.mean(t_prop))}
+        times["pipeline"].append(timed(lambda: prop.run(cur, ref, mv, st, proposals)))
+        if proposals:
+            times["network"].append(timed(lambda: prop.propose(cur, ref, st)))
+        times["refine"].append(timed(lambda: call("umi_me_refine", cur.ptr, ref.ptr, H, W, W, R, prop.prop.ptr,
+                                                  C.c_float(prop.slack), mv.ptr, prop.fb_list.ptr,
+                                                  prop.fb_count.ptr, st.handle)))
+        times["full_search"].append(timed(lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)))
+    out = {"at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
+           "fallback_frac": float(np.mean(fb))}
+    out.update({f"ms_{k}": float(np.mean(v)) for k, v in times.items() if v})
+    out["speedup"] = out["ms_full_search"] / out["ms_pipeline"]
+    return out
 
 
 def main(argv=None):
