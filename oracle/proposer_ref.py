@@ -5,6 +5,12 @@ network: 3x3 convolutions with dilation d and zero padding d, + bias, ReLU
 on all but the last layer; Huber loss (delta 1) against block labels / 4,
 mean over all outputs.  The backward pass is written out analytically and
 pinned by finite differences in tests/test_oracle_proposer.py.
+
+The refinement (seed, iterated +-2 descent, jump-flood propagation rounds,
+fallback check) has no counterpart in the reference; it is pinned in
+tests/test_oracle_proposer.py by the C oracle's SAD volume, by exact-shift
+frames (SAD 0 at the true vector), by the local-minimum property of every
+converged block and by a scalar recomputation of the fallback test.
 """
 from __future__ import annotations
 
@@ -87,32 +93,99 @@ def block_sad(cur, ref, by, bx, dy, dx):
     return int(np.abs(cur[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64) - ref[np.ix_(ys, xs)]).sum())
 
 
-def refine(cur, ref, prop, R, slack):
-    """(mv_sad [nblk][3], fallback set) of the +-2 refinement rule."""
-    H, W = cur.shape
-    nby, nbx = H // 8, W // 8
-    out = np.zeros((nby * nbx, 3), np.int64)
-    fb = set()
-    for by in range(nby):
-        for bx in range(nbx):
-            b = by * nbx + bx
-            py, px = prop[b]
-            best, bi = None, 0
-            for i in range(25):
-                dy = int(np.clip(py + i // 5 - 2, -R, R))
-                dx = int(np.clip(px + i % 5 - 2, -R, R))
-                s = block_sad(cur, ref, by, bx, dy, dx)
-                if best is None or s < best:
-                    best, bi, bmv = s, i, (dy, dx)
-            nb = None
-            for o in (0, 1, 2, 3, 5, 6, 7, 8):
-                ny, nx = by + o // 3 - 1, bx + o % 3 - 1
-                if 0 <= ny < nby and 0 <= nx < nbx:
-                    s = block_sad(cur, ref, by, bx, *prop[ny * nbx + nx])
-                    nb = s if nb is None else min(nb, s)
-            out[b] = (bmv[0], bmv[1], best)
-            border = bi // 5 in (0, 4) or bi % 5 in (0, 4)
-            worse = nb is not None and best > slack * min(nb, best)
-            if border or worse:
-                fb.add(b)
-    return out, fb
+JUMPS = (8, 4, 2, 1)                       # propagation distance of round p: JUMPS[p % 4]
+RING = (0, 1, 2, 3, 5, 6, 7, 8)            # 3x3 neighbourhood in raster order without the centre (4)
+
+
+class _Blocks:
+    """All 8x8 blocks of a frame pair, with vectorised clamped SADs."""
+
+    def __init__(self, cur, ref):
+        self.H, self.W = cur.shape
+        self.nby, self.nbx = self.H // 8, self.W // 8
+        self.n = self.nby * self.nbx
+        self.by, self.bx = np.divmod(np.arange(self.n), self.nbx)
+        self.cur = (cur[:self.nby * 8, :self.nbx * 8].reshape(self.nby, 8, self.nbx, 8)
+                    .transpose(0, 2, 1, 3).reshape(self.n, 8, 8).astype(np.int64))
+        self.ref = np.asarray(ref)
+
+    def sad(self, mv):
+        """SAD of every block at its vector mv [n][2] (ref coordinates clamped)."""
+        ys = np.clip(self.by[:, None] * 8 + np.arange(8)[None] + mv[:, :1], 0, self.H - 1)
+        xs = np.clip(self.bx[:, None] * 8 + np.arange(8)[None] + mv[:, 1:2], 0, self.W - 1)
+        return np.abs(self.cur - self.ref[ys[:, :, None], xs[:, None, :]]).sum((1, 2))
+
+    def neighbour(self, o, d):
+        """(in-frame mask, index) of neighbour o (0..8, raster, 4 = self) at distance d."""
+        ny, nx = self.by + (o // 3 - 1) * d, self.bx + (o % 3 - 1) * d
+        ok = (ny >= 0) & (ny < self.nby) & (nx >= 0) & (nx < self.nbx)
+        return ok, np.where(ok, ny * self.nbx + nx, 0)
+
+    def first_min(self, cands):
+        """First minimum (strict <, in list order) over [(mask, mv [n][2])]:
+        (sad, mv); sad = 2**28 - 1 where no candidate is valid."""
+        best = np.full(self.n, 2 ** 28 - 1, np.int64)
+        bmv = np.zeros((self.n, 2), np.int64)
+        for ok, mv in cands:
+            s = np.where(ok, self.sad(mv), 2 ** 28 - 1)
+            m = s < best
+            best, bmv = np.where(m, s, best), np.where(m[:, None], mv, bmv)
+        return best, bmv
+
+
+def _descend(B, c, active, R, iters):
+    """Iterated +-2 descent of the active blocks from c: (mv, sad, unconv)."""
+    c = c.copy()
+    sad = np.zeros(B.n, np.int64)
+    unconv = np.ones(B.n, bool)
+    act = active.copy()
+    offs = [(i // 5 - 2, i % 5 - 2) for i in range(25)]
+    for _ in range(iters):
+        if not act.any():
+            break
+        best, nmv = B.first_min([(np.ones(B.n, bool), np.clip(c + np.array(o), -R, R)) for o in offs])
+        moved = (np.abs(nmv - c) == 2).any(axis=1)
+        c = np.where(act[:, None], nmv, c)
+        sad = np.where(act, best, sad)
+        unconv = np.where(act & ~moved, False, unconv)
+        act = act & moved
+    return c, sad, unconv
+
+
+def refine(cur, ref, prop, R, iters, rounds, slack):
+    """Refinement of block proposals (seed, descent, propagation rounds,
+    check), restating umi_me_refine.  Returns (mv_sad [n][3], fallback set,
+    changed flags of the rounds)."""
+    B = _Blocks(cur, ref)
+    prop = np.clip(np.asarray(prop, np.int64).reshape(B.n, 2), -R, R)
+    seeds = [(np.ones(B.n, bool), prop)]
+    for o in RING:
+        ok, nb = B.neighbour(o, 1)
+        seeds.append((ok, prop[nb]))
+    _, c = B.first_min(seeds)
+    mv, sad, unconv = _descend(B, c, np.ones(B.n, bool), R, iters)
+    changed = [False] * rounds
+    for p in range(rounds):
+        if p > 0 and not changed[p - 1]:
+            break
+        k = JUMPS[p % 4]
+        cands = []
+        for d in ((1, k) if k > 1 else (1,)):
+            for o in RING:
+                ok, nb = B.neighbour(o, d)
+                cands.append((ok, mv[nb]))
+        best, bmv = B.first_min(cands)
+        imp = best < sad
+        changed[p] = bool(imp.any())
+        if changed[p]:
+            m2, s2, u2 = _descend(B, bmv, imp, R, iters)
+            mv = np.where(imp[:, None], m2, mv)
+            sad = np.where(imp, s2, sad)
+            unconv = np.where(imp, u2, unconv)
+    nbmin = np.full(B.n, 2 ** 32 - 1, np.int64)
+    for o in RING:
+        ok, nb = B.neighbour(o, 1)
+        nbmin = np.minimum(nbmin, np.where(ok, B.sad(mv[nb]), 2 ** 32 - 1))
+    worse = sad.astype(np.float32) > np.float32(slack) * np.minimum(sad, nbmin).astype(np.float32)
+    fb = set(np.nonzero(unconv | worse)[0].tolist())
+    return np.concatenate([mv, sad[:, None]], axis=1), fb, changed

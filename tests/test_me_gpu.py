@@ -96,3 +96,32 @@ def test_me_pitch_and_offsets(gpu):
     got = mv.download((6, 7, 3), np.int32)
     omv, _ = oracle.me_full_search(np.ascontiguousarray(big_c[:, :W]), np.ascontiguousarray(big_r[:, :W]), 8, 4)
     np.testing.assert_array_equal(got, omv)
+
+
+@pytest.mark.parametrize("H,W,B,R,nlist", [(64, 96, 8, 16, 20), (72, 80, 16, 7, 5), (100, 130, 8, 32, 40),
+                                           (37, 53, 8, 1, 3), (96, 160, 16, 16, 17), (80, 88, 8, 64, 9),
+                                           (64, 64, 8, 8, 0)])
+def test_me_listed_vs_oracle(gpu, H, W, B, R, nlist):
+    """umi_me_full_search_listed_u8 writes the oracle's (dy, dx, SAD) for the
+    listed blocks and leaves every other entry untouched."""
+    from umi_amd import DeviceBuffer
+    g = workloads.rng(H * 7 + W + R)
+    cur = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    ref = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    nby, nbx = H // B, W // B
+    n = nby * nbx
+    lst = g.permutation(n)[:nlist].astype(np.int32)
+    print(f"[me] listed H={H} W={W} B={B} R={R} n={nlist}", flush=True)
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    dl, dn = DeviceBuffer.from_array(np.concatenate([lst, [0]]).astype(np.int32)), DeviceBuffer.from_array(
+        np.array([nlist], np.int32))
+    init = np.full((n, 3), -7, np.int32)
+    mv = DeviceBuffer.from_array(init)
+    call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, B, R, dl.ptr, dn.ptr, n, mv.ptr, None)
+    got = mv.download((n, 3), np.int32)
+    want, _ = oracle.me_full_search(cur, ref, B, R)
+    want = want.reshape(n, 3)
+    listed = np.zeros(n, bool)
+    listed[lst] = True
+    np.testing.assert_array_equal(got[listed], want[listed])
+    np.testing.assert_array_equal(got[~listed], init[~listed])

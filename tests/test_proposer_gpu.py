@@ -4,11 +4,13 @@
   match the FP64 oracle (oracle/proposer_ref.py, pinned by finite
   differences) on a small input;
 * Adam matches its formula; training lowers the loss;
-* refinement and fallback match the oracle rule; the listed full search
-  equals the full search on the listed blocks;
+* refinement (seed, descent, propagation rounds, fallback check) matches the
+  oracle exactly, including ragged frames, padded rows and edge settings; the
+  listed full search equals the full search on the listed blocks and leaves
+  the others alone;
 * with the trained network (umi_amd/data checkpoint): >= 99 % of blocks reach
-  the full search's minimum SAD on held-out pairs, with fewer fallbacks than
-  the same refinement around zero motion (the method without the network).
+  the full search's minimum SAD on held-out pairs, more than the same
+  refinement from zero proposals (the method without the network).
 """
 import ctypes as C
 
@@ -69,33 +71,78 @@ def test_adam_formula(gpu):
 
 
 def test_training_lowers_loss(gpu):
-    data = proposer.Dataset(16, seed0=500)
+    """64 Adam steps on 8 full-size pairs (batch 4) lower the training loss."""
+    data = proposer.Dataset(8, seed0=500)
     net = proposer.Network(seed=1)
-    hist = proposer.train(net, data, epochs=4, batch=4, lr=2e-3)
-    assert hist[-1] < 0.8 * hist[0], hist
+    hist = proposer.train(net, data, epochs=32, batch=4, lr=2e-3)
+    print(f"[proposer] loss history {hist}", flush=True)
+    assert hist[-1] < 0.85 * hist[0], hist
 
 
-def test_refine_rule_and_listed_search_match_oracle(gpu):
+def _refine_gpu(cur, ref, prop, R, iters, rounds, slack, pitch=None):
+    H, W = cur.shape
+    pitch = pitch or W
+    n = (H // 8) * (W // 8)
+    pad = lambda f: np.pad(f, ((0, 0), (0, pitch - W)))  # noqa: E731
+    dc, dr = DeviceBuffer.from_array(pad(cur)), DeviceBuffer.from_array(pad(ref))
+    dp = DeviceBuffer.from_array(np.ascontiguousarray(prop, np.int32))
+    mv, fl, fc = DeviceBuffer(n * 12), DeviceBuffer(n * 4), DeviceBuffer(4)
+    nbytes = C.c_size_t()
+    call("umi_me_refine_workspace", H, W, rounds, C.byref(nbytes))
+    ws = DeviceBuffer(nbytes.value)
+    print(f"[refine] H={H} W={W} pitch={pitch} R={R} iters={iters} rounds={rounds} slack={slack}", flush=True)
+    call("umi_me_refine", dc.ptr, dr.ptr, H, W, pitch, R, dp.ptr, iters, rounds, C.c_float(slack), mv.ptr, fl.ptr,
+         fc.ptr, ws.ptr, ws.nbytes, None)
+    got = mv.download((n, 3), np.int32)
+    k = int(fc.download((1,), np.int32)[0])
+    fb = fl.download((max(k, 1),), np.int32)[:k].tolist()
+    raw = ws.download((ws.nbytes,), np.uint8)
+    off = 2 * ((n * 16 + 255) // 256 * 256)
+    changed = np.frombuffer(raw[off:off + 4 * rounds].tobytes(), np.int32) != 0
+    return got, fb, changed.tolist(), (dc, dr, fl, fc, mv)
+
+
+@pytest.mark.parametrize("case", [
+    dict(H=96, W=136, R=32, iters=8, rounds=8, slack=1.1, kind="tiles"),
+    dict(H=100, W=138, R=16, iters=8, rounds=8, slack=1.1, kind="smooth"),         # ragged, unaligned rows
+    dict(H=96, W=136, R=16, iters=8, rounds=8, slack=1.1, kind="smooth", pitch=140),
+    dict(H=256, W=384, R=24, iters=8, rounds=12, slack=1.1, kind="smooth"),
+    dict(H=64, W=64, R=8, iters=1, rounds=0, slack=1.1, kind="smooth"),            # no rounds, one step
+    dict(H=64, W=72, R=8, iters=4, rounds=3, slack=0.0, kind="smooth"),            # slack 0 forces fallbacks
+])
+def test_refine_matches_oracle(gpu, case):
+    H, W, R = case["H"], case["W"], case["R"]
+    if case["kind"] == "tiles":
+        cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)
+    else:
+        cur, ref = workloads.smooth_motion_pair(H, W, R, seed=H + W)
+    n = (H // 8) * (W // 8)
+    g = workloads.rng(10)
+    prop = g.integers(-R - 3, R + 4, size=(n, 2)).astype(np.int32)   # out-of-range proposals are clamped
+    prop[::3] = 0
+    got, fb, changed, _ = _refine_gpu(cur, ref, prop, R, case["iters"], case["rounds"], case["slack"],
+                                      case.get("pitch"))
+    want, want_fb, want_changed = PR.refine(cur, ref, prop, R, case["iters"], case["rounds"], case["slack"])
+    np.testing.assert_array_equal(got, want)
+    assert len(fb) == len(set(fb)) and set(fb) == want_fb
+    assert changed == want_changed
+
+
+def test_listed_search_completes_fallbacks(gpu):
     H, W, R = 96, 136, 32
     cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)
-    nby, nbx = H // 8, W // 8
-    g = workloads.rng(10)
-    prop = g.integers(-R, R + 1, size=(nby * nbx, 2)).astype(np.int32)
-    prop[::3] = 0
-    dc, dr, dp = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref), DeviceBuffer.from_array(prop)
-    mv = DeviceBuffer(nby * nbx * 12)
-    fl, fc = DeviceBuffer(nby * nbx * 4), DeviceBuffer(4)
-    call("umi_me_refine", dc.ptr, dr.ptr, H, W, W, R, dp.ptr, C.c_float(1.1), mv.ptr, fl.ptr, fc.ptr, None)
-    got = mv.download((nby * nbx, 3), np.int32)
-    n = int(fc.download((1,), np.int32)[0])
-    ref_mv, ref_fb = PR.refine(cur, ref, prop, R, 1.1)
-    np.testing.assert_array_equal(got, ref_mv)
-    assert set(fl.download((n,), np.int32).tolist()) == ref_fb
-    call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, 8, R, fl.ptr, fc.ptr, nby * nbx, mv.ptr, None)
+    n = (H // 8) * (W // 8)
+    prop = workloads.rng(11).integers(-R, R + 1, size=(n, 2)).astype(np.int32)
+    before, fb, _, (dc, dr, fl, fc, mv) = _refine_gpu(cur, ref, prop, R, 2, 1, 1.1)
+    assert 0 < len(fb) < n
+    call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, 8, R, fl.ptr, fc.ptr, n, mv.ptr, None)
     full, _ = ops.me_full_search_np(cur, ref, 8, R)
-    after = mv.download((nby, nbx, 3), np.int32)
-    for b in ref_fb:
-        np.testing.assert_array_equal(after[b // nbx, b % nbx], full[b // nbx, b % nbx])
+    after = mv.download((n, 3), np.int32)
+    full = full.reshape(n, 3)
+    listed = np.zeros(n, bool)
+    listed[fb] = True
+    np.testing.assert_array_equal(after[listed], full[listed])
+    np.testing.assert_array_equal(after[~listed], before[~listed])
 
 
 def test_synthetic_pairs_deterministic(gpu):
@@ -110,6 +157,10 @@ def test_synthetic_pairs_deterministic(gpu):
 
 
 def test_trained_proposer_quality_and_gain(gpu):
+    """Held-out 1920x1080 pairs, +-32 search: with the committed network >= 99 %
+    of blocks reach the full search's minimum SAD; the same refinement from
+    zero proposals (the method without the network), at the same round
+    budget, reaches fewer."""
     if not proposer.CHECKPOINT.exists():
         pytest.skip("no trained checkpoint committed yet")
     net = proposer.Network.load(proposer.CHECKPOINT)
@@ -118,4 +169,4 @@ def test_trained_proposer_quality_and_gain(gpu):
     without = proposer.evaluate(prop, 3, 2_000_000, proposals=False, reps=3)
     print(f"[proposer] with network {with_net}\n[proposer] without {without}", flush=True)
     assert with_net["at_min_sad"] >= 0.99
-    assert with_net["fallback_frac"] < without["fallback_frac"]
+    assert without["at_min_sad"] < with_net["at_min_sad"]

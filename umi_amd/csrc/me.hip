@@ -206,18 +206,73 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
 }
 
 // Listed blocks: a persistent grid walks list[0 .. *count) (block indices
-// by * nbx + bx), one block per iteration (tile 1 x 1).
-template <int B, bool QSAD>
-__global__ __launch_bounds__(me_max_threads<B>()) void me_listed_kernel(
-    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R, int nby, int nbx,
-    int G, int ws4, int wrows, int idx_bits, const int32_t *__restrict__ list, const int32_t *__restrict__ count,
-    int32_t *__restrict__ mv_sad) {
+// by * nbx + bx), one block per iteration.  Thread t searches the 4 horizontal
+// offsets 4 (t % G) .. + 3 over the vertical offsets of chunk t / G (DCH
+// offsets each), from the block's staged window; the first minimum in raster
+// order wins, as in the full search.
+template <int B>
+__global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restrict__ cur,
+                                                        const uint8_t *__restrict__ ref, int H, int W, int pitch,
+                                                        int R, int nbx, int G, int DCH, int ws4, int wrows,
+                                                        int idx_bits, const int32_t *__restrict__ list,
+                                                        const int32_t *__restrict__ count,
+                                                        int32_t *__restrict__ mv_sad) {
+    constexpr int B4 = B / 4;
     extern __shared__ uint32_t lds[];
+    uint32_t *win = lds, *ctile = win + wrows * ws4, *best = ctile + B * B4;
+    const int S = 2 * R + 1;
+    const int tid = threadIdx.x, g = tid % G, ch = tid / G;
     const int n = *count;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int blk = list[i];
-        me_tile<B, QSAD, false>(lds, cur, ref, H, W, pitch, R, nby, nbx, 1, 1, G, ws4, wrows, idx_bits, mv_sad,
-                                nullptr, blk % nbx, blk / nbx);
+        const int bx = blk % nbx, by = blk / nbx, x0 = bx * B, y0 = by * B;
+        for (int k = tid; k < wrows * ws4; k += blockDim.x) {
+            const int r = k / ws4, c4 = k - r * ws4;
+            win[k] = load4_clamped(ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch, x0 - R + 4 * c4, W);
+        }
+        for (int k = tid; k < B * B4; k += blockDim.x)
+            ctile[k] = load4_clamped(cur + (size_t)(y0 + k / B4) * pitch, x0 + 4 * (k % B4), W);
+        if (tid == 0) *best = 0xffffffffu;
+        __syncthreads();
+        const int d0 = ch * DCH, d1 = min(S, d0 + DCH);
+        if (d0 < S) {
+            uint32_t c[B][B4];
+#pragma unroll
+            for (int py = 0; py < B; ++py)
+#pragma unroll
+                for (int k = 0; k < B4; ++k) c[py][k] = ctile[py * B4 + k];
+            uint32_t bestk = 0xffffffffu;
+            for (int dyi = d0; dyi < d1; ++dyi) {
+                uint64_t acc = 0;  // 4 x u16 SADs at horizontal offsets 4g .. 4g + 3
+#pragma unroll
+                for (int py = 0; py < B; ++py) {
+                    const uint32_t *wr = win + (dyi + py) * ws4 + g;
+                    uint32_t w[B4 + 1];
+#pragma unroll
+                    for (int k = 0; k <= B4; ++k) w[k] = wr[k];
+#pragma unroll
+                    for (int k = 0; k < B4; ++k)
+                        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w[k] | ((uint64_t)w[k + 1] << 32), c[py][k],
+                                                              acc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dxi = 4 * g + q;
+                    if (dxi < S)
+                        bestk = min(bestk, ((uint32_t)(acc >> (16 * q)) & 0xffffu) << idx_bits |
+                                               (uint32_t)(dyi * S + dxi));
+                }
+            }
+            atomicMin(best, bestk);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t key = *best, idx = key & ((1u << idx_bits) - 1u);
+            int32_t *o = mv_sad + (size_t)blk * 3;
+            o[0] = (int32_t)(idx / S) - R;
+            o[1] = (int32_t)(idx % S) - R;
+            o[2] = (int32_t)(key >> idx_bits);
+        }
         __syncthreads();
     }
 }
@@ -318,16 +373,22 @@ extern "C" umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uin
     const int G = umi::ceil_div(2 * R + 1, 4);
     const int S = 2 * R + 1;
     const int idx_bits = umi::bits_for((uint32_t)(S * S - 1));
+    if (umi::bits_for(255u * B * B) + idx_bits > 32)
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_listed_u8: key overflow");
     const int ws4 = umi::ceil_div(B + 2 * R + 4, 4), wrows = B + 2 * R;
     const size_t lds = 4 * ((size_t)wrows * ws4 + (size_t)B * (B / 4) + 1);
-    const int threads = umi::round_up(G, 64);
+    // split the vertical offsets into chunks so that G x chunks threads fit 512
+    // (256 for B = 16, whose cur rows take 64 registers)
+    const int max_threads = B == 8 ? 512 : 256;
+    const int DCH = umi::ceil_div(S, max_threads / G);
+    const int threads = umi::round_up(G * umi::ceil_div(S, DCH), 64);
     const int grid = max_count < 2048 ? max_count : 2048;
     hipStream_t st = umi::as_stream(s);
     if (B == 8)
-        hipLaunchKernelGGL((me_listed_kernel<8, true>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
-                           H / B, W / B, G, ws4, wrows, idx_bits, list, count, mv_sad);
+        hipLaunchKernelGGL((me_listed_kernel<8>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R, W / B,
+                           G, DCH, ws4, wrows, idx_bits, list, count, mv_sad);
     else
-        hipLaunchKernelGGL((me_listed_kernel<16, true>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
-                           H / B, W / B, G, ws4, wrows, idx_bits, list, count, mv_sad);
+        hipLaunchKernelGGL((me_listed_kernel<16>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
+                           W / B, G, DCH, ws4, wrows, idx_bits, list, count, mv_sad);
     return umi::launch_check("me_listed_kernel");
 }

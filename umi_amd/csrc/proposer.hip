@@ -370,66 +370,327 @@ __global__ void proposals_kernel(const float *__restrict__ flow, int H4, int W4,
     }
 }
 
-// SAD of the 8x8 block (by, bx) of cur against ref at offset (dy, dx), ref clamped.
-__device__ inline int sad8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int by, int bx, int dy,
-                           int dx) {
-    int s = 0;
-    const int y0 = by * 8, x0 = bx * 8;
-    for (int py = 0; py < 8; ++py) {
-        const uint8_t *c = cur + (size_t)(y0 + py) * pitch + x0;
-        const uint8_t *r = ref + (size_t)clampi(y0 + py + dy, 0, H - 1) * pitch;
+// ---------------------------------------------------------------- refinement
+// Work layout: a workgroup owns a tile of RT_Y x RT_X blocks and stages in LDS
+// the cur tile and the ref region every vector in [-R, R]^2 can reach (ref
+// coordinates clamped at fill time); 8 lanes per block then evaluate the
+// block's candidate vectors from LDS.
+constexpr int RT_X = 8, RT_Y = 4, RT_THREADS = 256;  // 32 blocks x 8 lanes
+
+// Four clamped bytes of `row` from column gx, as one dword (aligned interior
+// columns take one dword load).
+__device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
+    if (gx >= 0 && gx + 3 < W && ((reinterpret_cast<uintptr_t>(row + gx) & 3) == 0))
+        return *reinterpret_cast<const uint32_t *>(row + gx);
+    uint32_t v = 0;
 #pragma unroll
-        for (int px = 0; px < 8; ++px) s += abs((int)c[px] - (int)r[clampi(x0 + px + dx, 0, W - 1)]);
+    for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
+    return v;
+}
+
+// Block state between refinement passes: motion vector, SAD, and whether the
+// descent stopped at its iteration limit.
+struct RefState {
+    int dy, dx, sad, unconv;
+};
+
+struct RefineArgs {
+    const uint8_t *cur, *ref;
+    int H, W, pitch, R, iters;
+    int tiles_x, ntiles;  // tile grid
+    int stride4, rows;    // LDS ref region: rows x stride4 dwords
+    int ref16, cur16;     // frame base and pitch are 16-byte aligned
+};
+
+// LDS geometry of a tile's ref region: rows 8 RT_Y + 2R; a row starts at the
+// 16-byte boundary xs at or below x0 - R and holds columns up to
+// x0 + 8 RT_X - 1 + R plus one dword, so that a three-dword read at any vector
+// stays inside the row; stride4 is a multiple of 4 (whole 16-byte chunks).
+inline void region_shape(int R, int &stride4, int &rows) {
+    stride4 = (8 * RT_X + 2 * R + 4 + 15 + 15) / 16 * 4;
+    rows = 8 * RT_Y + 2 * R;
+}
+inline size_t region_lds_bytes(int R) {
+    int s4, rows;
+    region_shape(R, s4, rows);
+    return (size_t)rows * s4 * 4 + 8 * RT_Y * 8 * RT_X;
+}
+
+// Workgroups of consecutive ids run on different XCDs; give each XCD a
+// contiguous band of tiles (the grid is padded to a multiple of 8).
+__device__ inline int tile_of_workgroup() {
+    const int per = gridDim.x / 8;
+    return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+__device__ inline int floor16(int x) { return x >= 0 ? x & ~15 : -((-x + 15) & ~15); }
+
+// Stages the tile's ref region (columns from xs = floor16(x0 - R), rows from
+// y0 - R, coordinates clamped to the frame) and cur tile.  Returns x0 - R - xs.
+// Interior tiles of 16-byte-aligned frames load 16-byte chunks, all issued
+// before the first LDS store; other tiles assemble clamped bytes.
+__device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs &a, int bx0, int by0) {
+    constexpr int U = 4;
+    const int xr = bx0 * 8 - a.R, y0 = by0 * 8 - a.R, xs = floor16(xr);
+    const int s16 = a.stride4 / 4, n16 = a.rows * s16;
+    if (a.ref16 && xs >= 0 && xs + 16 * s16 <= a.W) {
+        for (int base = 0; base < n16; base += U * RT_THREADS) {
+            uint4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = base + u * RT_THREADS + threadIdx.x, r = i / s16, c = i - r * s16;
+                if (i < n16)
+                    v[u] = *reinterpret_cast<const uint4 *>(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch +
+                                                           xs + 16 * c);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = base + u * RT_THREADS + threadIdx.x;
+                if (i < n16) reinterpret_cast<uint4 *>(reg)[i] = v[u];
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < a.rows * a.stride4; i += RT_THREADS) {
+            const int r = i / a.stride4, c = i - r * a.stride4;
+            reg[i] = load4_clamped(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch, xs + 4 * c, a.W);
+        }
+    }
+    // cur tile: 8 RT_Y rows of 8 RT_X bytes; rows past the frame belong to no block
+    constexpr int C16 = 8 * RT_Y * RT_X / 2;  // 16-byte chunks
+    static_assert(C16 <= RT_THREADS, "one chunk per thread");
+    if (a.cur16 && bx0 * 8 + 8 * RT_X <= a.W) {
+        if (threadIdx.x < C16) {
+            const int r = threadIdx.x / (RT_X / 2), c = threadIdx.x - r * (RT_X / 2);
+            reinterpret_cast<uint4 *>(ctile)[threadIdx.x] = *reinterpret_cast<const uint4 *>(
+                a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch + bx0 * 8 + 16 * c);
+        }
+    } else {
+        for (int i = threadIdx.x; i < 8 * RT_Y * 2 * RT_X; i += RT_THREADS) {
+            const int r = i / (2 * RT_X), c = i - r * (2 * RT_X);
+            ctile[i] = load4_clamped(a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch, bx0 * 8 + 4 * c, a.W);
+        }
+    }
+    return xr - xs;
+}
+
+// SAD of tile block (ty, tx) at vector (dy, dx) from the staged region.
+__device__ inline uint32_t tile_sad(const uint32_t (&cw)[16], const uint32_t *reg, const RefineArgs &a, int xoff,
+                                    int ty, int tx, int dy, int dx) {
+    const int o = (8 * ty + dy + a.R) * a.stride4 * 4 + 8 * tx + dx + a.R + xoff;
+    const uint32_t *p = reg + (o >> 2);
+    const uint32_t sh = (uint32_t)(o & 3);
+    uint32_t s = 0;
+#pragma unroll
+    for (int py = 0; py < 8; ++py) {
+        const uint32_t d0 = p[py * a.stride4], d1 = p[py * a.stride4 + 1], d2 = p[py * a.stride4 + 2];
+        s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cw[2 * py], s);
+        s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cw[2 * py + 1], s);
     }
     return s;
 }
 
-// Refinement: 64 threads per block.  Lanes 0..24 evaluate the +-2 window
-// around the block's proposal (raster order), lanes 25..32 the 8 neighbours'
-// proposals.  The refined result is the first minimum of the window; the
-// block falls back to full search when that minimum lies on the window's
-// border or exceeds 1.1 x the smallest SAD found at the neighbours' proposals.
-__global__ __launch_bounds__(256) void refine_kernel(const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref,
-                                                     int H, int W, int pitch, int R, const int32_t *__restrict__ prop,
-                                                     int32_t *__restrict__ mv_sad, int32_t *__restrict__ fb_list,
-                                                     int32_t *__restrict__ fb_count, float slack) {
-    const int nbx = W / 8, nby = H / 8;
-    const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const bool active = b < nbx * nby;
-    const int by = active ? b / nbx : 0, bx = active ? b - by * nbx : 0;
-    const int py = prop[(active ? b : 0) * 2], px = prop[(active ? b : 0) * 2 + 1];
-    uint32_t key = 0xffffffffu;  // (sad << 8) | window index, window lanes only
-    int nb_sad = 0x7fffffff;
-    if (active && lane < 25) {
-        const int dy = clampi(py + lane / 5 - 2, -R, R), dx = clampi(px + lane % 5 - 2, -R, R);
-        key = ((uint32_t)sad8(cur, ref, H, W, pitch, by, bx, dy, dx) << 8) | (uint32_t)lane;
-    } else if (active && lane < 33) {
-        const int k = lane - 25;                  // neighbours in raster order, skipping the centre
-        const int o = k < 4 ? k : k + 1;
-        const int ny = by + o / 3 - 1, nx = bx + o % 3 - 1;
-        if (ny >= 0 && ny < nby && nx >= 0 && nx < nbx) {
-            const int nb = ny * nbx + nx;
-            nb_sad = sad8(cur, ref, H, W, pitch, by, bx, prop[nb * 2], prop[nb * 2 + 1]);
-        }
+__device__ inline void tile_cur(uint32_t (&cw)[16], const uint32_t *ctile, int ty, int tx) {
+#pragma unroll
+    for (int py = 0; py < 8; ++py) {
+        cw[2 * py] = ctile[(8 * ty + py) * 2 * RT_X + 2 * tx];
+        cw[2 * py + 1] = ctile[(8 * ty + py) * 2 * RT_X + 2 * tx + 1];
     }
-    for (int s = 32; s > 0; s >>= 1) {
-        key = min(key, (uint32_t)__shfl_xor((int)key, s, 64));
-        nb_sad = min(nb_sad, __shfl_xor(nb_sad, s, 64));
-    }
-    if (!active || lane != 0) return;
-    const int idx = key & 0xff, sad = (int)(key >> 8);
-    const int dy = clampi(py + idx / 5 - 2, -R, R), dx = clampi(px + idx % 5 - 2, -R, R);
-    const bool border = (idx / 5 == 0 || idx / 5 == 4 || idx % 5 == 0 || idx % 5 == 4);
-    const bool worse = (float)sad > slack * (float)min(nb_sad, sad);
-    int32_t *o = mv_sad + (size_t)b * 3;
-    o[0] = dy;
-    o[1] = dx;
-    o[2] = sad;
-    if (border || worse) fb_list[atomicAdd(fb_count, 1)] = b;
 }
 
-__global__ void zero_i32_kernel(int32_t *p) { *p = 0; }
+// Minimum over the 8 lanes of a block.
+__device__ inline uint32_t group_min(uint32_t k) {
+#pragma unroll
+    for (int m = 4; m > 0; m >>= 1) k = min(k, (uint32_t)__shfl_xor((int)k, m, 8));
+    return k;
+}
+
+// Iterated +-2 descent from (cy, cx).  Each step evaluates the 25 positions
+// clamp(c + (i / 5 - 2, i % 5 - 2), -R, R), i = 0..24 (lane l takes i = l,
+// l + 8, l + 16, l + 24), and moves to the first minimum in i order; it stops
+// when that minimum is within 1 of the centre in both axes, else repeats up
+// to `iters` steps (unconv = 1 if it never stops).  The 8 lanes call it together.
+__device__ inline RefState descend(const RefineArgs &a, const uint32_t (&cw)[16], const uint32_t *reg, int xoff,
+                                   int ty, int tx, int l, int cy, int cx) {
+    RefState st{cy, cx, 0, 1};
+    for (int t = 0; t < a.iters; ++t) {
+        uint32_t key = 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = l + 8 * j;
+            if (i < 25) {
+                const int dy = clampi(cy + i / 5 - 2, -a.R, a.R), dx = clampi(cx + i % 5 - 2, -a.R, a.R);
+                key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, dy, dx) << 5) | (uint32_t)i);
+            }
+        }
+        key = group_min(key);
+        const int i = (int)(key & 31);
+        const int ny = clampi(cy + i / 5 - 2, -a.R, a.R), nx = clampi(cx + i % 5 - 2, -a.R, a.R);
+        const bool moved = abs(ny - cy) == 2 || abs(nx - cx) == 2;
+        cy = ny;
+        cx = nx;
+        st = {cy, cx, (int)(key >> 5), 1};
+        if (!moved) {
+            st.unconv = 0;
+            break;
+        }
+    }
+    return st;
+}
+
+// Neighbour o (0..8, raster order of the 3x3 neighbourhood, 4 = the block) at distance d.
+__device__ inline bool neighbour(int by, int bx, int o, int d, int nby, int nbx, int &nb) {
+    const int ny = by + (o / 3 - 1) * d, nx = bx + (o % 3 - 1) * d;
+    nb = ny * nbx + nx;
+    return ny >= 0 && ny < nby && nx >= 0 && nx < nbx;
+}
+
+// The 8 neighbours in raster order, skipping the block itself.
+__device__ inline int ring(int k) { return k < 4 ? k : k + 1; }
+
+// Candidate c is evaluated by lane c % 8 in slot c / 8; after the group
+// minimum every lane fetches the winner's vector from that lane.
+__device__ inline void winner(uint32_t key, const int (&my)[2], const int (&mx)[2], int &cy, int &cx) {
+    const int c = (int)(key & 31);
+    const int pick = c >= 8;
+    cy = __shfl(my[pick], c & 7, 8);
+    cx = __shfl(mx[pick], c & 7, 8);
+}
+
+// Seed: the first minimum over the block's own proposal and its 8 neighbours'
+// proposals (candidates 0..8 in that order), then the descent.
+__global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, const int32_t *__restrict__ prop,
+                                                                 RefState *__restrict__ out) {
+    extern __shared__ uint32_t lds[];
+    const int tile = tile_of_workgroup();
+    if (tile >= a.ntiles) return;
+    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
+    const int nbx = a.W / 8, nby = a.H / 8;
+    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
+    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
+    __syncthreads();
+    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
+    const int by = by0 + ty, bx = bx0 + tx;
+    if (by >= nby || bx >= nbx) return;  // no barrier follows
+    uint32_t cw[16];
+    tile_cur(cw, ctile, ty, tx);
+    uint32_t key = 0xffffffffu;
+    int my[2] = {0, 0}, mx[2] = {0, 0}, nb;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int c = l + 8 * j;
+        if (c < 9 && neighbour(by, bx, c == 0 ? 4 : ring(c - 1), 1, nby, nbx, nb)) {
+            my[j] = clampi(prop[2 * nb], -a.R, a.R);
+            mx[j] = clampi(prop[2 * nb + 1], -a.R, a.R);
+            key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, my[j], mx[j]) << 5) | (uint32_t)c);
+        }
+    }
+    key = group_min(key);
+    int cy, cx;
+    winner(key, my, mx, cy, cx);
+    const RefState st = descend(a, cw, reg, xoff, ty, tx, l, cy, cx);
+    if (l == 0) out[by * nbx + bx] = st;
+}
+
+// One propagation round: each block tries the vectors of its 8 neighbours at
+// distance 1 (candidates 0..7) and, when k > 1, of its 8 neighbours at
+// distance k (candidates 8..15), from state `in`; when the first minimum
+// beats its own SAD it descends from there.  A candidate equal to the
+// block's own vector is skipped: its SAD equals the block's, which never
+// wins.  Writes every block's state to `out` and sets *changed when some
+// block improved; a round whose predecessor changed nothing does nothing.
+__global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, int k, const RefState *__restrict__ in,
+                                                                  RefState *__restrict__ out,
+                                                                  const int32_t *prev_changed, int32_t *changed) {
+    if (prev_changed && *prev_changed == 0) return;
+    extern __shared__ uint32_t lds[];
+    const int tile = tile_of_workgroup();
+    if (tile >= a.ntiles) return;
+    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
+    const int nbx = a.W / 8, nby = a.H / 8;
+    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
+    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
+    __syncthreads();
+    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
+    const int by = by0 + ty, bx = bx0 + tx;
+    bool improved = false;
+    if (by < nby && bx < nbx) {
+        const int b = by * nbx + bx;
+        uint32_t cw[16];
+        tile_cur(cw, ctile, ty, tx);
+        const RefState own = in[b];
+        uint32_t key = 0xffffffffu;
+        int my[2] = {0, 0}, mx[2] = {0, 0}, nb;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = l + 8 * j;
+            if ((j == 0 || k > 1) && neighbour(by, bx, ring(l), j ? k : 1, nby, nbx, nb)) {
+                my[j] = in[nb].dy;
+                mx[j] = in[nb].dx;
+                if (my[j] != own.dy || mx[j] != own.dx)
+                    key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, my[j], mx[j]) << 5) | (uint32_t)c);
+            }
+        }
+        key = group_min(key);
+        RefState st = own;
+        improved = (int)(key >> 5) < own.sad;  // no candidate: key >> 5 = 2^27 - 1
+        if (improved) {
+            int cy, cx;
+            winner(key, my, mx, cy, cx);
+            st = descend(a, cw, reg, xoff, ty, tx, l, cy, cx);
+        }
+        if (l == 0) out[b] = st;
+    }
+    if (__syncthreads_or(improved) && threadIdx.x == 0) *changed = 1;
+}
+
+// Final pass: the state after the last round that ran (round p reads buffer
+// p & 1 and writes (p + 1) & 1; rounds run while the previous one changed
+// something).  A block falls back to the full search when its descent never
+// stopped or when its SAD exceeds slack x the smallest SAD it has at its 8
+// neighbours' vectors.
+__global__ __launch_bounds__(RT_THREADS) void refine_final_kernel(RefineArgs a, const RefState *__restrict__ buf0,
+                                                                  const RefState *__restrict__ buf1,
+                                                                  const int32_t *__restrict__ changed, int rounds,
+                                                                  float slack, int32_t *__restrict__ mv_sad,
+                                                                  int32_t *__restrict__ fb_list,
+                                                                  int32_t *__restrict__ fb_count) {
+    extern __shared__ uint32_t lds[];
+    const int tile = tile_of_workgroup();
+    if (tile >= a.ntiles) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t zero = __ballot(lane < rounds && changed[lane] == 0);
+    const int last = zero ? __ffsll((unsigned long long)zero) - 1 : rounds - 1;
+    const RefState *fin = ((last + 1) & 1) ? buf1 : buf0;
+    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
+    const int nbx = a.W / 8, nby = a.H / 8;
+    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
+    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
+    __syncthreads();
+    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
+    const int by = by0 + ty, bx = bx0 + tx;
+    if (by >= nby || bx >= nbx) return;
+    const int b = by * nbx + bx;
+    uint32_t cw[16];
+    tile_cur(cw, ctile, ty, tx);
+    const RefState own = fin[b];
+    uint32_t m = 0xffffffffu;
+    int nb;
+    if (neighbour(by, bx, ring(l), 1, nby, nbx, nb)) m = tile_sad(cw, reg, a, xoff, ty, tx, fin[nb].dy, fin[nb].dx);
+    m = group_min(m);
+    if (l != 0) return;
+    const bool worse = (float)own.sad > slack * (float)min((uint32_t)own.sad, m);
+    int32_t *o = mv_sad + (size_t)b * 3;
+    o[0] = own.dy;
+    o[1] = own.dx;
+    o[2] = own.sad;
+    if (own.unconv || worse) fb_list[atomicAdd(fb_count, 1)] = b;
+}
+
+__global__ void refine_clear_kernel(int32_t *changed, int rounds, int32_t *fb_count) {
+    const int i = threadIdx.x;
+    if (i < rounds) changed[i] = 0;
+    if (i == 0) *fb_count = 0;
+}
 
 }  // namespace
 
@@ -558,17 +819,53 @@ umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *p
     return launch_check("proposals_kernel");
 }
 
+static size_t refine_ws_bytes(int H, int W, int rounds) {
+    return 2 * round_up((size_t)(H / 8) * (W / 8) * sizeof(RefState), 256) + round_up((size_t)(rounds + 1) * 4, 256);
+}
+
+umi_status umi_me_refine_workspace(int H, int W, int rounds, size_t *bytes) {
+    if (!bytes || H < 8 || W < 8 || rounds < 0) return set_error(UMI_ERR_INVALID, "umi_me_refine_workspace: bad args");
+    *bytes = refine_ws_bytes(H, W, rounds);
+    return UMI_OK;
+}
+
 umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R, const int32_t *prop,
-                         float slack, int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count, umi_stream s) {
-    if (!cur || !ref || !prop || !mv_sad || !fb_list || !fb_count) return set_error(UMI_ERR_INVALID, "umi_me_refine: null");
-    if (H < 8 || W < 8 || pitch < W || R < 2) return set_error(UMI_ERR_INVALID, "umi_me_refine: bad frame");
-    const int nb = (H / 8) * (W / 8);
+                         int iters, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count,
+                         void *ws, size_t ws_bytes, umi_stream s) {
+    if (!cur || !ref || !prop || !mv_sad || !fb_list || !fb_count || !ws)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine: null pointer");
+    if (H < 8 || W < 8 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_me_refine: bad frame");
+    if (R < 1 || R > 64) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_refine: radius %d (1..64)", R);
+    if (iters < 1 || iters > 64 || rounds < 0 || rounds > 64)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine: iters %d (1..64), rounds %d (0..64)", iters, rounds);
+    if (ws_bytes < refine_ws_bytes(H, W, rounds))
+        return set_error(UMI_ERR_INVALID, "umi_me_refine: workspace %zu B < %zu B", ws_bytes,
+                         refine_ws_bytes(H, W, rounds));
+    const int nbx = W / 8, nby = H / 8, nb = nbx * nby;
+    const size_t sb = round_up((size_t)nb * sizeof(RefState), 256);
+    RefState *buf[2] = {(RefState *)ws, (RefState *)((char *)ws + sb)};
+    int32_t *changed = (int32_t *)((char *)ws + 2 * sb);
+    RefineArgs a{cur, ref, H, W, pitch, R, iters, ceil_div(nbx, RT_X), 0, 0, 0, 0, 0};
+    a.ref16 = ((reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 15) == 0;
+    a.cur16 = ((reinterpret_cast<uintptr_t>(cur) | (uintptr_t)pitch) & 15) == 0;
+    a.ntiles = a.tiles_x * ceil_div(nby, RT_Y);
+    region_shape(R, a.stride4, a.rows);
+    const size_t lds = region_lds_bytes(R);
     hipStream_t st = as_stream(s);
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, st, fb_count);
-    UMI_TRY(launch_check("zero_i32_kernel"));
-    hipLaunchKernelGGL(refine_kernel, dim3(ceil_div(nb, 4)), dim3(256), 0, st, cur, ref, H, W, pitch, R, prop, mv_sad,
-                       fb_list, fb_count, slack);
-    return launch_check("refine_kernel");
+    const dim3 grid(round_up(a.ntiles, 8)), block(RT_THREADS);
+    hipLaunchKernelGGL(refine_clear_kernel, dim3(1), dim3(64), 0, st, changed, rounds, fb_count);
+    UMI_TRY(launch_check("refine_clear_kernel"));
+    hipLaunchKernelGGL(refine_seed_kernel, grid, block, lds, st, a, prop, buf[0]);
+    UMI_TRY(launch_check("refine_seed_kernel"));
+    static const int kJump[4] = {8, 4, 2, 1};
+    for (int p = 0; p < rounds; ++p) {
+        hipLaunchKernelGGL(refine_round_kernel, grid, block, lds, st, a, kJump[p & 3], buf[p & 1], buf[(p + 1) & 1],
+                           p ? changed + p - 1 : (const int32_t *)nullptr, changed + p);
+        UMI_TRY(launch_check("refine_round_kernel"));
+    }
+    hipLaunchKernelGGL(refine_final_kernel, grid, block, lds, st, a, buf[0], buf[1], changed, rounds, slack, mv_sad,
+                       fb_list, fb_count);
+    return launch_check("refine_final_kernel");
 }
 
 }  // extern "C"

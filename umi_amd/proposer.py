@@ -4,10 +4,12 @@ A 4-layer 3x3 convolution network (2 -> 32 -> 32 -> 32 -> 2 channels,
 dilations 1, 2, 4, 8, ReLU between layers; 19,682 parameters) reads the frame
 pair at quarter resolution and predicts the motion field.  The mean of each
 8x8 block's 2x2 quarter-resolution outputs, times 4, is the block's proposal.
-An exact SAD search over +-2 around the proposal refines it.  A block falls
-back to the full +-R search when the refined minimum lies on the window
-border, or when the block's SAD at one of its 8 neighbours' proposals is
-lower than the refined SAD by more than `slack` (1.1).
+Exact SAD refinement (umi_me_refine) starts from the best of the block's
+own and its 8 neighbours' proposals, descends through +-2 windows, then
+runs jump-flood propagation rounds in which blocks adopt better neighbour
+vectors.  A block falls back to the full +-R search when its descent never
+settles, or when its SAD exceeds `slack` (1.1) x the smallest SAD it has at
+its 8 neighbours' final vectors.
 
 Training runs entirely on the GPU with this project's kernels (fp32 forward,
 backward and Adam).  The labels are the plain full search's motion vectors.
@@ -235,8 +237,9 @@ def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 
 class Proposer:
     """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
 
-    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1):
+    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=8, rounds=8):
         self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
+        self.iters, self.rounds = iters, rounds
         self.H4, self.W4 = H // 4, W // 4
         px = self.H4 * self.W4
         self.nblk = (H // 8) * (W // 8)
@@ -246,6 +249,9 @@ class Proposer:
         self.prop = DeviceBuffer(self.nblk * 8)
         self.fb_list = DeviceBuffer(self.nblk * 4)
         self.fb_count = DeviceBuffer(4)
+        n = C.c_size_t()
+        call("umi_me_refine_workspace", H, W, rounds, C.byref(n))
+        self.ws = DeviceBuffer(n.value)
         # bf16 KRSC filters of the two MFMA layers (rounded from the fp32 masters)
         self.wk = []
         for li in (1, 2):
@@ -266,10 +272,14 @@ class Proposer:
              self.flow.ptr, s)
         call("umi_me_proposals", self.flow.ptr, self.H4, self.W4, self.R, self.prop.ptr, s)
 
+    def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
+        call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, self.iters,
+             self.rounds, C.c_float(self.slack), mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr, self.ws.ptr,
+             self.ws.nbytes, _s(stream))
+
     def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         s = _s(stream)
-        call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, C.c_float(self.slack),
-             mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr, s)
+        self.refine_only(cur, ref, mv_sad, stream)
         call("umi_me_full_search_listed_u8", cur.ptr, ref.ptr, self.H, self.W, self.W, 8, self.R, self.fb_list.ptr,
              self.fb_count.ptr, self.nblk, mv_sad.ptr, s)
 
@@ -285,6 +295,14 @@ class Proposer:
     def fallback_count(self) -> int:
         return int(self.fb_count.download((1,), np.int32)[0])
 
+    def rounds_run(self) -> int:
+        """Propagation rounds of the last refinement that changed some block."""
+        nb = self.nblk * 16
+        off = 2 * ((nb + 255) // 256 * 256)
+        flags = np.frombuffer(self.ws.download((self.ws.nbytes,), np.uint8)[off:off + 4 * self.rounds].tobytes(),
+                              np.int32)
+        return int(np.count_nonzero(flags))
+
 
 def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10):
     """Fraction of blocks at the full search's minimum SAD, fallback fraction,
@@ -295,7 +313,7 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
     ref, cur = DeviceBuffer(H * W), DeviceBuffer(H * W)
     mv, judge = DeviceBuffer(prop.nblk * 12), DeviceBuffer(prop.nblk * 12)
     st = Stream()
-    at_min, fb = [], []
+    at_min, fb, rounds = [], [], []
     times = {"pipeline": [], "network": [], "refine": [], "full_search": []}
 
     def timed(fn):
@@ -317,15 +335,14 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
         j = judge.download((prop.nblk, 3), np.int32)
         at_min.append(float(np.mean(a[:, 2] == j[:, 2])))
         fb.append(prop.fallback_count() / prop.nblk)
+        rounds.append(prop.rounds_run())
         times["pipeline"].append(timed(lambda: prop.run(cur, ref, mv, st, proposals)))
         if proposals:
             times["network"].append(timed(lambda: prop.propose(cur, ref, st)))
-        times["refine"].append(timed(lambda: call("umi_me_refine", cur.ptr, ref.ptr, H, W, W, R, prop.prop.ptr,
-                                                  C.c_float(prop.slack), mv.ptr, prop.fb_list.ptr,
-                                                  prop.fb_count.ptr, st.handle)))
+        times["refine"].append(timed(lambda: prop.refine_only(cur, ref, mv, st)))
         times["full_search"].append(timed(lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)))
     out = {"at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
-           "fallback_frac": float(np.mean(fb))}
+           "fallback_frac": float(np.mean(fb)), "rounds_changed": float(np.mean(rounds))}
     out.update({f"ms_{k}": float(np.mean(v)) for k, v in times.items() if v})
     out["speedup"] = out["ms_full_search"] / out["ms_pipeline"]
     return out
@@ -347,6 +364,7 @@ def main(argv=None):
     ap.add_argument("--seed0", type=int, default=0, help="first training pair seed")
     ap.add_argument("--eval-seed0", type=int, default=1_000_000, help="first held-out pair seed")
     ap.add_argument("--ckpt", default=str(CHECKPOINT))
+    ap.add_argument("--rounds", type=int, default=8, help="propagation rounds of the refinement")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
@@ -363,10 +381,13 @@ def main(argv=None):
         print(json.dumps({"loss_history": hist, "checkpoint": out}), flush=True)
     else:
         net = Network.load(a.ckpt)
-        prop = Proposer(net)
+        prop = Proposer(net, rounds=a.rounds)
         with_net = evaluate(prop, a.pairs, a.eval_seed0, proposals=True)
         without = evaluate(prop, a.pairs, a.eval_seed0, proposals=False)
-        print(json.dumps({"with_network": with_net, "without_network": without}), flush=True)
+        long = Proposer(net, rounds=32)
+        without_32 = evaluate(long, a.pairs, a.eval_seed0, proposals=False)
+        print(json.dumps({"with_network": with_net, "without_network": without,
+                          "without_network_32_rounds": without_32}), flush=True)
 
 
 if __name__ == "__main__":

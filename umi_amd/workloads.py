@@ -48,6 +48,24 @@ def me_pair(H: int = 1080, W: int = 1920, R: int = 16, tile: int = 16, sigma: fl
     return f1, f0, mv
 
 
+def smooth_motion_pair(H: int, W: int, R: int = 16, sigma: float = 2.0, seed: int = 1):
+    """Frame pair with a smooth affine motion field: frame1(y, x) =
+    clip(round(frame0[cl(y+my)][cl(x+mx)] + N(0, sigma^2))) with
+    (my, mx) = round(t + A (y/H - 1/2, x/W - 1/2)), |t| <= R/2, |A| <= R/2.
+    Returns (cur=frame1, ref=frame0)."""
+    g = rng(seed)
+    f0 = smooth_noise_u8(H, W, g)
+    t = g.uniform(-R / 2, R / 2, size=2)
+    A = g.uniform(-R / 2, R / 2, size=(2, 2))
+    yy, xx = np.mgrid[0:H, 0:W]
+    u, v = yy / H - 0.5, xx / W - 0.5
+    my = np.clip(np.rint(t[0] + A[0, 0] * u + A[0, 1] * v), -R, R).astype(np.int64)
+    mx = np.clip(np.rint(t[1] + A[1, 0] * u + A[1, 1] * v), -R, R).astype(np.int64)
+    src = f0[np.clip(yy + my, 0, H - 1), np.clip(xx + mx, 0, W - 1)].astype(np.float64)
+    f1 = np.clip(np.rint(src + g.normal(0.0, sigma, size=(H, W))), 0, 255).astype(np.uint8)
+    return f1, f0
+
+
 def stereo_pair(H: int = 375, W: int = 1242, D: int = 128, tile: int = 32, sigma: float = 2.0, seed: int = 1):
     """Left/right pair: R[y][x] = L[y][cl(x + d)] + N(0, sigma^2), d piecewise
     constant over ``tile`` x ``tile`` tiles, uniform in [0, D-1]."""
