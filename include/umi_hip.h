@@ -294,17 +294,17 @@ umi_status umi_cnn_huber(const float *pred, const int32_t *labels, int N, int H4
                          float *loss_parts, int *nparts, umi_stream s);
 umi_status umi_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
                     int step, umi_stream s);
-/* Inference around the bf16 MFMA layers (umi_conv_nhwc_bf16): first layer
- * (2 -> 32, fp32 NCHW in, bf16 NHWC out, ReLU), last layer (32 -> 2, bf16
- * NHWC in, fp32 flow out); weights are rounded to bf16. */
-umi_status umi_cnn_first_layer(const float *in, int N, int H, int W, int dil, const float *w, const float *bias,
-                               uint16_t *out_nhwc, umi_stream s);
-umi_status umi_cnn_last_layer(const uint16_t *in_nhwc, int N, int H, int W, int dil, const float *w,
-                              const float *bias, float *flow, umi_stream s);
-/* Dilated 3x3 convolution on bf16 NHWC activations with a packed bf16 KRSC
- * filter (umi_conv_pack_filter), + bias, optional ReLU, bf16 NHWC out. */
-umi_status umi_conv_nhwc_bf16(int N, int C, int H, int W, int K, int R, int S, int pad, int dil, const void *in,
-                              const void *w, const float *bias, int relu, void *out, umi_stream s);
+/* One inference layer of the proposal network on bf16 MFMA (3x3, dilation
+ * dil, zero padding dil; fp32 accumulation; inputs rounded to bf16):
+ *   kind 0: 2 -> 32, fp32 NCHW in, bf16 NHWC out, ReLU;
+ *   kind 1: 32 -> 32, bf16 NHWC in and out, ReLU;
+ *   kind 2: 32 -> 2, bf16 NHWC in, fp32 NCHW out (the flow), no ReLU.
+ * umi_cnn_pack_layer rounds the fp32 weights [cout][cin][3][3] to bf16 in
+ * the MFMA operand order, into umi_cnn_packed_bytes(kind) bytes. */
+umi_status umi_cnn_packed_bytes(int kind, size_t *bytes);
+umi_status umi_cnn_pack_layer(int kind, const float *w, void *packed, umi_stream s);
+umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, int dil, const void *packed,
+                               const float *bias, void *out, umi_stream s);
 /* Block proposals from the quarter-resolution flow: round(4 x mean of the
  * block's 2x2 outputs), clamped to [-R, R]; prop [H4/2][W4/2][2]. */
 umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s);

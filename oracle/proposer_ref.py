@@ -21,6 +21,8 @@ def shift(x, dy, dx):
     """y[..., i, j] = x[..., i + dy, j + dx], zero outside."""
     out = np.zeros_like(x)
     H, W = x.shape[-2:]
+    if abs(dy) >= H or abs(dx) >= W:
+        return out
     ys, yd = (slice(dy, H), slice(0, H - dy)) if dy >= 0 else (slice(0, H + dy), slice(-dy, H))
     xs, xd = (slice(dx, W), slice(0, W - dx)) if dx >= 0 else (slice(0, W + dx), slice(-dx, W))
     out[..., yd, xd] = x[..., ys, xs]
@@ -50,6 +52,26 @@ def forward(x, params, layers):
             a = np.maximum(a, 0.0)
         acts.append(a)
     return acts
+
+
+def bf16(x):
+    """Round to bfloat16 (via float32, round to nearest even), as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def forward_bf16(x, params, layers):
+    """Inference form of the network: inputs, weights and the activations
+    between layers rounded to bf16, sums in float64; returns the last output."""
+    a = bf16(x)
+    for li, ((w, b), (_, _, d, relu)) in enumerate(zip(params, layers)):
+        a = conv(a, bf16(w), np.asarray(b, np.float64), d)
+        if relu:
+            a = np.maximum(a, 0.0)
+        if li < len(layers) - 1:
+            a = bf16(a)
+    return a
 
 
 def huber(pred, labels):

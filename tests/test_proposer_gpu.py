@@ -4,6 +4,8 @@
   match the FP64 oracle (oracle/proposer_ref.py, pinned by finite
   differences) on a small input;
 * Adam matches its formula; training lowers the loss;
+* the bf16 MFMA inference layers match the FP64 forward pass with the same
+  bf16 roundings;
 * refinement (seed, descent, propagation rounds, fallback check) matches the
   oracle exactly, including ragged frames, padded rows and edge settings; the
   listed full search equals the full search on the listed blocks and leaves
@@ -77,6 +79,43 @@ def test_training_lowers_loss(gpu):
     hist = proposer.train(net, data, epochs=32, batch=4, lr=2e-3)
     print(f"[proposer] loss history {hist}", flush=True)
     assert hist[-1] < 0.85 * hist[0], hist
+
+
+@pytest.mark.parametrize("N,H4,W4", [(1, 24, 40), (2, 18, 37), (1, 7, 5)])
+def test_inference_layers_match_bf16_oracle(gpu, N, H4, W4):
+    """The four bf16 MFMA inference layers against the FP64 forward pass with
+    the same bf16 roundings (weights, inputs, activations between layers).
+    The GPU sums in fp32 in MFMA order; the tolerance, 2e-3 x max |flow|,
+    covers that and the rare activations whose bf16 rounding it flips."""
+    g = workloads.rng(N * 100 + H4 + W4)
+    net = proposer.Network(seed=4)
+    for li in range(len(proposer.LAYERS)):   # nonzero biases exercise the bias path
+        b = g.standard_normal(proposer.LAYERS[li][1]).astype(np.float32) * 0.1
+        net.b[li].upload(b)
+    params = net.params()
+    x = g.standard_normal((N, 2, H4, W4)).astype(np.float32)
+    dx = DeviceBuffer.from_array(x)
+    acts = [DeviceBuffer(N * H4 * W4 * 32 * 2) for _ in range(3)]
+    flow = DeviceBuffer(N * 2 * H4 * W4 * 4)
+    packed = [proposer.pack_layer(net, li) for li in range(len(proposer.LAYERS))]
+    src = dx
+    for li, (ci, co, d, _) in enumerate(proposer.LAYERS):
+        kind = proposer.layer_kind(li)
+        dst = flow if kind == 2 else acts[li]
+        print(f"[infer] N={N} H4={H4} W4={W4} layer {li} kind {kind}", flush=True)
+        call("umi_cnn_infer_layer", kind, src.ptr, N, H4, W4, d, packed[li].ptr, net.b[li].ptr, dst.ptr, None)
+        src = dst
+    got = flow.download((N, 2, H4, W4), np.float32)
+    want = PR.forward_bf16(x, params, proposer.LAYERS)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"[infer] max |err| / max |flow| = {err:.2e}", flush=True)
+    assert err <= 2e-3
+    # the first layer alone is exact up to fp32 summation order
+    call("umi_cnn_infer_layer", 0, dx.ptr, N, H4, W4, 1, packed[0].ptr, net.b[0].ptr, acts[0].ptr, None)
+    a0 = acts[0].download((N, H4, W4, 32), np.uint16).astype(np.uint32) << 16
+    a0 = a0.view(np.float32).transpose(0, 3, 1, 2)
+    w0 = PR.bf16(PR.conv(PR.bf16(x), PR.bf16(params[0][0]), params[0][1], 1).clip(0))
+    assert np.abs(a0 - w0).max() <= 2 ** -7 * np.abs(w0).max()
 
 
 def _refine_gpu(cur, ref, prop, R, iters, rounds, slack, pitch=None):

@@ -143,12 +143,6 @@ struct DenseDst {
 };
 // NHWC bf16 output with per-channel bias and optional ReLU (the layer
 // output of the motion-proposal network, fed to the next layer as is).
-struct NhwcBf16Dst {
-    uint16_t *base;  // [pixels][K]
-    const float *bias;
-    int K, total, relu;
-    __device__ void put(int m, int j, float v) const;
-};
 struct ConvDst {
     float *base;  // [N][K][OH][OW]
     int K, npix, total;
@@ -159,15 +153,6 @@ struct ConvDst {
         }
     }
 };
-
-__device__ inline uint16_t f32_to_bf16_rne(float f);
-__device__ void NhwcBf16Dst::put(int m, int j, float v) const {
-    if (m < K && j < total) {
-        v += bias[m];
-        if (relu) v = fmaxf(v, 0.f);
-        base[(long)j * K + m] = f32_to_bf16_rne(v);
-    }
-}
 
 // 16 bytes of one panel row (zeros where the source row is absent).
 __device__ inline i32x4 fetch16(const uint8_t *p) {
@@ -549,22 +534,6 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
         case GM_FP8: return gemm_launch<GM_FP8>(ah, al, bh, bl, d, K, NP, KG, qw, qi, st);
         default: return gemm_launch<GM_FP8X3>(ah, al, bh, bl, d, K, NP, KG, qw, qi, st);
     }
-}
-
-umi_status umi_conv_nhwc_bf16(int N, int C, int H, int W, int K, int R, int S, int pad, int dil, const void *in,
-                              const void *w, const float *bias, int relu, void *out, umi_stream s) {
-    if (!in || !w || !bias || !out) return set_error(UMI_ERR_INVALID, "umi_conv_nhwc_bf16: null pointer");
-    if (N < 0 || C < 1 || H < 1 || W < 1 || K < 1 || R < 1 || S < 1 || pad < 0 || dil < 1)
-        return set_error(UMI_ERR_INVALID, "umi_conv_nhwc_bf16: bad shape");
-    const int OH = H + 2 * pad - dil * (R - 1), OW = W + 2 * pad - dil * (S - 1);
-    const int KG = R * S * C;
-    if ((C * 2) % 16 != 0 || (KG * 2) % ROWB != 0)
-        return set_error(UMI_ERR_UNSUPPORTED, "umi_conv_nhwc_bf16: C*2 must be a multiple of 16 B and R*S*C*2 of 64 B");
-    if (OH < 1 || OW < 1 || N == 0) return UMI_OK;
-    const DenseSrc ah{(const uint8_t *)w, (long)KG * 2, K};
-    const ConvSrc bh{(const uint8_t *)in, N, H, W, C, OH, OW, R, S, 1, pad, dil};
-    const NhwcBf16Dst d{(uint16_t *)out, bias, K, N * OH * OW, relu};
-    return gemm_launch<GM_BF16>(ah, ah, bh, bh, d, K, N * OH * OW, KG, nullptr, nullptr, as_stream(s));
 }
 
 }  // extern "C"

@@ -7,11 +7,13 @@
 //    plus N(0, sigma^2) noise, from a counter-based hash (deterministic);
 //  * the proposal network's training kernels in fp32 (forward, backward to
 //    inputs and to weights/biases, Huber loss, Adam): NCHW, 3x3 dilated;
-//  * the inference kernels around the bf16 MFMA layers (gemm.hip):
-//    downsampling, the 2->32 first layer and 32->2 last layer, block
-//    proposals, the exact +-2 SAD refinement with the neighbourhood test and
-//    the fallback list.
+//  * inference: downsampling, the four layers on bf16 MFMA, block
+//    proposals;
+//  * the exact SAD refinement of the proposals (seed, +-2 descent,
+//    propagation rounds, fallback check) and the fallback list.
 #include "common.h"
+
+#include <algorithm>
 
 namespace {
 using namespace umi;
@@ -286,73 +288,130 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 __device__ inline float bf16f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
 
-// First layer (2 -> 32, dilation d) from the fp32 NCHW input to NHWC bf16,
-// bias + ReLU (weights are the bf16-rounded training weights).
-__global__ __launch_bounds__(256) void first_layer_kernel(const float *__restrict__ in, int N, int H, int W, int d,
-                                                          const float *__restrict__ w, const float *__restrict__ bias,
-                                                          uint16_t *__restrict__ out) {
-    __shared__ float wl[2 * 9 * 32];
-    for (int i = threadIdx.x; i < 2 * 9 * 32; i += blockDim.x) {
-        const int co = i % 32, t = (i / 32) % 9, ci = i / (9 * 32);
-        wl[i] = bf16f(bf16_rne(w[(co * 2 + ci) * 9 + t]));
-    }
-    __syncthreads();
-    const long plane = (long)H * W;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N * plane) return;
-    const int n = (int)(p / plane), rem = (int)(p - n * plane), y = rem / W, x = rem - y * W;
-    float acc[32];
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+
+// Inference layers of the proposal network on v_mfma_f32_32x32x16_bf16,
+// out[co][pixel] = sum_k W[co][k] x in[k][pixel]:
+//   CNN_FIRST  2 -> 32: fp32 NCHW input, k = 2 t + ci over the 9 taps t (18
+//              of 32 k, two chunks of 16), bf16 NHWC output, ReLU;
+//   CNN_MID   32 -> 32: bf16 NHWC input, chunk 2 t + h = tap t, channels
+//              16 h .. 16 h + 15 (18 chunks), bf16 NHWC output, ReLU;
+//   CNN_LAST  32 -> 2:  as CNN_MID with output rows 2..31 zero, fp32 NCHW out.
+// Operand layout (per 16-k chunk): lane l holds A row l & 31 (output
+// channel) and B column l & 31 (pixel), each for k = 8 (l >> 5) .. + 7;
+// accumulator r of lane l is (row 8 (r >> 2) + 4 (l >> 5) + (r & 3), column
+// l & 31).  Weights and inputs are rounded to bf16 (RNE); zero padding d.
+// Each wave walks 32-pixel groups (pixels in N, y, x order).
+enum { CNN_FIRST = 0, CNN_MID = 1, CNN_LAST = 2 };
+
+// The A operands, packed once per layer: wp[c * 64 + lane] (16 B each).
+template <int MODE>
+__global__ void cnn_pack_kernel(const float *__restrict__ w, bf16x8 *__restrict__ wp) {
+    constexpr int NCH = MODE == CNN_FIRST ? 2 : 18;
+    constexpr int COUT = MODE == CNN_LAST ? 2 : 32;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NCH * 64) return;
+    const int c = i / 64, lane = i % 64, lr = lane & 31, lh = lane >> 5;
+    uint16_t v[8];
 #pragma unroll
-    for (int c = 0; c < 32; ++c) acc[c] = bias[c];
-    for (int ci = 0; ci < 2; ++ci)
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int yy = y + (t / 3 - 1) * d, xx = x + (t % 3 - 1) * d;
-            if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
-            const float v = bf16f(bf16_rne(in[((long)n * 2 + ci) * plane + (long)yy * W + xx]));
-#pragma unroll
-            for (int c = 0; c < 32; ++c) acc[c] = __builtin_fmaf(v, wl[(ci * 9 + t) * 32 + c], acc[c]);
+    for (int j = 0; j < 8; ++j) {
+        float x = 0.f;
+        if (lr < COUT) {
+            if constexpr (MODE == CNN_FIRST) {
+                const int k = 16 * c + 8 * lh + j;
+                if (k < 18) x = w[(lr * 2 + (k & 1)) * 9 + (k >> 1)];
+            } else {
+                x = w[(lr * 32 + 16 * (c & 1) + 8 * lh + j) * 9 + (c >> 1)];
+            }
         }
-    uint16_t *o = out + p * 32;
-#pragma unroll
-    for (int c = 0; c < 32; ++c) o[c] = bf16_rne(fmaxf(acc[c], 0.f));
+        v[j] = bf16_rne(x);
+    }
+    wp[i] = __builtin_bit_cast(bf16x8, v);
 }
 
-// Last layer (32 -> 2, dilation d) from NHWC bf16 to the fp32 flow [n][2][H][W], no ReLU.
-__global__ __launch_bounds__(256) void last_layer_kernel(const uint16_t *__restrict__ in, int N, int H, int W, int d,
-                                                         const float *__restrict__ w, const float *__restrict__ bias,
-                                                         float *__restrict__ flow) {
-    __shared__ float wl[32 * 9 * 2];
-    for (int i = threadIdx.x; i < 32 * 9 * 2; i += blockDim.x) {
-        const int co = i % 2, t = (i / 2) % 9, ci = i / 18;
-        wl[i] = bf16f(bf16_rne(w[(co * 32 + ci) * 9 + t]));
+template <int MODE>
+__global__ __launch_bounds__(256) void cnn_layer_kernel(const void *__restrict__ in_, int N, int H, int W, int d,
+                                                        const bf16x8 *__restrict__ wp,
+                                                        const float *__restrict__ bias, void *__restrict__ out_) {
+    constexpr int NCH = MODE == CNN_FIRST ? 2 : 18;
+    constexpr int COUT = MODE == CNN_LAST ? 2 : 32;
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+    bf16x8 A[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) A[c] = wp[c * 64 + lane];
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * lh + (r & 3);
+        bv[r] = m < COUT ? bias[m] : 0.f;
     }
-    __syncthreads();
-    const long plane = (long)H * W;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N * plane) return;
-    const int n = (int)(p / plane), rem = (int)(p - n * plane), y = rem / W, x = rem - y * W;
-    float a0 = bias[0], a1 = bias[1];
+    const long plane = (long)H * W, npx = (long)N * plane;
+    const long ngroups = (npx + 31) / 32;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    for (long g = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += nwaves) {
+        const long p = g * 32 + lr;
+        const bool valid = p < npx;
+        const int n = valid ? (int)(p / plane) : 0;
+        const int rem = valid ? (int)(p - n * plane) : 0, y = rem / W, x = rem - y * W;
+        bf16x8 B[NCH];
+        if constexpr (MODE == CNN_FIRST) {
+            const float *in = (const float *)in_;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int yy = y + (t / 3 - 1) * d, xx = x + (t % 3 - 1) * d;
-        if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
-        const uint4 *src = (const uint4 *)(in + ((long)n * plane + (long)yy * W + xx) * 32);
+            for (int c = 0; c < NCH; ++c) {
+                uint16_t v[8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint4 u = src[q];
-            const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+                for (int j = 0; j < 8; ++j) {
+                    const int k = 16 * c + 8 * lh + j, t = k >> 1;
+                    const int yy = y + (t / 3 - 1) * d, xx = x + (t % 3 - 1) * d;
+                    float val = 0.f;
+                    if (k < 18 && valid && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        val = in[((long)n * 2 + (k & 1)) * plane + (long)yy * W + xx];
+                    v[j] = bf16_rne(val);
+                }
+                B[c] = __builtin_bit_cast(bf16x8, v);
+            }
+        } else {
+            const uint16_t *in = (const uint16_t *)in_;
 #pragma unroll
-            for (int h = 0; h < 8; ++h) {
-                const int ci = q * 8 + h;
-                const float v = bf16f((uint16_t)(wd[h >> 1] >> (16 * (h & 1))));
-                a0 = __builtin_fmaf(v, wl[(ci * 9 + t) * 2 + 0], a0);
-                a1 = __builtin_fmaf(v, wl[(ci * 9 + t) * 2 + 1], a1);
+            for (int t = 0; t < 9; ++t) {
+                const int yy = y + (t / 3 - 1) * d, xx = x + (t % 3 - 1) * d;
+                u32x4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+                if (valid && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const u32x4 *src = (const u32x4 *)(in + (((long)n * plane + (long)yy * W + xx) * 32 + 8 * lh));
+                    v0 = src[0];
+                    v1 = src[2];  // channels 16 + 8 lh ..
+                }
+                B[2 * t] = __builtin_bit_cast(bf16x8, v0);
+                B[2 * t + 1] = __builtin_bit_cast(bf16x8, v1);
+            }
+        }
+        f32x16 acc = {};
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[c], B[c], acc, 0, 0, 0);
+        if (!valid) continue;
+        if constexpr (MODE == CNN_LAST) {
+            if (lh == 0) {
+                float *flow = (float *)out_;
+                flow[((long)n * 2 + 0) * plane + rem] = acc[0] + bv[0];
+                flow[((long)n * 2 + 1) * plane + rem] = acc[1] + bv[1];
+            }
+        } else {
+            uint16_t *o = (uint16_t *)out_ + p * 32;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t h[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
+                    const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
+                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+                }
+                *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
             }
         }
     }
-    flow[((long)n * 2 + 0) * plane + rem] = a0;
-    flow[((long)n * 2 + 1) * plane + rem] = a1;
 }
 
 // Block proposals: mean of the block's 2x2 quarter-resolution flow, x4,
@@ -794,22 +853,50 @@ umi_status umi_adam(float *w, const float *g, float *m, float *v, int n, float l
     return launch_check("adam_kernel");
 }
 
-umi_status umi_cnn_first_layer(const float *in, int N, int H, int W, int dil, const float *w, const float *bias,
-                               uint16_t *out_nhwc, umi_stream s) {
-    if (!in || !w || !bias || !out_nhwc) return set_error(UMI_ERR_INVALID, "umi_cnn_first_layer: null");
-    const long total = (long)N * H * W;
-    hipLaunchKernelGGL(first_layer_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, as_stream(s), in, N, H,
-                       W, dil, w, bias, out_nhwc);
-    return launch_check("first_layer_kernel");
+static int cnn_chunks(int kind) { return kind == CNN_FIRST ? 2 : 18; }
+
+umi_status umi_cnn_packed_bytes(int kind, size_t *bytes) {
+    if (!bytes || kind < CNN_FIRST || kind > CNN_LAST) return set_error(UMI_ERR_INVALID, "umi_cnn_packed_bytes: kind %d", kind);
+    *bytes = (size_t)cnn_chunks(kind) * 64 * 16;
+    return UMI_OK;
 }
 
-umi_status umi_cnn_last_layer(const uint16_t *in_nhwc, int N, int H, int W, int dil, const float *w, const float *bias,
-                              float *flow, umi_stream s) {
-    if (!in_nhwc || !w || !bias || !flow) return set_error(UMI_ERR_INVALID, "umi_cnn_last_layer: null");
-    const long total = (long)N * H * W;
-    hipLaunchKernelGGL(last_layer_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, as_stream(s), in_nhwc, N,
-                       H, W, dil, w, bias, flow);
-    return launch_check("last_layer_kernel");
+umi_status umi_cnn_pack_layer(int kind, const float *w, void *packed, umi_stream s) {
+    if (!w || !packed) return set_error(UMI_ERR_INVALID, "umi_cnn_pack_layer: null pointer");
+    hipStream_t st = as_stream(s);
+    const dim3 grid(ceil_div(cnn_chunks(kind) * 64, 256)), block(256);
+    switch (kind) {
+        case CNN_FIRST: hipLaunchKernelGGL(cnn_pack_kernel<CNN_FIRST>, grid, block, 0, st, w, (bf16x8 *)packed); break;
+        case CNN_MID: hipLaunchKernelGGL(cnn_pack_kernel<CNN_MID>, grid, block, 0, st, w, (bf16x8 *)packed); break;
+        case CNN_LAST: hipLaunchKernelGGL(cnn_pack_kernel<CNN_LAST>, grid, block, 0, st, w, (bf16x8 *)packed); break;
+        default: return set_error(UMI_ERR_INVALID, "umi_cnn_pack_layer: kind %d (0, 1, 2)", kind);
+    }
+    return launch_check("cnn_pack_kernel");
+}
+
+umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, int dil, const void *packed,
+                               const float *bias, void *out, umi_stream s) {
+    if (!in || !packed || !bias || !out) return set_error(UMI_ERR_INVALID, "umi_cnn_infer_layer: null pointer");
+    if (N < 0 || H < 1 || W < 1 || dil < 1) return set_error(UMI_ERR_INVALID, "umi_cnn_infer_layer: bad shape");
+    const long groups = ((long)N * H * W + 31) / 32;
+    if (groups == 0) return UMI_OK;
+    // persistent waves: two workgroups of 4 waves per CU
+    const unsigned grid = (unsigned)std::min<long>((groups + 3) / 4, 256L * 2);
+    hipStream_t st = as_stream(s);
+    const bf16x8 *wp = (const bf16x8 *)packed;
+    switch (kind) {
+        case CNN_FIRST:
+            hipLaunchKernelGGL(cnn_layer_kernel<CNN_FIRST>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
+            break;
+        case CNN_MID:
+            hipLaunchKernelGGL(cnn_layer_kernel<CNN_MID>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
+            break;
+        case CNN_LAST:
+            hipLaunchKernelGGL(cnn_layer_kernel<CNN_LAST>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
+            break;
+        default: return set_error(UMI_ERR_INVALID, "umi_cnn_infer_layer: kind %d (0, 1, 2)", kind);
+    }
+    return launch_check("cnn_layer_kernel");
 }
 
 umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s) {

@@ -13,7 +13,8 @@ its 8 neighbours' final vectors.
 
 Training runs entirely on the GPU with this project's kernels (fp32 forward,
 backward and Adam).  The labels are the plain full search's motion vectors.
-Inference runs the 32 -> 32 layers as bf16 MFMA convolutions.
+Inference runs all four layers on bf16 MFMA (weights, inputs and activations
+rounded to bf16, fp32 accumulation).
 """
 from __future__ import annotations
 
@@ -234,6 +235,20 @@ def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 
 
 
 # ---------------------------------------------------------------- inference
+def layer_kind(li: int) -> int:
+    """umi_cnn_infer_layer kind of layer li: 0 first, 2 last, 1 between."""
+    return 0 if li == 0 else (2 if li == len(LAYERS) - 1 else 1)
+
+
+def pack_layer(net: Network, li: int, stream=None) -> DeviceBuffer:
+    """Layer li's weights rounded to bf16 in the MFMA operand order."""
+    n = C.c_size_t()
+    call("umi_cnn_packed_bytes", layer_kind(li), C.byref(n))
+    buf = DeviceBuffer(n.value)
+    call("umi_cnn_pack_layer", layer_kind(li), net.w[li].ptr, buf.ptr, _s(stream))
+    return buf
+
+
 class Proposer:
     """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
 
@@ -249,27 +264,21 @@ class Proposer:
         self.prop = DeviceBuffer(self.nblk * 8)
         self.fb_list = DeviceBuffer(self.nblk * 4)
         self.fb_count = DeviceBuffer(4)
+        self.packed = [pack_layer(net, li) for li in range(len(LAYERS))]
         n = C.c_size_t()
         call("umi_me_refine_workspace", H, W, rounds, C.byref(n))
         self.ws = DeviceBuffer(n.value)
-        # bf16 KRSC filters of the two MFMA layers (rounded from the fp32 masters)
-        self.wk = []
-        for li in (1, 2):
-            op = ops.Operand(32, 9 * 32, ops.BF16)
-            call("umi_conv_pack_filter", net.w[li].ptr, 32, 32, 3, 3, ops.BF16, None, op.hi.ptr, None, None, None)
-            self.wk.append(op)
 
     def propose(self, cur: DeviceBuffer, ref: DeviceBuffer, stream=None):
         s = _s(stream)
         call("umi_cnn_downsample_pair", cur.ptr, ref.ptr, self.H, self.W, self.W, self.x.ptr, 0, s)
-        call("umi_cnn_first_layer", self.x.ptr, 1, self.H4, self.W4, 1, self.net.w[0].ptr, self.net.b[0].ptr,
-             self.a[0].ptr, s)
-        call("umi_conv_nhwc_bf16", 1, 32, self.H4, self.W4, 32, 3, 3, 2, 2, self.a[0].ptr, self.wk[0].hi.ptr,
-             self.net.b[1].ptr, 1, self.a[1].ptr, s)
-        call("umi_conv_nhwc_bf16", 1, 32, self.H4, self.W4, 32, 3, 3, 4, 4, self.a[1].ptr, self.wk[1].hi.ptr,
-             self.net.b[2].ptr, 1, self.a[2].ptr, s)
-        call("umi_cnn_last_layer", self.a[2].ptr, 1, self.H4, self.W4, 8, self.net.w[3].ptr, self.net.b[3].ptr,
-             self.flow.ptr, s)
+        src = self.x
+        for li, (ci, co, d, _) in enumerate(LAYERS):
+            kind = layer_kind(li)
+            dst = self.flow if kind == 2 else self.a[li]
+            call("umi_cnn_infer_layer", kind, src.ptr, 1, self.H4, self.W4, d, self.packed[li].ptr,
+                 self.net.b[li].ptr, dst.ptr, s)
+            src = dst
         call("umi_me_proposals", self.flow.ptr, self.H4, self.W4, self.R, self.prop.ptr, s)
 
     def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
