@@ -309,29 +309,33 @@ umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, in
  * block's 2x2 outputs), clamped to [-R, R]; prop [H4/2][W4/2][2]. */
 umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s);
 /* Exact refinement of the block proposals (no counterpart in the reference:
- * it serves the learned proposer).  Per 8x8 block, with every position
- * clamped to [-R, R]:
- *   seed   - the first minimum-SAD vector among the block's own proposal and
- *            its 8 neighbours' proposals (raster order);
+ * it serves the learned proposer).  Per 8x8 block, with every vector clamped
+ * to [-R, R]:
+ *   seed    - the first minimum-SAD vector among the block's own proposal and
+ *             its 8 neighbours' proposals (raster order);
  *   descent - repeat up to `iters` times: evaluate the 5x5 window around the
- *            centre, move to its first minimum (raster order), stop when that
- *            minimum is within 1 of the centre in both axes;
- *   rounds - up to `rounds` propagation rounds with jump distance
- *            k = 8, 4, 2, 1, 8, ...: each block tries its neighbours' vectors
- *            at distance 1 and k and descends from the first minimum when it
- *            beats its own SAD; the rounds stop after one that changed nothing;
- *   check  - a block falls back when its descent never stopped, or when its
- *            SAD exceeds slack x the smallest SAD it has at its 8 neighbours'
- *            final vectors.
+ *             centre, move to its first minimum (raster order), stop when that
+ *             minimum is within 1 of the centre in both axes;
+ *   in-tile - up to `local_rounds` rounds inside tiles of 4 x 8 blocks
+ *             (aligned to the block grid), distance k = 4, 2, 1, 4, ...: each
+ *             block tries its in-tile neighbours' vectors at distance 1 and k
+ *             and descends from the first minimum when it beats its own SAD;
+ *             a tile stops after a round without a change;
+ *   rounds  - up to `rounds` such rounds over the whole frame with
+ *             k = 8, 4, 2, 1, 8, ..., stopping after one without a change;
+ *   check   - a block falls back when its descent never stopped, or when its
+ *             SAD exceeds slack x the smallest SAD it has at its 8 neighbours'
+ *             final vectors.
  * mv_sad [H/8][W/8][3] receives (dy, dx, SAD); fallback block indices go to
  * fb_list, their number to *fb_count (cleared first), ready for
  * umi_me_full_search_listed_u8.  ws: umi_me_refine_workspace bytes; it also
- * holds, after the int32 state of two buffers, one int32 per round that is
- * nonzero when that round changed some block. */
+ * holds, after two state buffers of 16 B per block (each rounded up to 256 B),
+ * one int32 per global round that is nonzero when that round changed some
+ * block. */
 umi_status umi_me_refine_workspace(int H, int W, int rounds, size_t *bytes);
 umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R, const int32_t *prop,
-                         int iters, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count,
-                         void *ws, size_t ws_bytes, umi_stream s);
+                         int iters, int local_rounds, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list,
+                         int32_t *fb_count, void *ws, size_t ws_bytes, umi_stream s);
 
 #ifdef __cplusplus
 }

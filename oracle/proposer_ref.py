@@ -6,8 +6,8 @@ on all but the last layer; Huber loss (delta 1) against block labels / 4,
 mean over all outputs.  The backward pass is written out analytically and
 pinned by finite differences in tests/test_oracle_proposer.py.
 
-The refinement (seed, iterated +-2 descent, jump-flood propagation rounds,
-fallback check) has no counterpart in the reference; it is pinned in
+The refinement (seed, iterated +-2 descent, in-tile and global jump-flood
+propagation rounds, fallback check) has no counterpart in the reference; it is pinned in
 tests/test_oracle_proposer.py by the C oracle's SAD volume, by exact-shift
 frames (SAD 0 at the true vector), by the local-minimum property of every
 converged block and by a scalar recomputation of the fallback test.
@@ -115,7 +115,9 @@ def block_sad(cur, ref, by, bx, dy, dx):
     return int(np.abs(cur[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64) - ref[np.ix_(ys, xs)]).sum())
 
 
-JUMPS = (8, 4, 2, 1)                       # propagation distance of round p: JUMPS[p % 4]
+JUMPS = (8, 4, 2, 1)     # distance of global round p: JUMPS[p % 4]
+LOCAL_JUMPS = (4, 2, 1)  # distance of in-tile round p: LOCAL_JUMPS[p % 3]
+TILE = (4, 8)            # tile of blocks (rows, columns) for the in-tile rounds, aligned to the block grid
 RING = (0, 1, 2, 3, 5, 6, 7, 8)            # 3x3 neighbourhood in raster order without the centre (4)
 
 
@@ -137,11 +139,17 @@ class _Blocks:
         xs = np.clip(self.bx[:, None] * 8 + np.arange(8)[None] + mv[:, 1:2], 0, self.W - 1)
         return np.abs(self.cur - self.ref[ys[:, :, None], xs[:, None, :]]).sum((1, 2))
 
-    def neighbour(self, o, d):
-        """(in-frame mask, index) of neighbour o (0..8, raster, 4 = self) at distance d."""
+    def neighbour(self, o, d, tile=False):
+        """(mask, index) of neighbour o (0..8, raster, 4 = self) at distance d:
+        inside the frame and, when tile is set, inside the block's tile."""
         ny, nx = self.by + (o // 3 - 1) * d, self.bx + (o % 3 - 1) * d
         ok = (ny >= 0) & (ny < self.nby) & (nx >= 0) & (nx < self.nbx)
+        if tile:
+            ok &= (ny // TILE[0] == self.by // TILE[0]) & (nx // TILE[1] == self.bx // TILE[1])
         return ok, np.where(ok, ny * self.nbx + nx, 0)
+
+    def tile_id(self):
+        return (self.by // TILE[0]) * (-(-self.nbx // TILE[1])) + self.bx // TILE[1]
 
     def first_min(self, cands):
         """First minimum (strict <, in list order) over [(mask, mv [n][2])]:
@@ -174,10 +182,42 @@ def _descend(B, c, active, R, iters):
     return c, sad, unconv
 
 
-def refine(cur, ref, prop, R, iters, rounds, slack):
-    """Refinement of block proposals (seed, descent, propagation rounds,
-    check), restating umi_me_refine.  Returns (mv_sad [n][3], fallback set,
-    changed flags of the rounds)."""
+def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile):
+    """Up to n propagation rounds (Jacobi: every block reads the states from
+    before the round).  Round p tries the neighbours at distance 1 and, when
+    k = jumps[p % len] > 1, at distance k (inside the tile when tile is set);
+    a block descends from the first minimum when it beats its own SAD.  With
+    tile set each tile stops after its first round without a change, else
+    all stop together.  Returns (mv, sad, unconv, changed flags)."""
+    tid = B.tile_id() if tile else np.zeros(B.n, np.int64)
+    active = np.ones(tid.max() + 1, bool)
+    changed = []
+    for p in range(n):
+        if not active.any():
+            break
+        k = jumps[p % len(jumps)]
+        cands = []
+        for d in ((1, k) if k > 1 else (1,)):
+            for o in RING:
+                ok, nb = B.neighbour(o, d, tile)
+                cands.append((ok, mv[nb]))
+        best, bmv = B.first_min(cands)
+        imp = (best < sad) & active[tid]
+        m2, s2, u2 = _descend(B, bmv, imp, R, iters)
+        mv = np.where(imp[:, None], m2, mv)
+        sad = np.where(imp, s2, sad)
+        unconv = np.where(imp, u2, unconv)
+        hit = np.zeros_like(active)
+        hit[tid[imp]] = True
+        active &= hit
+        changed.append(bool(imp.any()))
+    return mv, sad, unconv, changed
+
+
+def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
+    """Refinement of block proposals restating umi_me_refine: seed, descent,
+    in-tile rounds, global rounds, fallback check.  Returns (mv_sad [n][3],
+    fallback set, changed flags of the global rounds)."""
     B = _Blocks(cur, ref)
     prop = np.clip(np.asarray(prop, np.int64).reshape(B.n, 2), -R, R)
     seeds = [(np.ones(B.n, bool), prop)]
@@ -186,24 +226,9 @@ def refine(cur, ref, prop, R, iters, rounds, slack):
         seeds.append((ok, prop[nb]))
     _, c = B.first_min(seeds)
     mv, sad, unconv = _descend(B, c, np.ones(B.n, bool), R, iters)
-    changed = [False] * rounds
-    for p in range(rounds):
-        if p > 0 and not changed[p - 1]:
-            break
-        k = JUMPS[p % 4]
-        cands = []
-        for d in ((1, k) if k > 1 else (1,)):
-            for o in RING:
-                ok, nb = B.neighbour(o, d)
-                cands.append((ok, mv[nb]))
-        best, bmv = B.first_min(cands)
-        imp = best < sad
-        changed[p] = bool(imp.any())
-        if changed[p]:
-            m2, s2, u2 = _descend(B, bmv, imp, R, iters)
-            mv = np.where(imp[:, None], m2, mv)
-            sad = np.where(imp, s2, sad)
-            unconv = np.where(imp, u2, unconv)
+    mv, sad, unconv, _ = _rounds(B, mv, sad, unconv, R, iters, local_rounds, LOCAL_JUMPS, True)
+    mv, sad, unconv, changed = _rounds(B, mv, sad, unconv, R, iters, rounds, JUMPS, False)
+    changed += [False] * (rounds - len(changed))
     nbmin = np.full(B.n, 2 ** 32 - 1, np.int64)
     for o in RING:
         ok, nb = B.neighbour(o, 1)

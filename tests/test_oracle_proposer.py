@@ -105,15 +105,36 @@ def test_refine_exact_shift_spreads_from_one_block():
     n = (H // 8) * (W // 8)
     prop = np.tile(np.array([[-20, 20]]), (n, 1))
     prop[37] = (5, -7)
-    out, fb, changed = P.refine(cur, ref, prop, R, iters=8, rounds=16, slack=1.1)
+    out, fb, changed = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=16, slack=1.1)
     nbx = W // 8
     inner = np.arange(n) % nbx != 0
     assert (out[:, 2] == 0).all() and (out[inner, :2] == (5, -7)).all()
     assert all(b % nbx == 0 for b in fb)
     assert changed[0] and not all(changed)
     # without propagation rounds only the seed's 3x3 neighbourhood finds it
-    out0, _, _ = P.refine(cur, ref, prop, R, iters=8, rounds=0, slack=1.1)
+    out0, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=0, slack=1.1)
     assert 0 < np.count_nonzero(out0[:, 2] == 0) <= 9
+
+
+def test_refine_in_tile_rounds_stay_in_the_tile():
+    """Same exact-shift frame with global rounds off: the in-tile rounds carry
+    block 37's vector (block row 1, column 13: tile rows 0..3, columns 8..15)
+    to exactly its 4 x 8-block tile."""
+    H, W, R = 128, 192, 24
+    ref = _noise(H, W, 6)
+    yy, xx = np.mgrid[0:H, 0:W]
+    cur = ref[np.clip(yy + 5, 0, H - 1), np.clip(xx - 7, 0, W - 1)]
+    nby, nbx = H // 8, W // 8
+    prop = np.tile(np.array([[-20, 20]]), (nby * nbx, 1))
+    prop[37] = (5, -7)
+    out, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=0, slack=1.1)
+    by, bx = np.divmod(np.arange(nby * nbx), nbx)
+    tile = (by < 4) & (bx >= 8) & (bx < 16)
+    assert P.TILE == (4, 8)
+    np.testing.assert_array_equal(out[:, 2] == 0, tile)
+    # a global round then reaches the neighbouring tiles
+    out1, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=1, slack=1.1)
+    assert np.count_nonzero(out1[:, 2] == 0) > np.count_nonzero(tile)
 
 
 def test_refine_results_are_local_minima_and_pass_the_check():
@@ -126,7 +147,7 @@ def test_refine_results_are_local_minima_and_pass_the_check():
     nby, nbx = H // 8, W // 8
     g = np.random.Generator(np.random.PCG64(8))
     prop = g.integers(-R, R + 1, size=(nby * nbx, 2))
-    out, fb, _ = P.refine(cur, ref, prop, R, iters=8, rounds=8, slack=slack)
+    out, fb, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=8, slack=slack)
     assert len(fb) < nby * nbx
     for b in range(nby * nbx):
         by, bx = divmod(b, nbx)
@@ -148,5 +169,5 @@ def test_refine_fallback_threshold_zero_forces_full_search():
     """slack 0 flags every block with a nonzero SAD (the host's way to force
     the full search)."""
     cur, ref = _noise(64, 64, 9), _noise(64, 64, 10)
-    out, fb, _ = P.refine(cur, ref, np.zeros((64, 2), np.int64), 8, iters=4, rounds=2, slack=0.0)
+    out, fb, _ = P.refine(cur, ref, np.zeros((64, 2), np.int64), 8, iters=4, local_rounds=2, rounds=2, slack=0.0)
     assert fb == set(np.nonzero(out[:, 2] > 0)[0].tolist())

@@ -6,8 +6,9 @@
 * Adam matches its formula; training lowers the loss;
 * the bf16 MFMA inference layers match the FP64 forward pass with the same
   bf16 roundings;
-* refinement (seed, descent, propagation rounds, fallback check) matches the
-  oracle exactly, including ragged frames, padded rows and edge settings; the
+* refinement (seed, descent, in-tile and global propagation rounds, fallback
+  check) matches the oracle exactly, including ragged frames, padded rows,
+  the largest radius and edge settings; the
   listed full search equals the full search on the listed blocks and leaves
   the others alone;
 * with the trained network (umi_amd/data checkpoint): >= 99 % of blocks reach
@@ -118,7 +119,7 @@ def test_inference_layers_match_bf16_oracle(gpu, N, H4, W4):
     assert np.abs(a0 - w0).max() <= 2 ** -7 * np.abs(w0).max()
 
 
-def _refine_gpu(cur, ref, prop, R, iters, rounds, slack, pitch=None):
+def _refine_gpu(cur, ref, prop, R, iters, local_rounds, rounds, slack, pitch=None):
     H, W = cur.shape
     pitch = pitch or W
     n = (H // 8) * (W // 8)
@@ -129,9 +130,10 @@ def _refine_gpu(cur, ref, prop, R, iters, rounds, slack, pitch=None):
     nbytes = C.c_size_t()
     call("umi_me_refine_workspace", H, W, rounds, C.byref(nbytes))
     ws = DeviceBuffer(nbytes.value)
-    print(f"[refine] H={H} W={W} pitch={pitch} R={R} iters={iters} rounds={rounds} slack={slack}", flush=True)
-    call("umi_me_refine", dc.ptr, dr.ptr, H, W, pitch, R, dp.ptr, iters, rounds, C.c_float(slack), mv.ptr, fl.ptr,
-         fc.ptr, ws.ptr, ws.nbytes, None)
+    print(f"[refine] H={H} W={W} pitch={pitch} R={R} iters={iters} local={local_rounds} rounds={rounds} "
+          f"slack={slack}", flush=True)
+    call("umi_me_refine", dc.ptr, dr.ptr, H, W, pitch, R, dp.ptr, iters, local_rounds, rounds, C.c_float(slack), mv.ptr,
+         fl.ptr, fc.ptr, ws.ptr, ws.nbytes, None)
     got = mv.download((n, 3), np.int32)
     k = int(fc.download((1,), np.int32)[0])
     fb = fl.download((max(k, 1),), np.int32)[:k].tolist()
@@ -142,26 +144,29 @@ def _refine_gpu(cur, ref, prop, R, iters, rounds, slack, pitch=None):
 
 
 @pytest.mark.parametrize("case", [
-    dict(H=96, W=136, R=32, iters=8, rounds=8, slack=1.1, kind="tiles"),
-    dict(H=100, W=138, R=16, iters=8, rounds=8, slack=1.1, kind="smooth"),         # ragged, unaligned rows
-    dict(H=96, W=136, R=16, iters=8, rounds=8, slack=1.1, kind="smooth", pitch=140),
-    dict(H=256, W=384, R=24, iters=8, rounds=12, slack=1.1, kind="smooth"),
-    dict(H=64, W=64, R=8, iters=1, rounds=0, slack=1.1, kind="smooth"),            # no rounds, one step
-    dict(H=64, W=72, R=8, iters=4, rounds=3, slack=0.0, kind="smooth"),            # slack 0 forces fallbacks
+    dict(H=96, W=136, R=32, iters=8, local=8, rounds=8, slack=1.1, kind="tiles"),
+    dict(H=100, W=138, R=16, iters=8, local=3, rounds=8, slack=1.1, kind="smooth"),    # ragged, unaligned rows
+    dict(H=96, W=136, R=16, iters=8, local=0, rounds=8, slack=1.1, kind="smooth", pitch=140),
+    dict(H=256, W=384, R=24, iters=8, local=8, rounds=3, slack=1.1, kind="smooth"),
+    dict(H=256, W=384, R=24, iters=8, local=8, rounds=12, slack=1.1, kind="tiles"),
+    dict(H=64, W=64, R=8, iters=1, local=0, rounds=0, slack=1.1, kind="smooth"),       # no rounds, one step
+    dict(H=64, W=72, R=8, iters=4, local=2, rounds=3, slack=0.0, kind="smooth"),       # slack 0 forces fallbacks
+    dict(H=200, W=264, R=64, iters=8, local=8, rounds=4, slack=1.1, kind="smooth"),    # largest radius
 ])
 def test_refine_matches_oracle(gpu, case):
     H, W, R = case["H"], case["W"], case["R"]
     if case["kind"] == "tiles":
-        cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)
+        cur, ref, _ = workloads.me_pair(H, W, min(R, 16), seed=9)
     else:
         cur, ref = workloads.smooth_motion_pair(H, W, R, seed=H + W)
     n = (H // 8) * (W // 8)
     g = workloads.rng(10)
     prop = g.integers(-R - 3, R + 4, size=(n, 2)).astype(np.int32)   # out-of-range proposals are clamped
     prop[::3] = 0
-    got, fb, changed, _ = _refine_gpu(cur, ref, prop, R, case["iters"], case["rounds"], case["slack"],
+    got, fb, changed, _ = _refine_gpu(cur, ref, prop, R, case["iters"], case["local"], case["rounds"], case["slack"],
                                       case.get("pitch"))
-    want, want_fb, want_changed = PR.refine(cur, ref, prop, R, case["iters"], case["rounds"], case["slack"])
+    want, want_fb, want_changed = PR.refine(cur, ref, prop, R, case["iters"], case["local"], case["rounds"],
+                                            case["slack"])
     np.testing.assert_array_equal(got, want)
     assert len(fb) == len(set(fb)) and set(fb) == want_fb
     assert changed == want_changed
@@ -172,7 +177,7 @@ def test_listed_search_completes_fallbacks(gpu):
     cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)
     n = (H // 8) * (W // 8)
     prop = workloads.rng(11).integers(-R, R + 1, size=(n, 2)).astype(np.int32)
-    before, fb, _, (dc, dr, fl, fc, mv) = _refine_gpu(cur, ref, prop, R, 2, 1, 1.1)
+    before, fb, _, (dc, dr, fl, fc, mv) = _refine_gpu(cur, ref, prop, R, 2, 0, 1, 1.1)
     assert 0 < len(fb) < n
     call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, 8, R, fl.ptr, fc.ptr, n, mv.ptr, None)
     full, _ = ops.me_full_search_np(cur, ref, 8, R)

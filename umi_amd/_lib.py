@@ -84,7 +84,7 @@ SIGNATURES = {
     "umi_cnn_infer_layer": [c_int, P, c_int, c_int, c_int, c_int, P, P, P, P],
     "umi_me_proposals": [P, c_int, c_int, c_int, P, P],
     "umi_me_refine_workspace": [c_int, c_int, c_int, P],
-    "umi_me_refine": [P, P, c_int, c_int, c_int, c_int, P, c_int, c_int, c_float, P, P, P, P, c_size_t, P],
+    "umi_me_refine": [P, P, c_int, c_int, c_int, c_int, P, c_int, c_int, c_int, c_float, P, P, P, P, c_size_t, P],
     "umi_conv_fprop": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P,
                        P],
 }

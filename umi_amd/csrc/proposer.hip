@@ -462,13 +462,15 @@ struct RefineArgs {
 };
 
 // LDS geometry of a tile's ref region: rows 8 RT_Y + 2R; a row starts at the
-// 16-byte boundary xs at or below x0 - R and holds columns up to
-// x0 + 8 RT_X - 1 + R plus one dword, so that a three-dword read at any vector
-// stays inside the row; stride4 is a multiple of 4 (whole 16-byte chunks).
+// 16-byte boundary xs at or below x0 - R (x0 - R - xs <= 15) and reaches far
+// enough that a four-dword read starting at or below any block's column for
+// a vector in [-R, R] stays inside the row: byte 8 (RT_X - 1) + 2R + 15 + 15
+// < 4 stride4; stride4 is a multiple of 4 (whole 16-byte chunks).
 inline void region_shape(int R, int &stride4, int &rows) {
-    stride4 = (8 * RT_X + 2 * R + 4 + 15 + 15) / 16 * 4;
+    stride4 = (8 * (RT_X - 1) + 2 * R + 31 + 16) / 16 * 4;
     rows = 8 * RT_Y + 2 * R;
 }
+// LDS of a tile: ref region and cur tile.
 inline size_t region_lds_bytes(int R) {
     int s4, rows;
     region_shape(R, s4, rows);
@@ -493,20 +495,21 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
     const int xr = bx0 * 8 - a.R, y0 = by0 * 8 - a.R, xs = floor16(xr);
     const int s16 = a.stride4 / 4, n16 = a.rows * s16;
     if (a.ref16 && xs >= 0 && xs + 16 * s16 <= a.W) {
+        // No branches around the loads or stores, so that all U loads are in
+        // flight before the first store: indices past the region repeat its
+        // last chunk (the same bytes stored to the same slot twice).
         for (int base = 0; base < n16; base += U * RT_THREADS) {
             uint4 v[U];
+            int idx[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int i = base + u * RT_THREADS + threadIdx.x, r = i / s16, c = i - r * s16;
-                if (i < n16)
-                    v[u] = *reinterpret_cast<const uint4 *>(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch +
-                                                           xs + 16 * c);
+                idx[u] = min(base + u * RT_THREADS + (int)threadIdx.x, n16 - 1);
+                const int r = idx[u] / s16, c = idx[u] - r * s16;
+                v[u] = *reinterpret_cast<const uint4 *>(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch + xs +
+                                                       16 * c);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = base + u * RT_THREADS + threadIdx.x;
-                if (i < n16) reinterpret_cast<uint4 *>(reg)[i] = v[u];
-            }
+            for (int u = 0; u < U; ++u) reinterpret_cast<uint4 *>(reg)[idx[u]] = v[u];
         }
     } else {
         for (int i = threadIdx.x; i < a.rows * a.stride4; i += RT_THREADS) {
@@ -518,11 +521,10 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
     constexpr int C16 = 8 * RT_Y * RT_X / 2;  // 16-byte chunks
     static_assert(C16 <= RT_THREADS, "one chunk per thread");
     if (a.cur16 && bx0 * 8 + 8 * RT_X <= a.W) {
-        if (threadIdx.x < C16) {
-            const int r = threadIdx.x / (RT_X / 2), c = threadIdx.x - r * (RT_X / 2);
-            reinterpret_cast<uint4 *>(ctile)[threadIdx.x] = *reinterpret_cast<const uint4 *>(
-                a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch + bx0 * 8 + 16 * c);
-        }
+        // threads past C16 repeat chunk t % C16 (same bytes, same slot)
+        const int i = threadIdx.x % C16, r = i / (RT_X / 2), c = i - r * (RT_X / 2);
+        reinterpret_cast<uint4 *>(ctile)[i] =
+            *reinterpret_cast<const uint4 *>(a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch + bx0 * 8 + 16 * c);
     } else {
         for (int i = threadIdx.x; i < 8 * RT_Y * 2 * RT_X; i += RT_THREADS) {
             const int r = i / (2 * RT_X), c = i - r * (2 * RT_X);
@@ -532,27 +534,155 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
     return xr - xs;
 }
 
-// SAD of tile block (ty, tx) at vector (dy, dx) from the staged region.
-__device__ inline uint32_t tile_sad(const uint32_t (&cw)[16], const uint32_t *reg, const RefineArgs &a, int xoff,
-                                    int ty, int tx, int dy, int dx) {
-    const int o = (8 * ty + dy + a.R) * a.stride4 * 4 + 8 * tx + dx + a.R + xoff;
-    const uint32_t *p = reg + (o >> 2);
-    const uint32_t sh = (uint32_t)(o & 3);
-    uint32_t s = 0;
+// SAD sources for one block's candidate vectors.  LdsSad reads the tile's
+// staged ref region; GlobalSad reads ref directly (coordinates clamped),
+// three aligned dwords per row for columns in [3, W - 12] (never outside the
+// row), clamped bytes elsewhere.
+struct LdsSad {
+    const uint32_t *reg;
+    int stride4, o0;  // o0: byte offset of the block's vector (-R, -R) in the region
+    __device__ uint32_t operator()(const uint32_t (&cw)[16], int R, int dy, int dx) const {
+        const int o = o0 + (dy + R) * stride4 * 4 + dx + R;
+        const uint32_t *p = reg + (o >> 2);
+        const uint32_t sh = (uint32_t)(o & 3);
+        uint32_t s = 0;
 #pragma unroll
-    for (int py = 0; py < 8; ++py) {
-        const uint32_t d0 = p[py * a.stride4], d1 = p[py * a.stride4 + 1], d2 = p[py * a.stride4 + 2];
-        s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cw[2 * py], s);
-        s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cw[2 * py + 1], s);
+        for (int py = 0; py < 8; ++py) {
+            const uint32_t d0 = p[py * stride4], d1 = p[py * stride4 + 1], d2 = p[py * stride4 + 2];
+            s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cw[2 * py], s);
+            s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cw[2 * py + 1], s);
+        }
+        return s;
     }
-    return s;
+    // SADs at (dy, dx0 + c), c = 0..4 (all within [-R, R]) from one pass over
+    // 8 rows of 16 bytes: v_qsad_pk_u16_u8 gives the 4 horizontal offsets of
+    // c = 0..3, two v_sad_u8 the fifth.
+    __device__ void row5(const uint32_t (&cw)[16], int R, int dy, int dx0, uint32_t (&s)[5]) const {
+        const int o = o0 + (dy + R) * stride4 * 4 + dx0 + R;
+        const uint32_t *p = reg + (o >> 2);
+        const uint32_t sh = (uint32_t)(o & 3);
+        uint64_t acc = 0;
+        uint32_t s4 = 0;
+#pragma unroll
+        for (int py = 0; py < 8; ++py) {
+            const uint32_t *q = p + py * stride4;
+            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
+            const uint32_t a0 = __builtin_amdgcn_alignbyte(d1, d0, sh), a1 = __builtin_amdgcn_alignbyte(d2, d1, sh),
+                           a2 = __builtin_amdgcn_alignbyte(d3, d2, sh);
+            acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a0 | ((uint64_t)a1 << 32), cw[2 * py], acc);
+            acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a1 | ((uint64_t)a2 << 32), cw[2 * py + 1], acc);
+            s4 = __builtin_amdgcn_sad_u8(a1, cw[2 * py], s4);
+            s4 = __builtin_amdgcn_sad_u8(a2, cw[2 * py + 1], s4);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s[c] = (uint32_t)(acc >> (16 * c)) & 0xffffu;
+        s[4] = s4;
+    }
+};
+__device__ inline LdsSad lds_sad(const uint32_t *reg, const RefineArgs &a, int xoff, int ty, int tx) {
+    return {reg, a.stride4, 8 * ty * a.stride4 * 4 + 8 * tx + xoff};
 }
+
+struct GlobalSad {
+    const uint8_t *ref;
+    int H, W, pitch, y0, x0;
+    __device__ uint32_t operator()(const uint32_t (&cw)[16], int /*R*/, int dy, int dx) const {
+        const int x = x0 + dx;
+        uint32_t s = 0;
+        if (x >= 3 && x + 12 <= W) {
+            uint32_t d[8][3], sh[8];
+#pragma unroll
+            for (int py = 0; py < 8; ++py) {  // all 24 loads before the first use
+                const uint8_t *p = ref + (size_t)clampi(y0 + dy + py, 0, H - 1) * pitch + x;
+                sh[py] = (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3);
+                const uint32_t *q = reinterpret_cast<const uint32_t *>(p - sh[py]);
+                d[py][0] = q[0];
+                d[py][1] = q[1];
+                d[py][2] = q[2];
+            }
+#pragma unroll
+            for (int py = 0; py < 8; ++py) {
+                s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d[py][1], d[py][0], sh[py]), cw[2 * py], s);
+                s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d[py][2], d[py][1], sh[py]), cw[2 * py + 1],
+                                            s);
+            }
+        } else {
+            for (int py = 0; py < 8; ++py) {
+                const uint8_t *r = ref + (size_t)clampi(y0 + dy + py, 0, H - 1) * pitch;
+                uint32_t lo = 0, hi = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    lo |= (uint32_t)r[clampi(x + i, 0, W - 1)] << (8 * i);
+                    hi |= (uint32_t)r[clampi(x + 4 + i, 0, W - 1)] << (8 * i);
+                }
+                s = __builtin_amdgcn_sad_u8(lo, cw[2 * py], s);
+                s = __builtin_amdgcn_sad_u8(hi, cw[2 * py + 1], s);
+            }
+        }
+        return s;
+    }
+    // As LdsSad::row5; columns x in [3, W - 16] read four aligned dwords per
+    // row (inside the row), other columns fall back to single positions.
+    __device__ void row5(const uint32_t (&cw)[16], int R, int dy, int dx0, uint32_t (&s)[5]) const {
+        const int x = x0 + dx0;
+        if (x >= 3 && x + 16 <= W) {
+            uint64_t acc = 0;
+            uint32_t s4 = 0;
+            uint32_t d[8][4], sh[8];
+#pragma unroll
+            for (int py = 0; py < 8; ++py) {  // all 32 loads before the first use
+                const uint8_t *p = ref + (size_t)clampi(y0 + dy + py, 0, H - 1) * pitch + x;
+                sh[py] = (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3);
+                const uint32_t *q = reinterpret_cast<const uint32_t *>(p - sh[py]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) d[py][k] = q[k];
+            }
+#pragma unroll
+            for (int py = 0; py < 8; ++py) {
+                const uint32_t a0 = __builtin_amdgcn_alignbyte(d[py][1], d[py][0], sh[py]),
+                               a1 = __builtin_amdgcn_alignbyte(d[py][2], d[py][1], sh[py]),
+                               a2 = __builtin_amdgcn_alignbyte(d[py][3], d[py][2], sh[py]);
+                acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a0 | ((uint64_t)a1 << 32), cw[2 * py], acc);
+                acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a1 | ((uint64_t)a2 << 32), cw[2 * py + 1], acc);
+                s4 = __builtin_amdgcn_sad_u8(a1, cw[2 * py], s4);
+                s4 = __builtin_amdgcn_sad_u8(a2, cw[2 * py + 1], s4);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s[c] = (uint32_t)(acc >> (16 * c)) & 0xffffu;
+            s[4] = s4;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) s[c] = (*this)(cw, R, dy, dx0 + c);
+        }
+    }
+};
 
 __device__ inline void tile_cur(uint32_t (&cw)[16], const uint32_t *ctile, int ty, int tx) {
 #pragma unroll
     for (int py = 0; py < 8; ++py) {
         cw[2 * py] = ctile[(8 * ty + py) * 2 * RT_X + 2 * tx];
         cw[2 * py + 1] = ctile[(8 * ty + py) * 2 * RT_X + 2 * tx + 1];
+    }
+}
+
+// Cur rows of the block at (y0, x0) from global memory.
+__device__ inline void global_cur(uint32_t (&cw)[16], const uint8_t *cur, int pitch, int y0, int x0) {
+#pragma unroll
+    for (int py = 0; py < 8; ++py) {
+        const uint8_t *r = cur + (size_t)(y0 + py) * pitch + x0;
+        if ((reinterpret_cast<uintptr_t>(r) & 3) == 0) {
+            cw[2 * py] = reinterpret_cast<const uint32_t *>(r)[0];
+            cw[2 * py + 1] = reinterpret_cast<const uint32_t *>(r)[1];
+        } else {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                lo |= (uint32_t)r[i] << (8 * i);
+                hi |= (uint32_t)r[4 + i] << (8 * i);
+            }
+            cw[2 * py] = lo;
+            cw[2 * py + 1] = hi;
+        }
     }
 }
 
@@ -563,23 +693,29 @@ __device__ inline uint32_t group_min(uint32_t k) {
     return k;
 }
 
-// Iterated +-2 descent from (cy, cx).  Each step evaluates the 25 positions
-// clamp(c + (i / 5 - 2, i % 5 - 2), -R, R), i = 0..24 (lane l takes i = l,
-// l + 8, l + 16, l + 24), and moves to the first minimum in i order; it stops
-// when that minimum is within 1 of the centre in both axes, else repeats up
-// to `iters` steps (unconv = 1 if it never stops).  The 8 lanes call it together.
-__device__ inline RefState descend(const RefineArgs &a, const uint32_t (&cw)[16], const uint32_t *reg, int xoff,
-                                   int ty, int tx, int l, int cy, int cx) {
+// Iterated +-2 descent from (cy, cx).  Each step takes the 25 positions
+// clamp(c + (i / 5 - 2, i % 5 - 2), -R, R), i = 0..24, and moves to the
+// first minimum in i order; it stops when that minimum is within 1 of the
+// centre in both axes, else repeats up to `iters` steps (unconv = 1 if it
+// never stops).  Lane r < 5 evaluates window row r (its 5 columns in one
+// pass when none of them is clamped).  The 8 lanes call it together.
+template <class Sad>
+__device__ inline RefState descend(const RefineArgs &a, const uint32_t (&cw)[16], const Sad &sad, int l, int cy,
+                                   int cx) {
     RefState st{cy, cx, 0, 1};
     for (int t = 0; t < a.iters; ++t) {
         uint32_t key = 0xffffffffu;
+        if (l < 5) {
+            const int vy = clampi(cy + l - 2, -a.R, a.R);
+            uint32_t sv[5];
+            if (cx - 2 >= -a.R && cx + 2 <= a.R) {
+                sad.row5(cw, a.R, vy, cx - 2, sv);
+            } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int i = l + 8 * j;
-            if (i < 25) {
-                const int dy = clampi(cy + i / 5 - 2, -a.R, a.R), dx = clampi(cx + i % 5 - 2, -a.R, a.R);
-                key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, dy, dx) << 5) | (uint32_t)i);
+                for (int c = 0; c < 5; ++c) sv[c] = sad(cw, a.R, vy, clampi(cx + c - 2, -a.R, a.R));
             }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) key = min(key, (sv[c] << 5) | (uint32_t)(5 * l + c));
         }
         key = group_min(key);
         const int i = (int)(key & 31);
@@ -596,11 +732,15 @@ __device__ inline RefState descend(const RefineArgs &a, const uint32_t (&cw)[16]
     return st;
 }
 
-// Neighbour o (0..8, raster order of the 3x3 neighbourhood, 4 = the block) at distance d.
+// Neighbour o (0..8, raster order of the 3x3 neighbourhood, 4 = the block)
+// at distance d.  nb is always a valid index (the block itself when the
+// neighbour is outside the frame), so that loads through it can be issued
+// unconditionally.
 __device__ inline bool neighbour(int by, int bx, int o, int d, int nby, int nbx, int &nb) {
     const int ny = by + (o / 3 - 1) * d, nx = bx + (o % 3 - 1) * d;
-    nb = ny * nbx + nx;
-    return ny >= 0 && ny < nby && nx >= 0 && nx < nbx;
+    const bool in = ny >= 0 && ny < nby && nx >= 0 && nx < nbx;
+    nb = in ? ny * nbx + nx : by * nbx + bx;
+    return in;
 }
 
 // The 8 neighbours in raster order, skipping the block itself.
@@ -615,140 +755,222 @@ __device__ inline void winner(uint32_t key, const int (&my)[2], const int (&mx)[
     cx = __shfl(mx[pick], c & 7, 8);
 }
 
-// Seed: the first minimum over the block's own proposal and its 8 neighbours'
-// proposals (candidates 0..8 in that order), then the descent.
-__global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, const int32_t *__restrict__ prop,
-                                                                 RefState *__restrict__ out) {
-    extern __shared__ uint32_t lds[];
-    const int tile = tile_of_workgroup();
-    if (tile >= a.ntiles) return;
-    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
-    const int nbx = a.W / 8, nby = a.H / 8;
-    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
-    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
-    __syncthreads();
-    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
-    const int by = by0 + ty, bx = bx0 + tx;
-    if (by >= nby || bx >= nbx) return;  // no barrier follows
-    uint32_t cw[16];
-    tile_cur(cw, ctile, ty, tx);
-    uint32_t key = 0xffffffffu;
-    int my[2] = {0, 0}, mx[2] = {0, 0}, nb;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int c = l + 8 * j;
-        if (c < 9 && neighbour(by, bx, c == 0 ? 4 : ring(c - 1), 1, nby, nbx, nb)) {
-            my[j] = clampi(prop[2 * nb], -a.R, a.R);
-            mx[j] = clampi(prop[2 * nb + 1], -a.R, a.R);
-            key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, my[j], mx[j]) << 5) | (uint32_t)c);
-        }
-    }
-    key = group_min(key);
-    int cy, cx;
-    winner(key, my, mx, cy, cx);
-    const RefState st = descend(a, cw, reg, xoff, ty, tx, l, cy, cx);
-    if (l == 0) out[by * nbx + bx] = st;
+// Distances of the global rounds (8, 4, 2, 1, 8, ...) and of the in-tile
+// rounds (4, 2, 1, 4, ...).
+__host__ __device__ inline int global_jump(int p) { return 8 >> (p & 3); }
+__device__ inline int local_jump(int p) { return 4 >> (p % 3); }
+
+// The block a thread works on: 8 lanes per block, RT_Y x RT_X blocks per tile.
+struct BlockCtx {
+    int by, bx, ty, tx, l, b;  // b: block index (0 for blocks past the frame)
+    bool valid;
+};
+__device__ inline BlockCtx block_ctx(const RefineArgs &a, int tile) {
+    BlockCtx c;
+    const int g = threadIdx.x >> 3;
+    c.l = threadIdx.x & 7;
+    c.ty = g / RT_X;
+    c.tx = g % RT_X;
+    c.by = (tile / a.tiles_x) * RT_Y + c.ty;
+    c.bx = (tile % a.tiles_x) * RT_X + c.tx;
+    c.valid = tile < a.ntiles && c.by < a.H / 8 && c.bx < a.W / 8;
+    c.b = c.valid ? c.by * (a.W / 8) + c.bx : 0;
+    return c;
 }
 
-// One propagation round: each block tries the vectors of its 8 neighbours at
-// distance 1 (candidates 0..7) and, when k > 1, of its 8 neighbours at
-// distance k (candidates 8..15), from state `in`; when the first minimum
-// beats its own SAD it descends from there.  A candidate equal to the
-// block's own vector is skipped: its SAD equals the block's, which never
-// wins.  Writes every block's state to `out` and sets *changed when some
-// block improved; a round whose predecessor changed nothing does nothing.
+// First minimum over a block's candidates (lane l: candidates l and l + 8,
+// used when ok), skipping a candidate equal to `skip_y, skip_x` when
+// skip is set (its SAD equals the block's own and never wins).
+template <class Sad>
+__device__ inline uint32_t candidates_min(const RefineArgs &a, const uint32_t (&cw)[16], const Sad &sad, int l,
+                                          const int (&my)[2], const int (&mx)[2], const bool (&ok)[2], bool skip,
+                                          int skip_y, int skip_x) {
+    uint32_t key = 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (ok[j] && !(skip && my[j] == skip_y && mx[j] == skip_x))
+            key = min(key, (sad(cw, a.R, my[j], mx[j]) << 5) | (uint32_t)(l + 8 * j));
+    return group_min(key);
+}
+
+// Seed kernel: per tile, stage the ref region; per block the first minimum
+// over its own and its 8 neighbours' proposals (candidates 0..8), the
+// descent, then up to local_rounds in-tile rounds (Jacobi over the tile's
+// states in LDS; candidates are the neighbours at distance 1 and
+// local_jump(p) inside the tile; the tile stops after a round without a
+// change).  Also clears the round flags and the fallback count.
+__global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, int local_rounds,
+                                                                 const int32_t *__restrict__ prop,
+                                                                 RefState *__restrict__ out, int32_t *changed,
+                                                                 int rounds, int32_t *fb_count) {
+    extern __shared__ uint32_t lds[];
+    __shared__ RefState tstate[RT_Y * RT_X];
+    const int tile = tile_of_workgroup();
+    if (blockIdx.x == 0) {
+        if ((int)threadIdx.x < rounds) changed[threadIdx.x] = 0;
+        if (threadIdx.x == 0) *fb_count = 0;
+    }
+    if (tile >= a.ntiles) return;  // whole workgroup
+    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
+    const BlockCtx c = block_ctx(a, tile);
+    const int nby = a.H / 8, nbx = a.W / 8;
+    int my[2], mx[2];
+    bool ok[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {  // proposal loads first: in flight with the fill
+        const int k = c.l + 8 * j;
+        int nb;
+        const bool in = neighbour(c.by, c.bx, k == 0 ? 4 : ring(min(k, 8) - 1), 1, nby, nbx, nb);
+        ok[j] = c.valid && k < 9 && in;
+        const int i = ok[j] ? nb : c.b;
+        my[j] = clampi(prop[2 * i], -a.R, a.R);
+        mx[j] = clampi(prop[2 * i + 1], -a.R, a.R);
+    }
+    const int xoff = fill_tile(reg, ctile, a, (tile % a.tiles_x) * RT_X, (tile / a.tiles_x) * RT_Y);
+    __syncthreads();
+    const LdsSad sad = lds_sad(reg, a, xoff, c.ty, c.tx);
+    uint32_t cw[16];
+    tile_cur(cw, ctile, c.ty, c.tx);
+    RefState st{0, 0, 0, 0};
+    if (c.valid) {
+        int cy, cx;
+        winner(candidates_min(a, cw, sad, c.l, my, mx, ok, false, 0, 0), my, mx, cy, cx);
+        st = descend(a, cw, sad, c.l, cy, cx);
+    }
+    const int g = c.ty * RT_X + c.tx;
+    if (c.l == 0) tstate[g] = st;
+    __syncthreads();
+    for (int p = 0; p < local_rounds; ++p) {
+        const int k = local_jump(p);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {  // candidates from the tile's states before the round
+            const int o = ring(c.l), d = j ? k : 1;
+            const int ty = c.ty + (o / 3 - 1) * d, tx = c.tx + (o % 3 - 1) * d;
+            ok[j] = c.valid && (j == 0 || k > 1) && ty >= 0 && ty < RT_Y && tx >= 0 && tx < RT_X &&
+                    c.by - c.ty + ty < nby && c.bx - c.tx + tx < nbx;
+            const RefState n = tstate[ok[j] ? ty * RT_X + tx : g];
+            my[j] = n.dy;
+            mx[j] = n.dx;
+        }
+        __syncthreads();  // all reads of the round precede its writes
+        bool improved = false;
+        if (c.valid) {
+            const uint32_t key = candidates_min(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
+            improved = (int)(key >> 5) < st.sad;  // no candidate: key >> 5 = 2^27 - 1
+            if (improved) {
+                int cy, cx;
+                winner(key, my, mx, cy, cx);
+                st = descend(a, cw, sad, c.l, cy, cx);
+            }
+        }
+        if (c.l == 0) tstate[g] = st;
+        if (!__syncthreads_or(improved)) break;  // uniform
+    }
+    if (c.valid && c.l == 0) out[c.b] = st;
+}
+
+// One global round: each block tries the vectors of its 8 neighbours at
+// distance 1 and, when k > 1, at distance k (state `in`), and descends from
+// the first minimum when it beats its own SAD; SADs from global memory.
+// A round whose predecessor changed nothing does nothing.
 __global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, int k, const RefState *__restrict__ in,
                                                                   RefState *__restrict__ out,
                                                                   const int32_t *prev_changed, int32_t *changed) {
-    if (prev_changed && *prev_changed == 0) return;
-    extern __shared__ uint32_t lds[];
     const int tile = tile_of_workgroup();
     if (tile >= a.ntiles) return;
-    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
-    const int nbx = a.W / 8, nby = a.H / 8;
-    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
-    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
-    __syncthreads();
-    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
-    const int by = by0 + ty, bx = bx0 + tx;
-    bool improved = false;
-    if (by < nby && bx < nbx) {
-        const int b = by * nbx + bx;
-        uint32_t cw[16];
-        tile_cur(cw, ctile, ty, tx);
-        const RefState own = in[b];
-        uint32_t key = 0xffffffffu;
-        int my[2] = {0, 0}, mx[2] = {0, 0}, nb;
+    if (prev_changed && *prev_changed == 0) return;
+    const BlockCtx c = block_ctx(a, tile);
+    const int nby = a.H / 8, nbx = a.W / 8;
+    RefState st = in[c.b];
+    int my[2], mx[2];
+    bool ok[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int c = l + 8 * j;
-            if ((j == 0 || k > 1) && neighbour(by, bx, ring(l), j ? k : 1, nby, nbx, nb)) {
-                my[j] = in[nb].dy;
-                mx[j] = in[nb].dx;
-                if (my[j] != own.dy || mx[j] != own.dx)
-                    key = min(key, (tile_sad(cw, reg, a, xoff, ty, tx, my[j], mx[j]) << 5) | (uint32_t)c);
-            }
-        }
-        key = group_min(key);
-        RefState st = own;
-        improved = (int)(key >> 5) < own.sad;  // no candidate: key >> 5 = 2^27 - 1
+    for (int j = 0; j < 2; ++j) {
+        int nb;
+        const bool inside = neighbour(c.by, c.bx, ring(c.l), j ? k : 1, nby, nbx, nb);
+        ok[j] = c.valid && (j == 0 || k > 1) && inside;
+        const RefState n = in[ok[j] ? nb : c.b];
+        my[j] = n.dy;
+        mx[j] = n.dx;
+    }
+    uint32_t cw[16];
+    global_cur(cw, a.cur, a.pitch, c.valid ? 8 * c.by : 0, c.valid ? 8 * c.bx : 0);
+    const GlobalSad sad{a.ref, a.H, a.W, a.pitch, 8 * c.by, 8 * c.bx};
+    bool improved = false;
+    if (c.valid) {
+        const uint32_t key = candidates_min(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
+        improved = (int)(key >> 5) < st.sad;
         if (improved) {
             int cy, cx;
             winner(key, my, mx, cy, cx);
-            st = descend(a, cw, reg, xoff, ty, tx, l, cy, cx);
+            st = descend(a, cw, sad, c.l, cy, cx);
         }
-        if (l == 0) out[b] = st;
+        if (c.l == 0) out[c.b] = st;
     }
     if (__syncthreads_or(improved) && threadIdx.x == 0) *changed = 1;
 }
 
-// Final pass: the state after the last round that ran (round p reads buffer
-// p & 1 and writes (p + 1) & 1; rounds run while the previous one changed
-// something).  A block falls back to the full search when its descent never
-// stopped or when its SAD exceeds slack x the smallest SAD it has at its 8
-// neighbours' vectors.
+// Writes lane 0's result and appends fallback blocks, one atomic per wave
+// (the wave's fallback blocks take consecutive slots).  All lanes call it.
+__device__ inline void emit(const BlockCtx &c, const RefState &own, bool fallback, int32_t *__restrict__ mv_sad,
+                            int32_t *__restrict__ fb_list, int32_t *__restrict__ fb_count) {
+    const bool lead = c.valid && c.l == 0;
+    if (lead) {
+        int32_t *o = mv_sad + (size_t)c.b * 3;
+        o[0] = own.dy;
+        o[1] = own.dx;
+        o[2] = own.sad;
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t fm = __ballot(lead && fallback);
+    if (fm) {
+        const int leader = __ffsll((unsigned long long)fm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(fb_count, __popcll(fm));
+        base = __shfl(base, leader, 64);
+        if (lead && fallback) fb_list[base + __popcll(fm & ((1ull << lane) - 1))] = c.b;
+    }
+}
+
+// Final pass: the state after the last global round that ran (round p reads
+// buffer p & 1 and writes (p + 1) & 1; rounds run while the previous one
+// changed something).  A block falls back to the full search when its
+// descent never stopped or when its SAD exceeds slack x the smallest SAD it
+// has at its 8 neighbours' vectors (a neighbour with the block's own vector
+// has the block's SAD and is not evaluated).
 __global__ __launch_bounds__(RT_THREADS) void refine_final_kernel(RefineArgs a, const RefState *__restrict__ buf0,
                                                                   const RefState *__restrict__ buf1,
                                                                   const int32_t *__restrict__ changed, int rounds,
                                                                   float slack, int32_t *__restrict__ mv_sad,
                                                                   int32_t *__restrict__ fb_list,
                                                                   int32_t *__restrict__ fb_count) {
-    extern __shared__ uint32_t lds[];
     const int tile = tile_of_workgroup();
     if (tile >= a.ntiles) return;
     const int lane = threadIdx.x & 63;
-    const uint64_t zero = __ballot(lane < rounds && changed[lane] == 0);
-    const int last = zero ? __ffsll((unsigned long long)zero) - 1 : rounds - 1;
-    const RefState *fin = ((last + 1) & 1) ? buf1 : buf0;
-    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
-    const int nbx = a.W / 8, nby = a.H / 8;
-    const int by0 = (tile / a.tiles_x) * RT_Y, bx0 = (tile % a.tiles_x) * RT_X;
-    const int xoff = fill_tile(reg, ctile, a, bx0, by0);
-    __syncthreads();
-    const int g = threadIdx.x >> 3, l = threadIdx.x & 7, ty = g / RT_X, tx = g % RT_X;
-    const int by = by0 + ty, bx = bx0 + tx;
-    if (by >= nby || bx >= nbx) return;
-    const int b = by * nbx + bx;
-    uint32_t cw[16];
-    tile_cur(cw, ctile, ty, tx);
-    const RefState own = fin[b];
-    uint32_t m = 0xffffffffu;
+    const int32_t flag = changed[min(lane, max(rounds - 1, 0))];
+    const BlockCtx c = block_ctx(a, tile);
     int nb;
-    if (neighbour(by, bx, ring(l), 1, nby, nbx, nb)) m = tile_sad(cw, reg, a, xoff, ty, tx, fin[nb].dy, fin[nb].dx);
-    m = group_min(m);
-    if (l != 0) return;
-    const bool worse = (float)own.sad > slack * (float)min((uint32_t)own.sad, m);
-    int32_t *o = mv_sad + (size_t)b * 3;
-    o[0] = own.dy;
-    o[1] = own.dx;
-    o[2] = own.sad;
-    if (own.unconv || worse) fb_list[atomicAdd(fb_count, 1)] = b;
-}
-
-__global__ void refine_clear_kernel(int32_t *changed, int rounds, int32_t *fb_count) {
-    const int i = threadIdx.x;
-    if (i < rounds) changed[i] = 0;
-    if (i == 0) *fb_count = 0;
+    const bool inside = neighbour(c.by, c.bx, ring(c.l), 1, a.H / 8, a.W / 8, nb);
+    const bool ok = c.valid && inside;
+    const int nbi = ok ? nb : c.b;
+    // both buffers are read; which one is final is known after the ballot
+    const RefState own0 = buf0[c.b], own1 = buf1[c.b], nb0 = buf0[nbi], nb1 = buf1[nbi];
+    uint32_t cw[16];
+    global_cur(cw, a.cur, a.pitch, c.valid ? 8 * c.by : 0, c.valid ? 8 * c.bx : 0);
+    const uint64_t zero = __ballot(lane < rounds && flag == 0);
+    const int last = zero ? __ffsll((unsigned long long)zero) - 1 : rounds - 1;
+    const bool one = (last + 1) & 1;
+    const RefState own = one ? own1 : own0, n = one ? nb1 : nb0;
+    const GlobalSad sad{a.ref, a.H, a.W, a.pitch, 8 * c.by, 8 * c.bx};
+    bool fb = false;
+    if (c.valid) {
+        uint32_t m = 0xffffffffu;
+        if (ok && (n.dy != own.dy || n.dx != own.dx)) m = sad(cw, a.R, n.dy, n.dx);
+        m = group_min(m);
+        const bool worse = (float)own.sad > slack * (float)min((uint32_t)own.sad, m);
+        fb = own.unconv || worse;
+    }
+    emit(c, own, fb, mv_sad, fb_list, fb_count);
 }
 
 }  // namespace
@@ -917,14 +1139,15 @@ umi_status umi_me_refine_workspace(int H, int W, int rounds, size_t *bytes) {
 }
 
 umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R, const int32_t *prop,
-                         int iters, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count,
-                         void *ws, size_t ws_bytes, umi_stream s) {
+                         int iters, int local_rounds, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list,
+                         int32_t *fb_count, void *ws, size_t ws_bytes, umi_stream s) {
     if (!cur || !ref || !prop || !mv_sad || !fb_list || !fb_count || !ws)
         return set_error(UMI_ERR_INVALID, "umi_me_refine: null pointer");
     if (H < 8 || W < 8 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_me_refine: bad frame");
     if (R < 1 || R > 64) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_refine: radius %d (1..64)", R);
-    if (iters < 1 || iters > 64 || rounds < 0 || rounds > 64)
-        return set_error(UMI_ERR_INVALID, "umi_me_refine: iters %d (1..64), rounds %d (0..64)", iters, rounds);
+    if (iters < 1 || iters > 64 || local_rounds < 0 || local_rounds > 64 || rounds < 0 || rounds > 64)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine: iters %d (1..64), local rounds %d, rounds %d (0..64)", iters,
+                         local_rounds, rounds);
     if (ws_bytes < refine_ws_bytes(H, W, rounds))
         return set_error(UMI_ERR_INVALID, "umi_me_refine: workspace %zu B < %zu B", ws_bytes,
                          refine_ws_bytes(H, W, rounds));
@@ -940,17 +1163,15 @@ umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, i
     const size_t lds = region_lds_bytes(R);
     hipStream_t st = as_stream(s);
     const dim3 grid(round_up(a.ntiles, 8)), block(RT_THREADS);
-    hipLaunchKernelGGL(refine_clear_kernel, dim3(1), dim3(64), 0, st, changed, rounds, fb_count);
-    UMI_TRY(launch_check("refine_clear_kernel"));
-    hipLaunchKernelGGL(refine_seed_kernel, grid, block, lds, st, a, prop, buf[0]);
+    hipLaunchKernelGGL(refine_seed_kernel, grid, block, lds, st, a, local_rounds, prop, buf[0], changed, rounds,
+                       fb_count);
     UMI_TRY(launch_check("refine_seed_kernel"));
-    static const int kJump[4] = {8, 4, 2, 1};
     for (int p = 0; p < rounds; ++p) {
-        hipLaunchKernelGGL(refine_round_kernel, grid, block, lds, st, a, kJump[p & 3], buf[p & 1], buf[(p + 1) & 1],
+        hipLaunchKernelGGL(refine_round_kernel, grid, block, 0, st, a, global_jump(p), buf[p & 1], buf[(p + 1) & 1],
                            p ? changed + p - 1 : (const int32_t *)nullptr, changed + p);
         UMI_TRY(launch_check("refine_round_kernel"));
     }
-    hipLaunchKernelGGL(refine_final_kernel, grid, block, lds, st, a, buf[0], buf[1], changed, rounds, slack, mv_sad,
+    hipLaunchKernelGGL(refine_final_kernel, grid, block, 0, st, a, buf[0], buf[1], changed, rounds, slack, mv_sad,
                        fb_list, fb_count);
     return launch_check("refine_final_kernel");
 }

@@ -6,8 +6,8 @@ pair at quarter resolution and predicts the motion field.  The mean of each
 8x8 block's 2x2 quarter-resolution outputs, times 4, is the block's proposal.
 Exact SAD refinement (umi_me_refine) starts from the best of the block's
 own and its 8 neighbours' proposals, descends through +-2 windows, then
-runs jump-flood propagation rounds in which blocks adopt better neighbour
-vectors.  A block falls back to the full +-R search when its descent never
+runs jump-flood propagation rounds, first inside 4 x 8-block tiles, then
+over the frame, in which blocks adopt better neighbour vectors.  A block falls back to the full +-R search when its descent never
 settles, or when its SAD exceeds `slack` (1.1) x the smallest SAD it has at
 its 8 neighbours' final vectors.
 
@@ -214,7 +214,15 @@ class _View:
         self.ptr, self.nbytes = int(ptr), int(nbytes)
 
 
-def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, log=print, stream=None):
+def learning_rate(lr: float, ep: int, epochs: int, schedule: str) -> float:
+    """'step': halve every epochs / 3; 'cosine': lr/2 (1 + cos(pi ep / epochs))."""
+    if schedule == "cosine":
+        return lr * 0.5 * (1.0 + np.cos(np.pi * ep / epochs))
+    return lr * (0.5 ** (ep // max(1, epochs // 3)))
+
+
+def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, log=print, stream=None,
+          schedule: str = "step"):
     tr = Trainer(net, batch, data.H4, data.W4)
     g = np.random.Generator(np.random.PCG64(1234))
     history = []
@@ -227,7 +235,7 @@ def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 
             x = _View(xp, 0)
             tr.forward(x, stream)
             tr.backward(x, _View(lp, 0), stream)
-            tr.adam(lr * (0.5 ** (ep // max(1, epochs // 3))), stream)
+            tr.adam(learning_rate(lr, ep, epochs, schedule), stream)
             tot += tr.loss()
         history.append(tot / nb)
         log(f"[proposer] epoch {ep}: mean Huber loss {history[-1]:.4f}")
@@ -252,9 +260,9 @@ def pack_layer(net: Network, li: int, stream=None) -> DeviceBuffer:
 class Proposer:
     """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
 
-    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=8, rounds=8):
+    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=8, local_rounds=8, rounds=3):
         self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
-        self.iters, self.rounds = iters, rounds
+        self.iters, self.local_rounds, self.rounds = iters, local_rounds, rounds
         self.H4, self.W4 = H // 4, W // 4
         px = self.H4 * self.W4
         self.nblk = (H // 8) * (W // 8)
@@ -283,8 +291,8 @@ class Proposer:
 
     def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, self.iters,
-             self.rounds, C.c_float(self.slack), mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr, self.ws.ptr,
-             self.ws.nbytes, _s(stream))
+             self.local_rounds, self.rounds, C.c_float(self.slack), mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr,
+             self.ws.ptr, self.ws.nbytes, _s(stream))
 
     def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         s = _s(stream)
@@ -370,10 +378,12 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--schedule", choices=["step", "cosine"], default="step")
     ap.add_argument("--seed0", type=int, default=0, help="first training pair seed")
     ap.add_argument("--eval-seed0", type=int, default=1_000_000, help="first held-out pair seed")
     ap.add_argument("--ckpt", default=str(CHECKPOINT))
-    ap.add_argument("--rounds", type=int, default=8, help="propagation rounds of the refinement")
+    ap.add_argument("--rounds", type=int, default=3, help="global propagation rounds of the refinement")
+    ap.add_argument("--local-rounds", type=int, default=8, help="in-tile propagation rounds of the refinement")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
@@ -382,7 +392,7 @@ def main(argv=None):
         print(f"[proposer] {a.pairs} pairs synthesised and labelled in {time.time() - t0:.1f} s", flush=True)
         net = Network(seed=1)
         t0 = time.time()
-        hist = train(net, data, a.epochs, a.batch, a.lr, log=lambda m: print(m, flush=True))
+        hist = train(net, data, a.epochs, a.batch, a.lr, log=lambda m: print(m, flush=True), schedule=a.schedule)
         print(f"[proposer] trained {a.epochs} epochs in {time.time() - t0:.1f} s", flush=True)
         out = a.out or a.ckpt
         Path(out).parent.mkdir(parents=True, exist_ok=True)
@@ -390,10 +400,10 @@ def main(argv=None):
         print(json.dumps({"loss_history": hist, "checkpoint": out}), flush=True)
     else:
         net = Network.load(a.ckpt)
-        prop = Proposer(net, rounds=a.rounds)
+        prop = Proposer(net, local_rounds=a.local_rounds, rounds=a.rounds)
         with_net = evaluate(prop, a.pairs, a.eval_seed0, proposals=True)
         without = evaluate(prop, a.pairs, a.eval_seed0, proposals=False)
-        long = Proposer(net, rounds=32)
+        long = Proposer(net, rounds=32)  # the same refinement with a larger budget of global rounds
         without_32 = evaluate(long, a.pairs, a.eval_seed0, proposals=False)
         print(json.dumps({"with_network": with_net, "without_network": without,
                           "without_network_32_rounds": without_32}), flush=True)
