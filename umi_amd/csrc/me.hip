@@ -22,6 +22,7 @@
 //    min implements "smallest SAD, then first in raster order"; per-thread
 //    minima are merged with an LDS atomic min per block.
 #include "common.h"
+#include "stage.h"
 
 #include <type_traits>
 
@@ -32,17 +33,6 @@ using namespace umi;
 // accumulators per thread, which needs the 256-VGPR budget of <= 512 threads.
 template <int B>
 constexpr int me_max_threads() { return B == 8 ? 1024 : 512; }
-
-// Four clamped bytes of row `row` starting at column gx, as one dword.
-// Interior, 4-byte-aligned columns take a single dword load.
-__device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
-    if (gx >= 0 && gx + 3 < W && ((reinterpret_cast<uintptr_t>(row + gx) & 3) == 0))
-        return *reinterpret_cast<const uint32_t *>(row + gx);
-    uint32_t v = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
-    return v;
-}
 
 // One TBX x TBY tile of blocks whose first block is (bx0, by0); all threads
 // of the workgroup call it (it contains barriers).
@@ -63,16 +53,18 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
     const int x0 = bx0 * B, y0 = by0 * B;
 
     // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(x0-R+c)].
-    for (int i = tid; i < wrows * ws4; i += nthr) {
-        const int r = i / ws4, c4 = i - r * ws4;
-        win[i] = load4_clamped(ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch, x0 - R + 4 * c4, W);
-    }
+    stage_clamped<8>(win, wrows * ws4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
+        const int r = i / ws4;
+        row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
+        gx = x0 - R + 4 * (i - r * ws4);
+    });
     // Current tile.  Blocks past the frame edge read clamped pixels and are
     // never written out.
-    for (int i = tid; i < tby * B * cs4; i += nthr) {
-        const int r = i / cs4, c4 = i - r * cs4;
-        ctile[i] = load4_clamped(cur + (size_t)min(y0 + r, H - 1) * pitch, x0 + 4 * c4, W);
-    }
+    stage_clamped<4>(ctile, tby * B * cs4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
+        const int r = i / cs4;
+        row = cur + (size_t)min(y0 + r, H - 1) * pitch;
+        gx = x0 + 4 * (i - r * cs4);
+    });
     for (int i = tid; i < nblk; i += nthr) best[i] = 0xffffffffu;
     __syncthreads();
 
@@ -226,12 +218,15 @@ __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restric
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int blk = list[i];
         const int bx = blk % nbx, by = blk / nbx, x0 = bx * B, y0 = by * B;
-        for (int k = tid; k < wrows * ws4; k += blockDim.x) {
-            const int r = k / ws4, c4 = k - r * ws4;
-            win[k] = load4_clamped(ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch, x0 - R + 4 * c4, W);
-        }
-        for (int k = tid; k < B * B4; k += blockDim.x)
-            ctile[k] = load4_clamped(cur + (size_t)(y0 + k / B4) * pitch, x0 + 4 * (k % B4), W);
+        stage_clamped<8>(win, wrows * ws4, tid, blockDim.x, W, [&](int k, const uint8_t *&row, int &gx) {
+            const int r = k / ws4;
+            row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
+            gx = x0 - R + 4 * (k - r * ws4);
+        });
+        stage_clamped<1>(ctile, B * B4, tid, blockDim.x, W, [&](int k, const uint8_t *&row, int &gx) {
+            row = cur + (size_t)(y0 + k / B4) * pitch;
+            gx = x0 + 4 * (k % B4);
+        });
         if (tid == 0) *best = 0xffffffffu;
         __syncthreads();
         const int d0 = ch * DCH, d1 = min(S, d0 + DCH);

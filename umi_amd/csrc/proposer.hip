@@ -12,6 +12,7 @@
 //  * the exact SAD refinement of the proposals (seed, +-2 descent,
 //    propagation rounds, fallback check) and the fallback list.
 #include "common.h"
+#include "stage.h"
 
 #include <algorithm>
 
@@ -436,17 +437,6 @@ __global__ void proposals_kernel(const float *__restrict__ flow, int H4, int W4,
 // block's candidate vectors from LDS.
 constexpr int RT_X = 8, RT_Y = 4, RT_THREADS = 256;  // 32 blocks x 8 lanes
 
-// Four clamped bytes of `row` from column gx, as one dword (aligned interior
-// columns take one dword load).
-__device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
-    if (gx >= 0 && gx + 3 < W && ((reinterpret_cast<uintptr_t>(row + gx) & 3) == 0))
-        return *reinterpret_cast<const uint32_t *>(row + gx);
-    uint32_t v = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
-    return v;
-}
-
 // Block state between refinement passes: motion vector, SAD, and whether the
 // descent stopped at its iteration limit.
 struct RefState {
@@ -512,10 +502,12 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
             for (int u = 0; u < U; ++u) reinterpret_cast<uint4 *>(reg)[idx[u]] = v[u];
         }
     } else {
-        for (int i = threadIdx.x; i < a.rows * a.stride4; i += RT_THREADS) {
-            const int r = i / a.stride4, c = i - r * a.stride4;
-            reg[i] = load4_clamped(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch, xs + 4 * c, a.W);
-        }
+        stage_clamped<8>(reg, a.rows * a.stride4, threadIdx.x, RT_THREADS, a.W,
+                         [&](int i, const uint8_t *&row, int &gx) {
+                             const int r = i / a.stride4;
+                             row = a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch;
+                             gx = xs + 4 * (i - r * a.stride4);
+                         });
     }
     // cur tile: 8 RT_Y rows of 8 RT_X bytes; rows past the frame belong to no block
     constexpr int C16 = 8 * RT_Y * RT_X / 2;  // 16-byte chunks
@@ -526,10 +518,12 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
         reinterpret_cast<uint4 *>(ctile)[i] =
             *reinterpret_cast<const uint4 *>(a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch + bx0 * 8 + 16 * c);
     } else {
-        for (int i = threadIdx.x; i < 8 * RT_Y * 2 * RT_X; i += RT_THREADS) {
-            const int r = i / (2 * RT_X), c = i - r * (2 * RT_X);
-            ctile[i] = load4_clamped(a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch, bx0 * 8 + 4 * c, a.W);
-        }
+        stage_clamped<2>(ctile, 8 * RT_Y * 2 * RT_X, threadIdx.x, RT_THREADS, a.W,
+                         [&](int i, const uint8_t *&row, int &gx) {
+                             const int r = i / (2 * RT_X);
+                             row = a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch;
+                             gx = bx0 * 8 + 4 * (i - r * (2 * RT_X));
+                         });
     }
     return xr - xs;
 }

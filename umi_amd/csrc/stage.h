@@ -1,0 +1,53 @@
+// Staging clamp-to-edge byte rows into LDS as dwords, for gfx950.
+//
+// A frame row is read four bytes at a time; columns outside [0, W) repeat
+// the edge pixel.  stage_clamped keeps U global loads in flight per thread
+// before its first LDS store: every thread issues its loads unconditionally
+// (indices past the end repeat the last dword, stored twice with the same
+// value), and only dwords that are not 4-byte aligned or that cross the
+// frame edge take the byte path.
+#pragma once
+
+#include "common.h"
+
+namespace umi {
+
+// Four clamped bytes of `row` from column gx, as one dword.
+__device__ inline uint32_t load4_clamped(const uint8_t *row, int gx, int W) {
+    if (gx >= 0 && gx + 3 < W && ((reinterpret_cast<uintptr_t>(row + gx) & 3) == 0))
+        return *reinterpret_cast<const uint32_t *>(row + gx);
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(gx + b, 0, W - 1)] << (8 * b);
+    return v;
+}
+
+// dst[i] = load4_clamped(row_i, 4 c_i, W) for i < n, where src(i, row, gx)
+// yields the row pointer and first column of dword i.  Rows hold at least 8
+// bytes (W >= 8).  All threads of the workgroup (tid < nthr) call it; the
+// caller synchronises before reading dst.
+template <int U, class Src>
+__device__ inline void stage_clamped(uint32_t *dst, int n, int tid, int nthr, int W, Src src) {
+    for (int base = 0; base < n; base += U * nthr) {
+        uint32_t v[U];
+        int idx[U], gx[U];
+        const uint8_t *row[U];
+        bool fast[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            idx[u] = min(base + u * nthr + tid, n - 1);
+            src(idx[u], row[u], gx[u]);
+            fast[u] = gx[u] >= 0 && gx[u] + 3 < W && ((reinterpret_cast<uintptr_t>(row[u] + gx[u]) & 3) == 0);
+            // a dword-aligned address inside the row stands in for the others
+            const uint8_t *safe = row[u] + ((0u - (uint32_t)reinterpret_cast<uintptr_t>(row[u])) & 3u);
+            v[u] = *reinterpret_cast<const uint32_t *>(fast[u] ? row[u] + gx[u] : safe);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (!fast[u]) v[u] = load4_clamped(row[u], gx[u], W);
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[idx[u]] = v[u];
+    }
+}
+
+}  // namespace umi
