@@ -321,8 +321,10 @@ umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *p
  *             block tries its in-tile neighbours' vectors at distance 1 and k
  *             and descends from the first minimum when it beats its own SAD;
  *             a tile stops after a round without a change;
- *   rounds  - up to `rounds` such rounds over the whole frame with
- *             k = 8, 4, 2, 1, 8, ..., stopping after one without a change;
+ *   rounds  - up to `rounds` such rounds over the whole frame, round p
+ *             trying the neighbours at distance 1 and at the two distances
+ *             (8, 32), (4, 16), (2, 8), (4, 32) of entry p % 4, stopping after
+ *             one without a change;
  *   check   - a block falls back when its descent never stopped, or when its
  *             SAD exceeds slack x the smallest SAD it has at its 8 neighbours'
  *             final vectors.

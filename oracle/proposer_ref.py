@@ -115,8 +115,10 @@ def block_sad(cur, ref, by, bx, dy, dx):
     return int(np.abs(cur[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64) - ref[np.ix_(ys, xs)]).sum())
 
 
-JUMPS = (8, 4, 2, 1)     # distance of global round p: JUMPS[p % 4]
-LOCAL_JUMPS = (4, 2, 1)  # distance of in-tile round p: LOCAL_JUMPS[p % 3]
+# distances of in-tile round p: 1 and LOCAL_JUMPS[p % 3]; of global round p:
+# 1 and the two of GLOBAL_JUMPS[p % 4]
+LOCAL_JUMPS = ((4,), (2,), ())
+GLOBAL_JUMPS = ((8, 32), (4, 16), (2, 8), (4, 32))
 TILE = (4, 8)            # tile of blocks (rows, columns) for the in-tile rounds, aligned to the block grid
 RING = (0, 1, 2, 3, 5, 6, 7, 8)            # 3x3 neighbourhood in raster order without the centre (4)
 
@@ -184,9 +186,9 @@ def _descend(B, c, active, R, iters):
 
 def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile):
     """Up to n propagation rounds (Jacobi: every block reads the states from
-    before the round).  Round p tries the neighbours at distance 1 and, when
-    k = jumps[p % len] > 1, at distance k (inside the tile when tile is set);
-    a block descends from the first minimum when it beats its own SAD.  With
+    before the round).  Round p tries the 8 neighbours at distance 1, then at
+    each distance of jumps[p % len] (inside the tile when tile is set); a
+    block descends from the first minimum when it beats its own SAD.  With
     tile set each tile stops after its first round without a change, else
     all stop together.  Returns (mv, sad, unconv, changed flags)."""
     tid = B.tile_id() if tile else np.zeros(B.n, np.int64)
@@ -195,9 +197,8 @@ def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile):
     for p in range(n):
         if not active.any():
             break
-        k = jumps[p % len(jumps)]
         cands = []
-        for d in ((1, k) if k > 1 else (1,)):
+        for d in (1,) + tuple(jumps[p % len(jumps)]):
             for o in RING:
                 ok, nb = B.neighbour(o, d, tile)
                 cands.append((ok, mv[nb]))
@@ -227,7 +228,7 @@ def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
     _, c = B.first_min(seeds)
     mv, sad, unconv = _descend(B, c, np.ones(B.n, bool), R, iters)
     mv, sad, unconv, _ = _rounds(B, mv, sad, unconv, R, iters, local_rounds, LOCAL_JUMPS, True)
-    mv, sad, unconv, changed = _rounds(B, mv, sad, unconv, R, iters, rounds, JUMPS, False)
+    mv, sad, unconv, changed = _rounds(B, mv, sad, unconv, R, iters, rounds, GLOBAL_JUMPS, False)
     changed += [False] * (rounds - len(changed))
     nbmin = np.full(B.n, 2 ** 32 - 1, np.int64)
     for o in RING:

@@ -415,6 +415,83 @@ __global__ __launch_bounds__(256) void cnn_layer_kernel(const void *__restrict__
     }
 }
 
+// CNN_MID with the input staged in LDS: a workgroup computes an output tile
+// of CT_H x CT_W pixels of one image; its (CT_H + 2d) x (CT_W + 2d) input
+// pixels (zero outside the image) sit in LDS at 80 B per pixel (64 B of
+// channels + 16 B that spread the 16-byte operand reads of 16 consecutive
+// pixels over all banks).  Wave w computes rows 2w and 2w + 1 (32 pixels
+// each).  Same operands, order and results as cnn_layer_kernel<CNN_MID>.
+constexpr int CT_H = 8, CT_W = 32, CT_PX = 80;
+
+__global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__restrict__ in, int H, int W, int d,
+                                                           const bf16x8 *__restrict__ wp,
+                                                           const float *__restrict__ bias,
+                                                           uint16_t *__restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t tile[];
+    const int tiles_x = (W + CT_W - 1) / CT_W, tiles_y = (H + CT_H - 1) / CT_H;
+    const int n = blockIdx.x / (tiles_x * tiles_y), t = blockIdx.x % (tiles_x * tiles_y);
+    const int y0 = (t / tiles_x) * CT_H, x0 = (t % tiles_x) * CT_W;
+    const int hw = CT_W + 2 * d, hh = CT_H + 2 * d, nchunk = hh * hw * 4;
+    const long plane = (long)H * W;
+    const uint16_t *img = in + (long)n * plane * 32;
+    // stage: 16-byte chunks, 8 in flight per thread, zero outside the image
+    for (int base = 0; base < nchunk; base += 8 * 256) {
+        u32x4 v[8];
+        int idx[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            idx[u] = min(base + u * 256 + (int)threadIdx.x, nchunk - 1);
+            const int px = idx[u] >> 2, q = idx[u] & 3, r = px / hw, c = px - r * hw;
+            const int yy = y0 - d + r, xx = x0 - d + c;
+            const bool in_img = yy >= 0 && yy < H && xx >= 0 && xx < W;
+            const u32x4 *src = (const u32x4 *)(img + ((long)clampi(yy, 0, H - 1) * W + clampi(xx, 0, W - 1)) * 32) + q;
+            v[u] = *src;
+            if (!in_img) v[u] = u32x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int px = idx[u] >> 2, q = idx[u] & 3;
+            *(u32x4 *)(tile + px * CT_PX + 16 * q) = v[u];
+        }
+    }
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5, w = threadIdx.x >> 6;
+    bf16x8 A[18];
+#pragma unroll
+    for (int c = 0; c < 18; ++c) A[c] = wp[c * 64 + lane];
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bv[r] = bias[8 * (r >> 2) + 4 * lh + (r & 3)];
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int r = 2 * w + rr, y = y0 + r, x = x0 + lr;
+        f32x16 acc = {};
+#pragma unroll
+        for (int tp = 0; tp < 9; ++tp) {
+            const int tr = r + d + (tp / 3 - 1) * d, tc = lr + d + (tp % 3 - 1) * d;
+            const uint8_t *src = tile + (tr * hw + tc) * CT_PX + 16 * lh;
+            const bf16x8 b0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)src);
+            const bf16x8 b1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(src + 32));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp], b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp + 1], b1, acc, 0, 0, 0);
+        }
+        if (y < H && x < W) {
+            uint16_t *o = out + ((long)n * plane + (long)y * W + x) * 32;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t h[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
+                    const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
+                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+                }
+                *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
+            }
+        }
+    }
+}
+
 // Block proposals: mean of the block's 2x2 quarter-resolution flow, x4,
 // rounded, clamped to [-R, R].  prop[blk] = (dy, dx).
 __global__ void proposals_kernel(const float *__restrict__ flow, int H4, int W4, int R, int32_t *__restrict__ prop) {
@@ -742,17 +819,27 @@ __device__ inline int ring(int k) { return k < 4 ? k : k + 1; }
 
 // Candidate c is evaluated by lane c % 8 in slot c / 8; after the group
 // minimum every lane fetches the winner's vector from that lane.
-__device__ inline void winner(uint32_t key, const int (&my)[2], const int (&mx)[2], int &cy, int &cx) {
-    const int c = (int)(key & 31);
-    const int pick = c >= 8;
-    cy = __shfl(my[pick], c & 7, 8);
-    cx = __shfl(mx[pick], c & 7, 8);
+template <int NS>
+__device__ inline void winner(uint32_t key, const int (&my)[NS], const int (&mx)[NS], int &cy, int &cx) {
+    const int c = (int)(key & 31), pick = c >> 3;
+    int vy = my[0], vx = mx[0];
+#pragma unroll
+    for (int j = 1; j < NS; ++j)
+        if (pick == j) {
+            vy = my[j];
+            vx = mx[j];
+        }
+    cy = __shfl(vy, c & 7, 8);
+    cx = __shfl(vx, c & 7, 8);
 }
 
-// Distances of the global rounds (8, 4, 2, 1, 8, ...) and of the in-tile
-// rounds (4, 2, 1, 4, ...).
-__host__ __device__ inline int global_jump(int p) { return 8 >> (p & 3); }
+// Distances of in-tile round p: 1 and local_jump(p) (4, 2, 1: none);
+// of global round p: 1, global_jump(p, 0) and global_jump(p, 1).
 __device__ inline int local_jump(int p) { return 4 >> (p % 3); }
+__host__ __device__ inline int global_jump(int p, int i) {
+    constexpr int kJ[4][2] = {{8, 32}, {4, 16}, {2, 8}, {4, 32}};
+    return kJ[p & 3][i];
+}
 
 // The block a thread works on: 8 lanes per block, RT_Y x RT_X blocks per tile.
 struct BlockCtx {
@@ -772,16 +859,16 @@ __device__ inline BlockCtx block_ctx(const RefineArgs &a, int tile) {
     return c;
 }
 
-// First minimum over a block's candidates (lane l: candidates l and l + 8,
+// First minimum over a block's candidates (lane l: candidates l + 8 j,
 // used when ok), skipping a candidate equal to `skip_y, skip_x` when
 // skip is set (its SAD equals the block's own and never wins).
-template <class Sad>
+template <int NS, class Sad>
 __device__ inline uint32_t candidates_min(const RefineArgs &a, const uint32_t (&cw)[16], const Sad &sad, int l,
-                                          const int (&my)[2], const int (&mx)[2], const bool (&ok)[2], bool skip,
+                                          const int (&my)[NS], const int (&mx)[NS], const bool (&ok)[NS], bool skip,
                                           int skip_y, int skip_x) {
     uint32_t key = 0xffffffffu;
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NS; ++j)
         if (ok[j] && !(skip && my[j] == skip_y && mx[j] == skip_x))
             key = min(key, (sad(cw, a.R, my[j], mx[j]) << 5) | (uint32_t)(l + 8 * j));
     return group_min(key);
@@ -828,7 +915,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, i
     RefState st{0, 0, 0, 0};
     if (c.valid) {
         int cy, cx;
-        winner(candidates_min(a, cw, sad, c.l, my, mx, ok, false, 0, 0), my, mx, cy, cx);
+        winner<2>(candidates_min<2>(a, cw, sad, c.l, my, mx, ok, false, 0, 0), my, mx, cy, cx);
         st = descend(a, cw, sad, c.l, cy, cx);
     }
     const int g = c.ty * RT_X + c.tx;
@@ -849,11 +936,11 @@ __global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, i
         __syncthreads();  // all reads of the round precede its writes
         bool improved = false;
         if (c.valid) {
-            const uint32_t key = candidates_min(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
+            const uint32_t key = candidates_min<2>(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
             improved = (int)(key >> 5) < st.sad;  // no candidate: key >> 5 = 2^27 - 1
             if (improved) {
                 int cy, cx;
-                winner(key, my, mx, cy, cx);
+                winner<2>(key, my, mx, cy, cx);
                 st = descend(a, cw, sad, c.l, cy, cx);
             }
         }
@@ -863,40 +950,45 @@ __global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, i
     if (c.valid && c.l == 0) out[c.b] = st;
 }
 
-// One global round: each block tries the vectors of its 8 neighbours at
-// distance 1 and, when k > 1, at distance k (state `in`), and descends from
-// the first minimum when it beats its own SAD; SADs from global memory.
-// A round whose predecessor changed nothing does nothing.
-__global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, int k, const RefState *__restrict__ in,
+// One global round p: each block tries the vectors of its 8 neighbours at
+// distance 1, global_jump(p, 0) and global_jump(p, 1) (candidates 0..23,
+// state `in`) and descends from the first minimum when it beats its own SAD;
+// SADs from the tile's staged ref region.  A round whose predecessor changed
+// nothing does nothing.
+__global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, int p, const RefState *__restrict__ in,
                                                                   RefState *__restrict__ out,
                                                                   const int32_t *prev_changed, int32_t *changed) {
+    extern __shared__ uint32_t lds[];
     const int tile = tile_of_workgroup();
     if (tile >= a.ntiles) return;
     if (prev_changed && *prev_changed == 0) return;
+    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
     const BlockCtx c = block_ctx(a, tile);
     const int nby = a.H / 8, nbx = a.W / 8;
     RefState st = in[c.b];
-    int my[2], mx[2];
-    bool ok[2];
+    int my[3], mx[3];
+    bool ok[3];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < 3; ++j) {  // state loads first: in flight with the fill
         int nb;
-        const bool inside = neighbour(c.by, c.bx, ring(c.l), j ? k : 1, nby, nbx, nb);
-        ok[j] = c.valid && (j == 0 || k > 1) && inside;
+        const bool inside = neighbour(c.by, c.bx, ring(c.l), j ? global_jump(p, j - 1) : 1, nby, nbx, nb);
+        ok[j] = c.valid && inside;
         const RefState n = in[ok[j] ? nb : c.b];
         my[j] = n.dy;
         mx[j] = n.dx;
     }
-    uint32_t cw[16];
-    global_cur(cw, a.cur, a.pitch, c.valid ? 8 * c.by : 0, c.valid ? 8 * c.bx : 0);
-    const GlobalSad sad{a.ref, a.H, a.W, a.pitch, 8 * c.by, 8 * c.bx};
+    const int xoff = fill_tile(reg, ctile, a, (tile % a.tiles_x) * RT_X, (tile / a.tiles_x) * RT_Y);
+    __syncthreads();
     bool improved = false;
     if (c.valid) {
-        const uint32_t key = candidates_min(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
-        improved = (int)(key >> 5) < st.sad;
+        const LdsSad sad = lds_sad(reg, a, xoff, c.ty, c.tx);
+        uint32_t cw[16];
+        tile_cur(cw, ctile, c.ty, c.tx);
+        const uint32_t key = candidates_min<3>(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
+        improved = (int)(key >> 5) < st.sad;  // no candidate: key >> 5 = 2^27 - 1
         if (improved) {
             int cy, cx;
-            winner(key, my, mx, cy, cx);
+            winner<3>(key, my, mx, cy, cx);
             st = descend(a, cw, sad, c.l, cy, cx);
         }
         if (c.l == 0) out[c.b] = st;
@@ -1104,9 +1196,17 @@ umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, in
         case CNN_FIRST:
             hipLaunchKernelGGL(cnn_layer_kernel<CNN_FIRST>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
             break;
-        case CNN_MID:
+        case CNN_MID: {
+            const size_t lds = (size_t)(CT_H + 2 * dil) * (CT_W + 2 * dil) * CT_PX;
+            if (lds <= 64 * 1024) {  // staged tile; very large dilations read global memory per tap
+                const long tiles = (long)N * ceil_div(H, CT_H) * ceil_div(W, CT_W);
+                hipLaunchKernelGGL(cnn_mid_tile_kernel, dim3((unsigned)tiles), dim3(256), lds, st,
+                                   (const uint16_t *)in, H, W, dil, wp, bias, (uint16_t *)out);
+                return launch_check("cnn_mid_tile_kernel");
+            }
             hipLaunchKernelGGL(cnn_layer_kernel<CNN_MID>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
             break;
+        }
         case CNN_LAST:
             hipLaunchKernelGGL(cnn_layer_kernel<CNN_LAST>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
             break;
@@ -1161,7 +1261,7 @@ umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, i
                        fb_count);
     UMI_TRY(launch_check("refine_seed_kernel"));
     for (int p = 0; p < rounds; ++p) {
-        hipLaunchKernelGGL(refine_round_kernel, grid, block, 0, st, a, global_jump(p), buf[p & 1], buf[(p + 1) & 1],
+        hipLaunchKernelGGL(refine_round_kernel, grid, block, lds, st, a, p, buf[p & 1], buf[(p + 1) & 1],
                            p ? changed + p - 1 : (const int32_t *)nullptr, changed + p);
         UMI_TRY(launch_check("refine_round_kernel"));
     }
