@@ -305,6 +305,14 @@ umi_status umi_cnn_packed_bytes(int kind, size_t *bytes);
 umi_status umi_cnn_pack_layer(int kind, const float *w, void *packed, umi_stream s);
 umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, int dil, const void *packed,
                                const float *bias, void *out, umi_stream s);
+/* Fused forms for one frame pair: the first layer computed straight from the
+ * u8 frames (umi_cnn_downsample_pair's values, then kind 0, identical
+ * results), and the last layer (kind 2, identical flow) that also writes the
+ * block proposals of umi_me_proposals into prop when prop is not NULL. */
+umi_status umi_cnn_first_layer_frames(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                      const void *packed, const float *bias, void *out_nhwc, umi_stream s);
+umi_status umi_cnn_last_layer_proposals(const void *in_nhwc, int H4, int W4, int dil, const void *packed,
+                                        const float *bias, int R, float *flow, int32_t *prop, umi_stream s);
 /* Block proposals from the quarter-resolution flow: round(4 x mean of the
  * block's 2x2 outputs), clamped to [-R, R]; prop [H4/2][W4/2][2]. */
 umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s);
@@ -328,9 +336,10 @@ umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *p
  *   check   - a block falls back when its descent never stopped, or when its
  *             SAD exceeds slack x the smallest SAD it has at its 8 neighbours'
  *             final vectors.
- * mv_sad [H/8][W/8][3] receives (dy, dx, SAD); fallback block indices go to
- * fb_list, their number to *fb_count (cleared first), ready for
- * umi_me_full_search_listed_u8.  ws: umi_me_refine_workspace bytes; it also
+ * mv_sad [H/8][W/8][3] receives (dy, dx, SAD): for fallback blocks the full
+ * search's result over [-R, R]^2 (first minimum in raster order, as
+ * umi_me_full_search_u8), for the others the refined one.  Fallback block
+ * indices go to fb_list, their number to *fb_count (cleared first).  ws: umi_me_refine_workspace bytes; it also
  * holds, after two state buffers of 16 B per block (each rounded up to 256 B),
  * one int32 per global round that is nonzero when that round changed some
  * block. */

@@ -215,10 +215,26 @@ def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile):
     return mv, sad, unconv, changed
 
 
+def full_search_block(cur, ref, by, bx, R):
+    """(dy, dx, SAD) of block (by, bx) over [-R, R]^2, first minimum in raster
+    order, ref coordinates clamped (oracle_me_full_search's rule)."""
+    H, W = cur.shape
+    ys = np.clip(np.arange(by * 8 - R, by * 8 + 8 + R), 0, H - 1)
+    xs = np.clip(np.arange(bx * 8 - R, bx * 8 + 8 + R), 0, W - 1)
+    win = ref[np.ix_(ys, xs)].astype(np.int64)
+    c = cur[by * 8:by * 8 + 8, bx * 8:bx * 8 + 8].astype(np.int64)
+    S = 2 * R + 1
+    v = np.lib.stride_tricks.sliding_window_view(win, (8, 8))[:S, :S]
+    sads = np.abs(v - c).sum((2, 3))
+    i = int(np.argmin(sads))  # first minimum in raster order
+    return i // S - R, i % S - R, int(sads.flat[i])
+
+
 def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
     """Refinement of block proposals restating umi_me_refine: seed, descent,
-    in-tile rounds, global rounds, fallback check.  Returns (mv_sad [n][3],
-    fallback set, changed flags of the global rounds)."""
+    in-tile rounds, global rounds, fallback check, full search of the
+    fallback blocks.  Returns (mv_sad [n][3], fallback set, changed flags of
+    the global rounds)."""
     B = _Blocks(cur, ref)
     prop = np.clip(np.asarray(prop, np.int64).reshape(B.n, 2), -R, R)
     seeds = [(np.ones(B.n, bool), prop)]
@@ -236,4 +252,7 @@ def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
         nbmin = np.minimum(nbmin, np.where(ok, B.sad(mv[nb]), 2 ** 32 - 1))
     worse = sad.astype(np.float32) > np.float32(slack) * np.minimum(sad, nbmin).astype(np.float32)
     fb = set(np.nonzero(unconv | worse)[0].tolist())
-    return np.concatenate([mv, sad[:, None]], axis=1), fb, changed
+    out = np.concatenate([mv, sad[:, None]], axis=1)
+    for b in fb:
+        out[b] = full_search_block(cur, ref, b // B.nbx, b % B.nbx, R)
+    return out, fb, changed

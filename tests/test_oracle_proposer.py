@@ -92,6 +92,18 @@ def test_refine_block_sad_matches_c_oracle_volume():
         np.testing.assert_array_equal(B.sad(mv), want)
 
 
+def test_full_search_block_matches_c_oracle():
+    """The per-block full search of the fallback blocks equals
+    oracle_me_full_search (first minimum in raster order, clamped ref)."""
+    for H, W, R, seed in ((40, 56, 6, 11), (48, 64, 16, 12), (24, 32, 3, 13)):
+        cur, ref = _noise(H, W, seed), _noise(H, W, seed + 100)
+        ref[1::2] = ref[0::2][:ref[1::2].shape[0]]  # repeated rows make ties
+        mv, _ = oracle.me_full_search(cur, ref, 8, R)
+        for by in range(H // 8):
+            for bx in range(W // 8):
+                assert P.full_search_block(cur, ref, by, bx, R) == tuple(int(v) for v in mv[by, bx])
+
+
 def test_refine_exact_shift_spreads_from_one_block():
     """cur = ref shifted by (5, -7) with clamped coordinates, so that vector
     has SAD 0 for every block.  All proposals are wrong except block 37's;
@@ -111,9 +123,12 @@ def test_refine_exact_shift_spreads_from_one_block():
     assert (out[:, 2] == 0).all() and (out[inner, :2] == (5, -7)).all()
     assert all(b % nbx == 0 for b in fb)
     assert changed[0] and not all(changed)
-    # without propagation rounds only the seed's 3x3 neighbourhood finds it
-    out0, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=0, slack=1.1)
-    assert 0 < np.count_nonzero(out0[:, 2] == 0) <= 9
+    # without propagation rounds only the seed's 3x3 neighbourhood finds it;
+    # the blocks the check sends to the full search come back exact
+    out0, fb0, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=0, slack=1.1)
+    kept = np.array([b not in fb0 for b in range(n)])
+    assert 0 < np.count_nonzero(out0[kept, 2] == 0) <= 9
+    assert (out0[~kept, 2] == 0).all()
 
 
 def test_refine_in_tile_rounds_stay_in_the_tile():
@@ -127,20 +142,24 @@ def test_refine_in_tile_rounds_stay_in_the_tile():
     nby, nbx = H // 8, W // 8
     prop = np.tile(np.array([[-20, 20]]), (nby * nbx, 1))
     prop[37] = (5, -7)
-    out, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=0, slack=1.1)
+    out, fb, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=0, slack=1.1)
     by, bx = np.divmod(np.arange(nby * nbx), nbx)
     tile = (by < 4) & (bx >= 8) & (bx < 16)
+    kept = np.array([b not in fb for b in range(nby * nbx)])
     assert P.TILE == (4, 8)
-    np.testing.assert_array_equal(out[:, 2] == 0, tile)
+    assert not (fb & set(np.nonzero(tile)[0].tolist()))
+    np.testing.assert_array_equal(out[kept, 2] == 0, tile[kept])
     # a global round then reaches the neighbouring tiles
-    out1, _, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=1, slack=1.1)
-    assert np.count_nonzero(out1[:, 2] == 0) > np.count_nonzero(tile)
+    out1, fb1, _ = P.refine(cur, ref, prop, R, iters=8, local_rounds=8, rounds=1, slack=1.1)
+    kept1 = np.array([b not in fb1 for b in range(nby * nbx)])
+    assert np.count_nonzero(out1[kept1, 2] == 0) > np.count_nonzero(tile)
 
 
 def test_refine_results_are_local_minima_and_pass_the_check():
     """Every block that does not fall back sits at a minimum of its +-1
     neighbourhood, its SAD is the scalar block SAD at its vector, and it passes
-    the neighbour test recomputed with scalar SADs."""
+    the neighbour test recomputed with scalar SADs; every fallback block holds
+    the full search's result."""
     from umi_amd import workloads
     H, W, R, slack = 96, 136, 16, 1.1
     cur, ref = workloads.smooth_motion_pair(H, W, R, seed=7)
@@ -154,6 +173,7 @@ def test_refine_results_are_local_minima_and_pass_the_check():
         dy, dx, s = (int(v) for v in out[b])
         assert s == P.block_sad(cur, ref, by, bx, dy, dx)
         if b in fb:
+            assert (dy, dx, s) == P.full_search_block(cur, ref, by, bx, R)
             continue
         for oy in (-1, 0, 1):
             for ox in (-1, 0, 1):
@@ -166,8 +186,11 @@ def test_refine_results_are_local_minima_and_pass_the_check():
 
 
 def test_refine_fallback_threshold_zero_forces_full_search():
-    """slack 0 flags every block with a nonzero SAD (the host's way to force
-    the full search)."""
+    """slack 0 flags every block with a nonzero refined SAD (the host's way to
+    force the full search): on noise every block falls back and the result is
+    the full search's."""
     cur, ref = _noise(64, 64, 9), _noise(64, 64, 10)
     out, fb, _ = P.refine(cur, ref, np.zeros((64, 2), np.int64), 8, iters=4, local_rounds=2, rounds=2, slack=0.0)
-    assert fb == set(np.nonzero(out[:, 2] > 0)[0].tolist())
+    assert fb == set(range(64))
+    full, _ = oracle.me_full_search(cur, ref, 8, 8)
+    np.testing.assert_array_equal(out, full.reshape(64, 3))

@@ -5,7 +5,8 @@
   differences) on a small input;
 * Adam matches its formula; training lowers the loss;
 * the bf16 MFMA inference layers match the FP64 forward pass with the same
-  bf16 roundings;
+  bf16 roundings; the fused first (from the frames) and last (with the
+  proposals) layers equal the unfused ones bit for bit;
 * refinement (seed, descent, in-tile and global propagation rounds, fallback
   check) matches the oracle exactly, including ragged frames, padded rows,
   the largest radius and edge settings; the
@@ -119,6 +120,53 @@ def test_inference_layers_match_bf16_oracle(gpu, N, H4, W4):
     assert np.abs(a0 - w0).max() <= 2 ** -7 * np.abs(w0).max()
 
 
+@pytest.mark.parametrize("H,W", [(72, 136), (1080, 1920), (44, 70)])
+def test_fused_first_layer_equals_downsample_path(gpu, H, W):
+    """umi_cnn_first_layer_frames = umi_cnn_downsample_pair + layer kind 0, bit for bit."""
+    g = workloads.rng(H + W)
+    cur = g.integers(0, 256, size=(H, W)).astype(np.uint8)
+    ref = g.integers(0, 256, size=(H, W)).astype(np.uint8)
+    net = proposer.Network(seed=5)
+    packed = proposer.pack_layer(net, 0)
+    H4, W4 = H // 4, W // 4
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    x = DeviceBuffer(2 * H4 * W4 * 4)
+    a, b = DeviceBuffer(H4 * W4 * 64), DeviceBuffer(H4 * W4 * 64)
+    print(f"[fused] first layer H={H} W={W}", flush=True)
+    call("umi_cnn_downsample_pair", dc.ptr, dr.ptr, H, W, W, x.ptr, 0, None)
+    call("umi_cnn_infer_layer", 0, x.ptr, 1, H4, W4, 1, packed.ptr, net.b[0].ptr, a.ptr, None)
+    call("umi_cnn_first_layer_frames", dc.ptr, dr.ptr, H, W, W, packed.ptr, net.b[0].ptr, b.ptr, None)
+    np.testing.assert_array_equal(a.download((H4 * W4 * 32,), np.uint16), b.download((H4 * W4 * 32,), np.uint16))
+
+
+@pytest.mark.parametrize("H4,W4,R", [(270, 480, 32), (19, 37, 5), (8, 16, 32)])
+def test_fused_last_layer_equals_flow_and_proposals(gpu, H4, W4, R):
+    """umi_cnn_last_layer_proposals writes layer kind 2's flow and
+    umi_me_proposals' block proposals, bit for bit."""
+    g = workloads.rng(H4 * W4)
+    act = (g.standard_normal((H4, W4, 32)).astype(np.float32) * 3.0)
+    act16 = (act.view(np.uint32) >> 16).astype(np.uint16)
+    net = proposer.Network(seed=6)
+    net.b[3].upload(np.array([0.3, -0.7], np.float32))
+    packed = proposer.pack_layer(net, 3)
+    di = DeviceBuffer.from_array(act16)
+    f1, f2 = DeviceBuffer(2 * H4 * W4 * 4), DeviceBuffer(2 * H4 * W4 * 4)
+    nb = (H4 // 2) * (W4 // 2)
+    p1, p2 = DeviceBuffer(max(nb, 1) * 8), DeviceBuffer(max(nb, 1) * 8)
+    print(f"[fused] last layer H4={H4} W4={W4} R={R}", flush=True)
+    call("umi_cnn_infer_layer", 2, di.ptr, 1, H4, W4, 8, packed.ptr, net.b[3].ptr, f1.ptr, None)
+    call("umi_me_proposals", f1.ptr, H4, W4, R, p1.ptr, None)
+    call("umi_cnn_last_layer_proposals", di.ptr, H4, W4, 8, packed.ptr, net.b[3].ptr, R, f2.ptr, p2.ptr, None)
+    np.testing.assert_array_equal(f1.download((2, H4, W4), np.float32), f2.download((2, H4, W4), np.float32))
+    np.testing.assert_array_equal(p1.download((nb, 2), np.int32), p2.download((nb, 2), np.int32))
+    flow = f1.download((2, H4, W4), np.float32).astype(np.float64)
+    want = np.rint(flow[:, :H4 // 2 * 2:2, :W4 // 2 * 2:2] + flow[:, :H4 // 2 * 2:2, 1:W4 // 2 * 2:2]
+                   + flow[:, 1:H4 // 2 * 2:2, :W4 // 2 * 2:2] + flow[:, 1:H4 // 2 * 2:2, 1:W4 // 2 * 2:2])
+    want = np.clip(want, -R, R).transpose(1, 2, 0).reshape(nb, 2)
+    # float32 rounding of the quad sum can move a value sitting at .5 by one
+    assert np.abs(p2.download((nb, 2), np.int32) - want).max() <= 1
+
+
 def _refine_gpu(cur, ref, prop, R, iters, local_rounds, rounds, slack, pitch=None):
     H, W = cur.shape
     pitch = pitch or W
@@ -179,12 +227,13 @@ def test_listed_search_completes_fallbacks(gpu):
     prop = workloads.rng(11).integers(-R, R + 1, size=(n, 2)).astype(np.int32)
     before, fb, _, (dc, dr, fl, fc, mv) = _refine_gpu(cur, ref, prop, R, 2, 0, 1, 1.1)
     assert 0 < len(fb) < n
-    call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, 8, R, fl.ptr, fc.ptr, n, mv.ptr, None)
     full, _ = ops.me_full_search_np(cur, ref, 8, R)
-    after = mv.download((n, 3), np.int32)
     full = full.reshape(n, 3)
     listed = np.zeros(n, bool)
     listed[fb] = True
+    np.testing.assert_array_equal(before[listed], full[listed])  # the refinement's own full search
+    call("umi_me_full_search_listed_u8", dc.ptr, dr.ptr, H, W, W, 8, R, fl.ptr, fc.ptr, n, mv.ptr, None)
+    after = mv.download((n, 3), np.int32)
     np.testing.assert_array_equal(after[listed], full[listed])
     np.testing.assert_array_equal(after[~listed], before[~listed])
 

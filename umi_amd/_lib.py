@@ -82,6 +82,8 @@ SIGNATURES = {
     "umi_cnn_packed_bytes": [c_int, P],
     "umi_cnn_pack_layer": [c_int, P, P, P],
     "umi_cnn_infer_layer": [c_int, P, c_int, c_int, c_int, c_int, P, P, P, P],
+    "umi_cnn_first_layer_frames": [P, P, c_int, c_int, c_int, P, P, P, P],
+    "umi_cnn_last_layer_proposals": [P, c_int, c_int, c_int, P, P, c_int, P, P, P],
     "umi_me_proposals": [P, c_int, c_int, c_int, P, P],
     "umi_me_refine_workspace": [c_int, c_int, c_int, P],
     "umi_me_refine": [P, P, c_int, c_int, c_int, c_int, P, c_int, c_int, c_int, c_float, P, P, P, P, c_size_t, P],

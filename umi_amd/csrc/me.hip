@@ -377,7 +377,7 @@ extern "C" umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uin
     const int max_threads = B == 8 ? 512 : 256;
     const int DCH = umi::ceil_div(S, max_threads / G);
     const int threads = umi::round_up(G * umi::ceil_div(S, DCH), 64);
-    const int grid = max_count < 2048 ? max_count : 2048;
+    const int grid = max_count < 512 ? max_count : 512;  // persistent: each workgroup walks the list
     hipStream_t st = umi::as_stream(s);
     if (B == 8)
         hipLaunchKernelGGL((me_listed_kernel<8>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R, W / B,

@@ -492,6 +492,136 @@ __global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__res
     }
 }
 
+// The first layer read straight from the frames: the workgroup computes its
+// (CT_H + 2) x (CT_W + 2) input pixels (dilation 1) of the quarter-
+// resolution pair, (mean of the 4x4 pixels - 128) / 8 of cur and of ref as
+// in umi_cnn_downsample_pair, rounded to bf16 (zero outside the quarter-
+// resolution image), into LDS, then runs cnn_layer_kernel<CNN_FIRST>'s
+// operands and order on them: bf16 NHWC output, ReLU.  One image (N = 1).
+__global__ __launch_bounds__(256) void cnn_first_frames_kernel(const uint8_t *__restrict__ cur,
+                                                               const uint8_t *__restrict__ ref, int H, int W,
+                                                               int pitch, const bf16x8 *__restrict__ wp,
+                                                               const float *__restrict__ bias,
+                                                               uint16_t *__restrict__ out) {
+    constexpr int HW = CT_W + 2, HH = CT_H + 2;
+    __shared__ uint16_t xin[HH * HW][2];
+    const int H4 = H / 4, W4 = W / 4;
+    const int tiles_x = (W4 + CT_W - 1) / CT_W;
+    const int y0 = (blockIdx.x / tiles_x) * CT_H, x0 = (blockIdx.x % tiles_x) * CT_W;
+    for (int i = threadIdx.x; i < HH * HW * 2; i += 256) {
+        const int px = i >> 1, ch = i & 1, r = px / HW, c = px - r * HW;
+        const int Y = y0 - 1 + r, X = x0 - 1 + c;
+        float v = 0.f;
+        if (Y >= 0 && Y < H4 && X >= 0 && X < W4) {
+            const uint8_t *f = (ch ? ref : cur) + (size_t)(4 * Y) * pitch + 4 * X;
+            int sum = 0;
+#pragma unroll
+            for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 4; ++dx) sum += f[(size_t)dy * pitch + dx];
+            v = (sum * (1.f / 16.f) - 128.f) * 0.125f;
+        }
+        xin[px][ch] = bf16_rne(v);
+    }
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5, w = threadIdx.x >> 6;
+    const bf16x8 A0 = wp[lane], A1 = wp[64 + lane];
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bv[r] = bias[8 * (r >> 2) + 4 * lh + (r & 3)];
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int r = 2 * w + rr, y = y0 + r, x = x0 + lr;
+        bf16x8 B[2];
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            uint16_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 16 * ch + 8 * lh + j, t = k >> 1;
+                v[j] = k < 18 ? xin[(r + 1 + t / 3 - 1) * HW + lr + 1 + t % 3 - 1][k & 1] : (uint16_t)0;
+            }
+            B[ch] = __builtin_bit_cast(bf16x8, v);
+        }
+        f32x16 acc = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0, B[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, B[1], acc, 0, 0, 0);
+        if (y < H4 && x < W4) {
+            uint16_t *o = out + ((long)y * W4 + x) * 32;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t h[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
+                    const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
+                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+                }
+                *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
+            }
+        }
+    }
+}
+
+// The last layer with the block proposals: a 32-pixel group is 2 rows x 16
+// columns (lane lr: row lr / 16, column lr % 16), so each 2x2 quad of the
+// flow lies in one group.  Per pixel the operands and order of
+// cnn_layer_kernel<CNN_LAST>; the flow is written as that kernel writes it,
+// and, when prop is given, the proposal of block (Y / 2, X / 2) is
+// round(4 x ((f(Y, X) + f(Y, X + 1)) + f(Y + 1, X)) + f(Y + 1, X + 1)) / 4),
+// clamped to [-R, R], as umi_me_proposals.  One image (N = 1).
+__global__ __launch_bounds__(256) void cnn_last_prop_kernel(const uint16_t *__restrict__ in, int H, int W, int d,
+                                                            const bf16x8 *__restrict__ wp,
+                                                            const float *__restrict__ bias, int R,
+                                                            float *__restrict__ flow, int32_t *__restrict__ prop) {
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+    bf16x8 A[18];
+#pragma unroll
+    for (int c = 0; c < 18; ++c) A[c] = wp[c * 64 + lane];
+    const float b0 = bias[0], b1 = bias[1];
+    const int gx = (W + 15) / 16, gy = (H + 1) / 2, ngroups = gx * gy;
+    const long plane = (long)H * W;
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += nwaves) {
+        const int y = (g / gx) * 2 + (lr >> 4), x = (g % gx) * 16 + (lr & 15);
+        const bool valid = y < H && x < W;
+        bf16x8 B[18];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + (t / 3 - 1) * d, xx = x + (t % 3 - 1) * d;
+            u32x4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+            if (valid && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const u32x4 *src = (const u32x4 *)(in + (((long)yy * W + xx) * 32 + 8 * lh));
+                v0 = src[0];
+                v1 = src[2];
+            }
+            B[2 * t] = __builtin_bit_cast(bf16x8, v0);
+            B[2 * t + 1] = __builtin_bit_cast(bf16x8, v1);
+        }
+        f32x16 acc = {};
+#pragma unroll
+        for (int c = 0; c < 18; ++c) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[c], B[c], acc, 0, 0, 0);
+        const float f0 = acc[0] + b0, f1 = acc[1] + b1;  // rows 0, 1 of lanes 0..31
+        if (lh == 0 && valid) {
+            flow[(long)y * W + x] = f0;
+            flow[plane + (long)y * W + x] = f1;
+        }
+        if (prop) {
+            // quad of lane 2q (row 0): +1 (column + 1), +16 (row + 1), +17
+            const float r0 = __shfl(f0, lane + 1), r1 = __shfl(f1, lane + 1);
+            const float d0 = __shfl(f0, lane + 16), d1 = __shfl(f1, lane + 16);
+            const float e0 = __shfl(f0, lane + 17), e1 = __shfl(f1, lane + 17);
+            const int X = x / 2, Y = y / 2;
+            if (lh == 0 && lr < 16 && (lr & 1) == 0 && Y < H / 2 && X < W / 2) {
+                const float m0 = (f0 + r0 + d0 + e0) * 0.25f * 4.f, m1 = (f1 + r1 + d1 + e1) * 0.25f * 4.f;
+                int32_t *o = prop + ((long)Y * (W / 2) + X) * 2;
+                o[0] = clampi((int)rintf(m0), -R, R);
+                o[1] = clampi((int)rintf(m1), -R, R);
+            }
+        }
+    }
+}
+
 // Block proposals: mean of the block's 2x2 quarter-resolution flow, x4,
 // rounded, clamped to [-R, R].  prop[blk] = (dy, dx).
 __global__ void proposals_kernel(const float *__restrict__ flow, int H4, int W4, int R, int32_t *__restrict__ prop) {
@@ -1047,11 +1177,11 @@ __global__ __launch_bounds__(RT_THREADS) void refine_final_kernel(RefineArgs a, 
     const int last = zero ? __ffsll((unsigned long long)zero) - 1 : rounds - 1;
     const bool one = (last + 1) & 1;
     const RefState own = one ? own1 : own0, n = one ? nb1 : nb0;
-    const GlobalSad sad{a.ref, a.H, a.W, a.pitch, 8 * c.by, 8 * c.bx};
+    const GlobalSad gsad{a.ref, a.H, a.W, a.pitch, 8 * c.by, 8 * c.bx};
     bool fb = false;
     if (c.valid) {
         uint32_t m = 0xffffffffu;
-        if (ok && (n.dy != own.dy || n.dx != own.dx)) m = sad(cw, a.R, n.dy, n.dx);
+        if (ok && (n.dy != own.dy || n.dx != own.dx)) m = gsad(cw, a.R, n.dy, n.dx);
         m = group_min(m);
         const bool worse = (float)own.sad > slack * (float)min((uint32_t)own.sad, m);
         fb = own.unconv || worse;
@@ -1215,6 +1345,28 @@ umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, in
     return launch_check("cnn_layer_kernel");
 }
 
+umi_status umi_cnn_first_layer_frames(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                      const void *packed, const float *bias, void *out_nhwc, umi_stream s) {
+    if (!cur || !ref || !packed || !bias || !out_nhwc)
+        return set_error(UMI_ERR_INVALID, "umi_cnn_first_layer_frames: null pointer");
+    if (H < 4 || W < 4 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_cnn_first_layer_frames: bad frame");
+    const int tiles = ceil_div(H / 4, CT_H) * ceil_div(W / 4, CT_W);
+    hipLaunchKernelGGL(cnn_first_frames_kernel, dim3(tiles), dim3(256), 0, as_stream(s), cur, ref, H, W, pitch,
+                       (const bf16x8 *)packed, bias, (uint16_t *)out_nhwc);
+    return launch_check("cnn_first_frames_kernel");
+}
+
+umi_status umi_cnn_last_layer_proposals(const void *in_nhwc, int H4, int W4, int dil, const void *packed,
+                                        const float *bias, int R, float *flow, int32_t *prop, umi_stream s) {
+    if (!in_nhwc || !packed || !bias || !flow) return set_error(UMI_ERR_INVALID, "umi_cnn_last_layer_proposals: null");
+    if (H4 < 1 || W4 < 1 || dil < 1 || R < 0) return set_error(UMI_ERR_INVALID, "umi_cnn_last_layer_proposals: shape");
+    const long groups = (long)((H4 + 1) / 2) * ((W4 + 15) / 16);
+    const unsigned grid = (unsigned)std::min<long>((groups + 3) / 4, 256L * 2);
+    hipLaunchKernelGGL(cnn_last_prop_kernel, dim3(grid), dim3(256), 0, as_stream(s), (const uint16_t *)in_nhwc, H4,
+                       W4, dil, (const bf16x8 *)packed, bias, R, flow, prop);
+    return launch_check("cnn_last_prop_kernel");
+}
+
 umi_status umi_me_proposals(const float *flow, int H4, int W4, int R, int32_t *prop, umi_stream s) {
     if (!flow || !prop) return set_error(UMI_ERR_INVALID, "umi_me_proposals: null");
     const int nb = (H4 / 2) * (W4 / 2);
@@ -1267,7 +1419,9 @@ umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, i
     }
     hipLaunchKernelGGL(refine_final_kernel, grid, block, 0, st, a, buf[0], buf[1], changed, rounds, slack, mv_sad,
                        fb_list, fb_count);
-    return launch_check("refine_final_kernel");
+    UMI_TRY(launch_check("refine_final_kernel"));
+    // the fallback blocks: one workgroup each, over the list the final pass wrote
+    return umi_me_full_search_listed_u8(cur, ref, H, W, pitch, 8, R, fb_list, fb_count, nb, mv_sad, s);
 }
 
 }  // extern "C"

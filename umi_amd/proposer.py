@@ -278,16 +278,16 @@ class Proposer:
         self.ws = DeviceBuffer(n.value)
 
     def propose(self, cur: DeviceBuffer, ref: DeviceBuffer, stream=None):
+        """Network and block proposals: the first layer reads the frames, the
+        last one writes the proposals (4 launches)."""
         s = _s(stream)
-        call("umi_cnn_downsample_pair", cur.ptr, ref.ptr, self.H, self.W, self.W, self.x.ptr, 0, s)
-        src = self.x
-        for li, (ci, co, d, _) in enumerate(LAYERS):
-            kind = layer_kind(li)
-            dst = self.flow if kind == 2 else self.a[li]
-            call("umi_cnn_infer_layer", kind, src.ptr, 1, self.H4, self.W4, d, self.packed[li].ptr,
-                 self.net.b[li].ptr, dst.ptr, s)
-            src = dst
-        call("umi_me_proposals", self.flow.ptr, self.H4, self.W4, self.R, self.prop.ptr, s)
+        call("umi_cnn_first_layer_frames", cur.ptr, ref.ptr, self.H, self.W, self.W, self.packed[0].ptr,
+             self.net.b[0].ptr, self.a[0].ptr, s)
+        for li in (1, 2):
+            call("umi_cnn_infer_layer", 1, self.a[li - 1].ptr, 1, self.H4, self.W4, LAYERS[li][2], self.packed[li].ptr,
+                 self.net.b[li].ptr, self.a[li].ptr, s)
+        call("umi_cnn_last_layer_proposals", self.a[2].ptr, self.H4, self.W4, LAYERS[3][2], self.packed[3].ptr,
+             self.net.b[3].ptr, self.R, self.flow.ptr, self.prop.ptr, s)
 
     def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, self.iters,
@@ -295,10 +295,8 @@ class Proposer:
              self.ws.ptr, self.ws.nbytes, _s(stream))
 
     def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
-        s = _s(stream)
+        """Refinement; the fallback blocks come back with the full search's result."""
         self.refine_only(cur, ref, mv_sad, stream)
-        call("umi_me_full_search_listed_u8", cur.ptr, ref.ptr, self.H, self.W, self.W, 8, self.R, self.fb_list.ptr,
-             self.fb_count.ptr, self.nblk, mv_sad.ptr, s)
 
     def run(self, cur, ref, mv_sad, stream=None, proposals=True):
         """proposals=False: the same refinement and fallback around zero motion
