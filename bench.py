@@ -66,16 +66,41 @@ def hbm_copy_probe(st, mb=256):
     return {"bytes": 2 * n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
 
 
+def learned_proposer(st):
+    """Learned-proposal ME on held-out 1920x1080 pairs (+-32 search, 8x8
+    blocks; smooth motion from 4 Gaussian-weighted affine bumps, |mv| <= 32,
+    noise sigma 2): the committed network, its refinement and fallbacks,
+    against the plain +-32 full search; the same refinement from zero
+    proposals shows what the network adds."""
+    from umi_amd import proposer
+
+    if not proposer.CHECKPOINT.exists():
+        return None
+    net = proposer.Network.load(proposer.CHECKPOINT)
+    prop = proposer.Proposer(net)
+    pairs, seed0 = 3, 2_000_000
+    with_net = proposer.evaluate(prop, pairs, seed0, proposals=True, reps=20)
+    without = proposer.evaluate(prop, pairs, seed0, proposals=False, reps=20)
+    return {"pairs": pairs, "first_seed": seed0, "search": "+-32, 8x8 blocks, 32400 blocks per pair",
+            "settings": {"iters": prop.iters, "local_rounds": prop.local_rounds, "rounds": prop.rounds,
+                         "slack": prop.slack},
+            "blocks_per_s_pipeline": prop.nblk / (with_net["ms_pipeline"] * 1e-3),
+            "blocks_per_s_full_search": prop.nblk / (with_net["ms_full_search"] * 1e-3),
+            "with_network": with_net, "without_network": without}
+
+
 def extra_configs(st):
-    """Secondary configurations, each on its own workload (not part of `value`):
-    C3 GEMM 4096^3 and conv 16x64x56x56 K64 3x3 in five precision forms with
-    FP64-oracle error; C2 cost volume; C1 filter; C4 uneven batch."""
+    """Secondary workloads, each on its own data (not part of `value`): GEMM
+    4096^3 and conv 16x64x56x56 -> 64, 3x3, in five precision forms with FP64
+    error; stereo cost volume 1242x375, 128 disparities, 5x5; weighted 3x3
+    filter on 1024^2 float32; 256 uneven images in graph-captured batched
+    launches; learned-proposal motion estimation on 1920x1080 pairs."""
     import oracle
     from umi_amd import DeviceBuffer, batch, ops, workloads
 
     out = {}
     g = workloads.rng(3)
-    # --- C3 GEMM (error on 64 sampled rows against FP64)
+    # --- GEMM 4096^3 (error on 64 sampled rows against FP64)
     M = N = K = 4096
     A64, B64 = g.standard_normal((M, K)), g.standard_normal((K, N))
     dA, dB = DeviceBuffer.from_array(A64.astype(np.float32)), DeviceBuffer.from_array(B64.astype(np.float32))
@@ -91,7 +116,7 @@ def extra_configs(st):
         e = oracle.contraction_error(C, ref, np.linalg.norm(A64[rows], axis=1), np.linalg.norm(B64, axis=0))
         gemm[ops.MODE_NAMES[mode]] = {"ms": ms, "tflops": 2 * M * N * K / (ms * 1e-3) / 1e12, **e}
     out["gemm_4096"] = gemm
-    # --- C3 conv (full FP64 oracle)
+    # --- conv 16x64x56x56 -> 64, 3x3 (full FP64 oracle)
     x64, w64 = g.standard_normal((16, 64, 56, 56)), g.standard_normal((64, 64, 3, 3))
     ref = oracle.conv_f64(x64, w64, 1, 1)
     cols, _ = oracle.im2col_f64(x64, 3, 3, 1, 1)
@@ -109,19 +134,19 @@ def extra_configs(st):
         conv[ops.MODE_NAMES[mode]] = {"us": ms * 1e3, "tflops": flops / (ms * 1e-3) / 1e12,
                                       "max_rel": float(rel.max()), "mean_rel": float(rel.mean())}
     out["conv_16x64x56x56_k64_3x3"] = conv
-    # --- C2 cost volume
+    # --- stereo cost volume
     L, R, _ = workloads.stereo_pair(375, 1242, 128, seed=1)
     dl, dr, cv = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R), DeviceBuffer(375 * 1242 * 128 * 4)
     ms = timed(lambda: ops.cost_volume(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, cv, stream=st), st, 20)
     out["cost_volume_1242x375_d128_5x5"] = {"ms": ms, "gbs": (375 * 1242 * 128 * 4 + 2 * 375 * 1242) / (ms * 1e-3) / 1e9}
-    # --- C1 filter
+    # --- weighted 3x3 filter
     img = g.random((1024, 1024), dtype=np.float32)
     wt = g.normal(size=9)
     di, do = DeviceBuffer.from_array(img), DeviceBuffer(img.nbytes)
     dwt = DeviceBuffer.from_array((wt / wt.sum()).astype(np.float32))
     ms = timed(lambda: ops.wfilter3x3(di, 1024, 1024, 1024, dwt, 0.0, 0.0, 0, do, 1024, stream=st), st, 100)
     out["wfilter3x3_1024_f32"] = {"us": ms * 1e3, "gbs": 8 * 1024 * 1024 / (ms * 1e-3) / 1e9}
-    # --- C4 uneven batch, graph-captured launches
+    # --- 256 uneven images, graph-captured batched launches
     imgs, u8 = workloads.uneven_batch(256, 64, 1024, seed=1)
     bf, bu = batch.ImageBatch(imgs), batch.ImageBatch(u8, out_dtype=np.uint32)
     px = bf.total
@@ -135,6 +160,9 @@ def extra_configs(st):
                                "bilateral_ms": t_bl, "bilateral_gpx_s": px / (t_bl * 1e-3) / 1e9,
                                "iimg_u8_ms": t_iu, "iimg_u8_gbs": 5 * px / (t_iu * 1e-3) / 1e9,
                                "iimg_f32_ms": t_if, "iimg_f32_gbs": 8 * px / (t_if * 1e-3) / 1e9}
+    lp = learned_proposer(st)
+    if lp is not None:
+        out["learned_proposal_me_1080p_r32"] = lp
     return out
 
 
@@ -171,8 +199,8 @@ def cpu_baseline(cur, ref, seconds):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {a.gpus}); the frame-band sharding of the "
+              "search across GPUs is a design, not run here.", file=sys.stderr)
         sys.exit(2)
 
     import umi_amd

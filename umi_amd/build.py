@@ -70,7 +70,6 @@ def build_library(force: bool = True, jobs: int | None = None) -> Path:
 
 
 TEST_CPP = ROOT / "tests" / "cpp"
-REF_MAIN = Path("/root/reference/main")
 
 
 def _hipcc_program(src: Path, out: Path, extra=()) -> None:
@@ -95,20 +94,13 @@ def build_tests() -> list[Path]:
 
 
 def build_dropin() -> list[Path]:
-    """The reference's own example programs (main/*.cu, read where they lie in
-    /root/reference) compiled unchanged against this project's UMI headers into
-    oracle/_ref/<name>_gpu.  Each program checks its results with its own CPU
-    loop.  main/bilateral.cu calls two CUDA runtime functions outside the UMI
-    API (bilateral.cu:101); this recipe maps them to their HIP equivalents on the
-    command line.  Skipped when /root/reference is absent."""
-    if not REF_MAIN.is_dir():
-        return []
-    outdir = ROOT / "oracle" / "_ref"
-    srcs = sorted(REF_MAIN.glob("*.cu"))
-    extra = ["-DcudaGetLastError=hipGetLastError", "-DcudaGetErrorString=hipGetErrorString", "-UNDEBUG", "-w"]
-    with cf.ThreadPoolExecutor(max_workers=max(1, len(srcs))) as ex:
-        list(ex.map(lambda s: _hipcc_program(s, outdir / f"{s.stem}_gpu", extra), srcs))
-    return [outdir / f"{s.stem}_gpu" for s in srcs]
+    """The reference's example programs against this library (oracle/build_ref.py)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    try:
+        import build_ref
+    finally:
+        sys.path.pop(0)
+    return build_ref.build()
 
 
 def build_oracle() -> Path:
