@@ -272,7 +272,8 @@ umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uint8_t *ref, 
 umi_status umi_synth_frame0(uint8_t *f0, int H, int W, int pitch, uint64_t seed, umi_stream s);
 umi_status umi_synth_warp(const uint8_t *f0, uint8_t *f1, int H, int W, int pitch, const float *bumps, float sigma,
                           uint64_t seed, umi_stream s);
-/* Network input: fp32 [n][2][H/4][W/4], (4x4 box mean - 128) / 8. */
+/* Network input of pair slot n (n >= 0) of an fp32 [.][2][H/4][W/4] array:
+ * channel 0 cur, channel 1 ref, (4x4 box mean - 128) / 8. */
 umi_status umi_cnn_downsample_pair(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, float *out, int n,
                                    umi_stream s);
 /* Training kernels, fp32 NCHW, 3x3 convolution with dilation dil and zero

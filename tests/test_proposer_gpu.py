@@ -120,6 +120,23 @@ def test_inference_layers_match_bf16_oracle(gpu, N, H4, W4):
     assert np.abs(a0 - w0).max() <= 2 ** -7 * np.abs(w0).max()
 
 
+def test_downsample_matches_numpy(gpu):
+    """Quarter-resolution input: (mean of each 4x4 block - 128) / 8 of cur and
+    ref; every step is exact in float32, so the comparison is bit-exact."""
+    H, W = 92, 138
+    g = workloads.rng(12)
+    cur = g.integers(0, 256, size=(H, W)).astype(np.uint8)
+    ref = g.integers(0, 256, size=(H, W)).astype(np.uint8)
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    H4, W4 = H // 4, W // 4
+    x = DeviceBuffer(2 * 2 * H4 * W4 * 4)
+    call("umi_cnn_downsample_pair", dc.ptr, dr.ptr, H, W, W, x.ptr, 1, None)  # pair slot 1 of 2
+    got = x.download((2, 2, H4, W4), np.float32)[1]
+    for ch, f in enumerate((cur, ref)):
+        m = f[:H4 * 4, :W4 * 4].reshape(H4, 4, W4, 4).astype(np.float64).sum((1, 3))
+        np.testing.assert_array_equal(got[ch], ((m / 16 - 128) / 8).astype(np.float32))
+
+
 @pytest.mark.parametrize("H,W", [(72, 136), (1080, 1920), (44, 70)])
 def test_fused_first_layer_equals_downsample_path(gpu, H, W):
     """umi_cnn_first_layer_frames = umi_cnn_downsample_pair + layer kind 0, bit for bit."""

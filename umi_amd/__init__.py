@@ -6,7 +6,7 @@ SyncedMemory.h) lives in ``umi_amd/src``; the C ABI in ``include/umi_hip.h``.
 """
 from ._lib import LIB_PATH, UmiError, lib  # noqa: F401
 from .device import DeviceBuffer, Graph, Stream, Timer, device_count, device_name, synchronize  # noqa: F401
-from . import batch, ops, proposer, workloads  # noqa: F401
+from . import batch, ops, workloads  # noqa: F401  (umi_amd.proposer is imported on use)
 
 __all__ = ["LIB_PATH", "UmiError", "lib", "DeviceBuffer", "Graph", "Stream", "Timer", "device_count",
            "device_name", "synchronize", "batch", "ops", "workloads"]

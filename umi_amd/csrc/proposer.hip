@@ -1210,7 +1210,8 @@ umi_status umi_synth_warp(const uint8_t *f0, uint8_t *f1, int H, int W, int pitc
 
 umi_status umi_cnn_downsample_pair(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, float *out, int n,
                                    umi_stream s) {
-    if (!cur || !ref || !out || H < 4 || W < 4 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_cnn_downsample_pair");
+    if (!cur || !ref || !out || H < 4 || W < 4 || pitch < W || n < 0)
+        return set_error(UMI_ERR_INVALID, "umi_cnn_downsample_pair: bad arguments");
     hipLaunchKernelGGL(downsample_pair_kernel, dim3(ceil_div(W / 4, 256), H / 4), dim3(256), 0, as_stream(s), cur, ref,
                        H, W, pitch, out, n);
     return launch_check("downsample_pair_kernel");
