@@ -29,6 +29,8 @@
 namespace {
 using namespace umi;
 
+typedef __attribute__((ext_vector_type(2))) unsigned short u16x2;
+
 // Largest workgroup per block size: B = 16 holds 64 cur dwords and 16 packed
 // accumulators per thread, which needs the 256-VGPR budget of <= 512 threads.
 template <int B>
@@ -94,6 +96,15 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
         }
         uint32_t bestk = 0xffffffffu;
         const int nrows = 2 * R + B;
+        // 8x8 blocks without the volume: a closed row's four SADs are reduced
+        // with packed 16-bit mins.  SAD <= 64 x 255 < 2^14, so (SAD << 2 | q)
+        // fits 16 bits and orders (SAD, column) as the first minimum; columns
+        // past the window are forced to 0xffff.  The thread keeps
+        // (SAD << 10 | dyi << 2 | q) and forms the raster key once at the end.
+        constexpr bool PACKED = B == 8 && QSAD && !VOLUME;
+        const uint32_t inv_lo = (4 * g + 0 >= S ? 0x0000ffffu : 0u) | (4 * g + 1 >= S ? 0xffff0000u : 0u);
+        const uint32_t inv_hi = (4 * g + 2 >= S ? 0x0000ffffu : 0u) | (4 * g + 3 >= S ? 0xffff0000u : 0u);
+        uint32_t bestp = 0xffffffffu;
 
         // One window row r (= r0 + j): add its contribution to the open
         // vertical offsets dyi = r - py, then close dyi = r - (B-1).
@@ -142,7 +153,16 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
             }
             const int fs = (j + 1) % B;
             const int dyi = r - (B - 1);
-            if ((!FIRST || j == B - 1) && dyi >= 0) {
+            if (PACKED && (!FIRST || j == B - 1) && dyi >= 0) {
+                const uint32_t lo = (uint32_t)acc[fs], hi = (uint32_t)(acc[fs] >> 32);
+                const uint32_t klo = ((lo << 2) & 0xfffcfffcu) | 0x00010000u | inv_lo;  // q = 0, 1
+                const uint32_t khi = ((hi << 2) & 0xfffcfffcu) | 0x00030002u | inv_hi;  // q = 2, 3
+                const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(
+                                                                    __builtin_bit_cast(u16x2, klo),
+                                                                    __builtin_bit_cast(u16x2, khi)));
+                const uint32_t k16 = min(m & 0xffffu, m >> 16);
+                bestp = min(bestp, ((k16 & ~3u) << 8) | ((uint32_t)dyi << 2) | (k16 & 3u));
+            } else if ((!FIRST || j == B - 1) && dyi >= 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int dxi = 4 * g + q;
@@ -168,6 +188,10 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
         }
 #pragma unroll
         for (int j = 0; j < B; ++j) row((nchunks - 1) * B, j, F{}, T{});
+        if constexpr (PACKED) {
+            const uint32_t sad = bestp >> 10, dyi = (bestp >> 2) & 0xffu, q = bestp & 3u;
+            bestk = (sad << idx_bits) | (dyi * (uint32_t)S + 4u * g + q);
+        }
         atomicMin(&best[bl], bestk);
     }
     __syncthreads();

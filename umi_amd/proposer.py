@@ -222,7 +222,8 @@ def learning_rate(lr: float, ep: int, epochs: int, schedule: str) -> float:
 
 
 def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, log=print, stream=None,
-          schedule: str = "step"):
+          schedule: str = "step", on_epoch=None):
+    """Adam over shuffled batches; on_epoch(ep, net) runs after each epoch."""
     tr = Trainer(net, batch, data.H4, data.W4)
     g = np.random.Generator(np.random.PCG64(1234))
     history = []
@@ -239,6 +240,8 @@ def train(net: Network, data: Dataset, epochs: int, batch: int = 8, lr: float = 
             tot += tr.loss()
         history.append(tot / nb)
         log(f"[proposer] epoch {ep}: mean Huber loss {history[-1]:.4f}")
+        if on_epoch is not None:
+            on_epoch(ep, net)
     return history
 
 
@@ -390,10 +393,16 @@ def main(argv=None):
         print(f"[proposer] {a.pairs} pairs synthesised and labelled in {time.time() - t0:.1f} s", flush=True)
         net = Network(seed=1)
         t0 = time.time()
-        hist = train(net, data, a.epochs, a.batch, a.lr, log=lambda m: print(m, flush=True), schedule=a.schedule)
-        print(f"[proposer] trained {a.epochs} epochs in {time.time() - t0:.1f} s", flush=True)
         out = a.out or a.ckpt
         Path(out).parent.mkdir(parents=True, exist_ok=True)
+
+        def checkpoint(ep, n):  # every 10 epochs, so that an interrupted run keeps its progress
+            if (ep + 1) % 10 == 0:
+                n.save(out)
+
+        hist = train(net, data, a.epochs, a.batch, a.lr, log=lambda m: print(m, flush=True), schedule=a.schedule,
+                     on_epoch=checkpoint)
+        print(f"[proposer] trained {a.epochs} epochs in {time.time() - t0:.1f} s", flush=True)
         net.save(out)
         print(json.dumps({"loss_history": hist, "checkpoint": out}), flush=True)
     else:
