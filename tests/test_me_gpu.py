@@ -12,7 +12,9 @@ GOLD = Path(__file__).resolve().parent / "golden"
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[1, 0], ids=["qsad", "sad"])
+# bit 0: v_qsad_pk_u16_u8 (1) or v_sad_u8 (0); bit 1: four-column groups only,
+# without the light lane that takes the last column when S = 4G + 1
+@pytest.fixture(params=[1, 0, 3, 2], ids=["qsad", "sad", "qsad_groups", "sad_groups"])
 def variant(request, gpu):
     call("umi_me_set_variant", request.param)
     yield request.param

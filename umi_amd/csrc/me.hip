@@ -36,13 +36,148 @@ typedef __attribute__((ext_vector_type(2))) unsigned short u16x2;
 template <int B>
 constexpr int me_max_threads() { return B == 8 ? 1024 : 512; }
 
+// Walks the 2R+B window rows of one block for the horizontal offset indices
+// 4g .. 4g+NQ-1 and returns the thread's first minimum as the key
+// (SAD << idx_bits) | (dyi * S + dxi).  NQ = 4: v_qsad_pk_u16_u8 (or four
+// v_sad_u8 over v_alignbyte_b32-shifted words) per cur dword.  NQ = 1: the
+// single column 4g, whose window dwords are dword-aligned, by v_sad_u8.
+// `wbase` points at the block's first window row, dword g.
+template <int B, bool QSAD, bool VOLUME, int NQ>
+__device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t *wbase, int ws4, int R, int g,
+                                   int idx_bits, bool valid, size_t blk, int32_t *__restrict__ volume) {
+    constexpr int B4 = B / 4;
+    constexpr bool Q = QSAD && NQ == 4;
+    constexpr int NW = NQ == 4 ? B4 + 1 : B4;  // window dwords read per row
+    const int S = 2 * R + 1;
+    const int nrows = 2 * R + B;
+    // Q: one packed u64 (4 x u16) per ring slot; otherwise NQ x u32.
+    uint64_t acc[B];
+    uint32_t acc32[B][NQ];
+#pragma unroll
+    for (int s = 0; s < B; ++s) {
+        acc[s] = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc32[s][q] = 0;
+    }
+    uint32_t bestk = 0xffffffffu;
+    // Closings without the volume.  PACKED4 (8x8, four columns): a closed
+    // row's four SADs are reduced with packed 16-bit mins.  SAD <= 64 x 255 <
+    // 2^14, so (SAD << 2 | q) fits 16 bits and orders (SAD, column) as the
+    // first minimum; columns past the window are forced to 0xffff.  The thread
+    // keeps (SAD << 10 | dyi << 2 | q).  PACKED1 (one column): the thread keeps
+    // (SAD << 8 | dyi); dyi <= 2R <= 128.  Both form the raster key once at
+    // the end.
+    constexpr bool PACKED4 = B == 8 && Q && !VOLUME;
+    constexpr bool PACKED1 = NQ == 1 && !VOLUME;
+    const uint32_t inv_lo = (4 * g + 0 >= S ? 0x0000ffffu : 0u) | (4 * g + 1 >= S ? 0xffff0000u : 0u);
+    const uint32_t inv_hi = (4 * g + 2 >= S ? 0x0000ffffu : 0u) | (4 * g + 3 >= S ? 0xffff0000u : 0u);
+    uint32_t bestp = 0xffffffffu;
+
+    // One window row r (= r0 + j): add its contribution to the open vertical
+    // offsets dyi = r - py, then close dyi = r - (B-1).  FIRST (r0 == 0)
+    // skips py > j (dyi < 0); LAST skips dyi > 2R and rows past the window.
+    // Skipped terms only feed discarded slots.
+    auto row = [&](const int r0, const int j, auto first_tag, auto last_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;
+        constexpr bool LAST = decltype(last_tag)::value;
+        const int r = r0 + j;
+        if (LAST && r >= nrows) return;
+        const uint32_t *wr = wbase + r * ws4;
+        uint32_t w[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = wr[k];
+        if constexpr (!Q) {
+            uint32_t u[NQ][B4];
+#pragma unroll
+            for (int k = 0; k < B4; ++k) {
+                u[0][k] = w[k];
+#pragma unroll
+                for (int q = 1; q < NQ; ++q) u[q][k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], q);
+            }
+#pragma unroll
+            for (int py = 0; py < B; ++py) {
+                if (FIRST && py > j) continue;
+                if (LAST && r - py > 2 * R) continue;
+                const int slot = (j - py + B) % B;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                    for (int k = 0; k < B4; ++k)
+                        acc32[slot][q] = __builtin_amdgcn_sad_u8(u[q][k], c[py][k], acc32[slot][q]);
+            }
+        } else {
+#pragma unroll
+            for (int py = 0; py < B; ++py) {
+                if (FIRST && py > j) continue;
+                if (LAST && r - py > 2 * R) continue;
+                const int slot = (j - py + B) % B;
+#pragma unroll
+                for (int k = 0; k < B4; ++k) {
+                    const uint64_t s0 = (uint64_t)w[k] | ((uint64_t)w[k + 1] << 32);
+                    acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(s0, c[py][k], acc[slot]);
+                }
+            }
+        }
+        const int fs = (j + 1) % B;
+        const int dyi = r - (B - 1);
+        const bool closes = (!FIRST || j == B - 1) && dyi >= 0;
+        if (PACKED4 && closes) {
+            const uint32_t lo = (uint32_t)acc[fs], hi = (uint32_t)(acc[fs] >> 32);
+            const uint32_t klo = ((lo << 2) & 0xfffcfffcu) | 0x00010000u | inv_lo;  // q = 0, 1
+            const uint32_t khi = ((hi << 2) & 0xfffcfffcu) | 0x00030002u | inv_hi;  // q = 2, 3
+            const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, klo),
+                                                                                       __builtin_bit_cast(u16x2, khi)));
+            const uint32_t k16 = min(m & 0xffffu, m >> 16);
+            bestp = min(bestp, ((k16 & ~3u) << 8) | ((uint32_t)dyi << 2) | (k16 & 3u));
+        } else if (PACKED1 && closes) {
+            bestp = min(bestp, (acc32[fs][0] << 8) | (uint32_t)dyi);
+        } else if (closes) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int dxi = 4 * g + q;
+                if (dxi < S) {
+                    const uint32_t sad = Q ? (uint32_t)(acc[fs] >> (16 * q)) & 0xffffu : acc32[fs][q];
+                    bestk = min(bestk, (sad << idx_bits) | (uint32_t)(dyi * S + dxi));
+                    if (VOLUME && valid) volume[(blk * S + dyi) * S + dxi] = (int32_t)sad;
+                }
+            }
+        }
+        acc[fs] = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc32[fs][q] = 0;
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    const int nchunks = (nrows + B - 1) / B;  // >= 2 since R >= 1
+#pragma unroll
+    for (int j = 0; j < B; ++j) row(0, j, T{}, F{});
+    for (int ch = 1; ch < nchunks - 1; ++ch) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) row(ch * B, j, F{}, F{});
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) row((nchunks - 1) * B, j, F{}, T{});
+    if constexpr (PACKED4) {
+        const uint32_t sad = bestp >> 10, dyi = (bestp >> 2) & 0xffu, q = bestp & 3u;
+        bestk = (sad << idx_bits) | (dyi * (uint32_t)S + 4u * g + q);
+    } else if constexpr (PACKED1) {
+        const uint32_t sad = bestp >> 8, dyi = bestp & 0xffu;
+        bestk = (sad << idx_bits) | (dyi * (uint32_t)S + 4u * g);
+    }
+    return bestk;
+}
+
 // One TBX x TBY tile of blocks whose first block is (bx0, by0); all threads
-// of the workgroup call it (it contains barriers).
+// of the workgroup call it (it contains barriers).  Threads [0, nblk*G) own
+// (block, 4-column group g).  With GL = 1 (S = 4G + 1) the next nblk threads
+// own (block, the last column 4G): one light lane per block instead of a
+// fifth group with three dead columns, so that a 64-block tile is 8 waves of
+// four-column work plus one light wave.
 template <int B, bool QSAD, bool VOLUME>
 __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H,
-                               int W, int pitch, int R, int nby, int nbx, int tbx, int tby, int G, int ws4, int wrows,
-                               int idx_bits, int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume, int bx0,
-                               int by0) {
+                               int W, int pitch, int R, int nby, int nbx, int tbx, int tby, int G, int GL, int ws4,
+                               int wrows, int idx_bits, int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume,
+                               int bx0, int by0) {
     constexpr int B4 = B / 4;
     const int nblk = tbx * tby;
     const int cs4 = tbx * B4;             // cur tile row stride (dwords)
@@ -70,13 +205,15 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
     for (int i = tid; i < nblk; i += nthr) best[i] = 0xffffffffu;
     __syncthreads();
 
-    const int bl = tid / G, g = tid - bl * G;
-    if (bl < nblk) {
+    const int nmain = nblk * G;
+    const bool light = tid >= nmain;
+    const int bl = light ? tid - nmain : tid / G;
+    const int g = light ? G : tid - bl * G;
+    if (bl < nblk && (!light || GL)) {
         const int bxl = bl % tbx, byl = bl / tbx;
         const int bx = bx0 + bxl, by = by0 + byl;
         const bool valid = bx < nbx && by < nby;
         const size_t blk = (size_t)by * nbx + bx;
-        const int S = 2 * R + 1;
 
         uint32_t c[B][B4];
 #pragma unroll
@@ -85,113 +222,9 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
             for (int k = 0; k < B4; ++k) c[py][k] = ctile[(byl * B + py) * cs4 + bxl * B4 + k];
 
         const uint32_t *wbase = win + (byl * B) * ws4 + bxl * B4 + g;
-        // QSAD: one packed u64 (4 x u16) per ring slot; otherwise 4 x u32.
-        uint64_t acc[B];
-        uint32_t acc32[B][4];
-#pragma unroll
-        for (int s = 0; s < B; ++s) {
-            acc[s] = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc32[s][q] = 0;
-        }
-        uint32_t bestk = 0xffffffffu;
-        const int nrows = 2 * R + B;
-        // 8x8 blocks without the volume: a closed row's four SADs are reduced
-        // with packed 16-bit mins.  SAD <= 64 x 255 < 2^14, so (SAD << 2 | q)
-        // fits 16 bits and orders (SAD, column) as the first minimum; columns
-        // past the window are forced to 0xffff.  The thread keeps
-        // (SAD << 10 | dyi << 2 | q) and forms the raster key once at the end.
-        constexpr bool PACKED = B == 8 && QSAD && !VOLUME;
-        const uint32_t inv_lo = (4 * g + 0 >= S ? 0x0000ffffu : 0u) | (4 * g + 1 >= S ? 0xffff0000u : 0u);
-        const uint32_t inv_hi = (4 * g + 2 >= S ? 0x0000ffffu : 0u) | (4 * g + 3 >= S ? 0xffff0000u : 0u);
-        uint32_t bestp = 0xffffffffu;
-
-        // One window row r (= r0 + j): add its contribution to the open
-        // vertical offsets dyi = r - py, then close dyi = r - (B-1).
-        // FIRST (r0 == 0) skips py > j (dyi < 0); LAST skips dyi > 2R and
-        // rows past the window.  Skipped terms only feed discarded slots.
-        auto row = [&](const int r0, const int j, auto first_tag, auto last_tag) {
-            constexpr bool FIRST = decltype(first_tag)::value;
-            constexpr bool LAST = decltype(last_tag)::value;
-            const int r = r0 + j;
-            if (LAST && r >= nrows) return;
-            const uint32_t *wr = wbase + r * ws4;
-            uint32_t w[B4 + 1];
-#pragma unroll
-            for (int k = 0; k <= B4; ++k) w[k] = wr[k];
-            if constexpr (!QSAD) {
-                uint32_t u[4][B4];
-#pragma unroll
-                for (int k = 0; k < B4; ++k) {
-                    u[0][k] = w[k];
-#pragma unroll
-                    for (int q = 1; q < 4; ++q) u[q][k] = __builtin_amdgcn_alignbyte(w[k + 1], w[k], q);
-                }
-#pragma unroll
-                for (int py = 0; py < B; ++py) {
-                    if (FIRST && py > j) continue;
-                    if (LAST && r - py > 2 * R) continue;
-                    const int slot = (j - py + B) % B;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int k = 0; k < B4; ++k)
-                            acc32[slot][q] = __builtin_amdgcn_sad_u8(u[q][k], c[py][k], acc32[slot][q]);
-                }
-            } else {
-#pragma unroll
-                for (int py = 0; py < B; ++py) {
-                    if (FIRST && py > j) continue;
-                    if (LAST && r - py > 2 * R) continue;
-                    const int slot = (j - py + B) % B;
-#pragma unroll
-                    for (int k = 0; k < B4; ++k) {
-                        const uint64_t s0 = (uint64_t)w[k] | ((uint64_t)w[k + 1] << 32);
-                        acc[slot] = __builtin_amdgcn_qsad_pk_u16_u8(s0, c[py][k], acc[slot]);
-                    }
-                }
-            }
-            const int fs = (j + 1) % B;
-            const int dyi = r - (B - 1);
-            if (PACKED && (!FIRST || j == B - 1) && dyi >= 0) {
-                const uint32_t lo = (uint32_t)acc[fs], hi = (uint32_t)(acc[fs] >> 32);
-                const uint32_t klo = ((lo << 2) & 0xfffcfffcu) | 0x00010000u | inv_lo;  // q = 0, 1
-                const uint32_t khi = ((hi << 2) & 0xfffcfffcu) | 0x00030002u | inv_hi;  // q = 2, 3
-                const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(
-                                                                    __builtin_bit_cast(u16x2, klo),
-                                                                    __builtin_bit_cast(u16x2, khi)));
-                const uint32_t k16 = min(m & 0xffffu, m >> 16);
-                bestp = min(bestp, ((k16 & ~3u) << 8) | ((uint32_t)dyi << 2) | (k16 & 3u));
-            } else if ((!FIRST || j == B - 1) && dyi >= 0) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int dxi = 4 * g + q;
-                    if (dxi < S) {
-                        const uint32_t sad = QSAD ? (uint32_t)(acc[fs] >> (16 * q)) & 0xffffu : acc32[fs][q];
-                        bestk = min(bestk, (sad << idx_bits) | (uint32_t)(dyi * S + dxi));
-                        if (VOLUME && valid) volume[(blk * S + dyi) * S + dxi] = (int32_t)sad;
-                    }
-                }
-            }
-            acc[fs] = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc32[fs][q] = 0;
-        };
-        using T = std::true_type;
-        using F = std::false_type;
-        const int nchunks = (nrows + B - 1) / B;  // >= 2 since R >= 1
-#pragma unroll
-        for (int j = 0; j < B; ++j) row(0, j, T{}, F{});
-        for (int ch = 1; ch < nchunks - 1; ++ch) {
-#pragma unroll
-            for (int j = 0; j < B; ++j) row(ch * B, j, F{}, F{});
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) row((nchunks - 1) * B, j, F{}, T{});
-        if constexpr (PACKED) {
-            const uint32_t sad = bestp >> 10, dyi = (bestp >> 2) & 0xffu, q = bestp & 3u;
-            bestk = (sad << idx_bits) | (dyi * (uint32_t)S + 4u * g + q);
-        }
+        const uint32_t bestk =
+            light ? me_walk<B, QSAD, VOLUME, 1>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume)
+                  : me_walk<B, QSAD, VOLUME, 4>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume);
         atomicMin(&best[bl], bestk);
     }
     __syncthreads();
@@ -214,10 +247,10 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
 template <int B, bool QSAD, bool VOLUME>
 __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
     const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R, int nby, int nbx,
-    int tbx, int tby, int G, int ws4, int wrows, int idx_bits, int32_t *__restrict__ mv_sad,
+    int tbx, int tby, int G, int GL, int ws4, int wrows, int idx_bits, int32_t *__restrict__ mv_sad,
     int32_t *__restrict__ volume) {
     extern __shared__ uint32_t lds[];
-    me_tile<B, QSAD, VOLUME>(lds, cur, ref, H, W, pitch, R, nby, nbx, tbx, tby, G, ws4, wrows, idx_bits, mv_sad,
+    me_tile<B, QSAD, VOLUME>(lds, cur, ref, H, W, pitch, R, nby, nbx, tbx, tby, G, GL, ws4, wrows, idx_bits, mv_sad,
                              volume, blockIdx.x * tbx, blockIdx.y * tby);
 }
 
@@ -297,23 +330,27 @@ __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restric
 }
 
 struct MePlan {
-    int G, tbx, tby, threads, ws4, wrows, idx_bits;
+    int G, GL, tbx, tby, threads, ws4, wrows, idx_bits;
     size_t lds;
 };
 
-MePlan me_plan(int B, int R) {
+// S = 2R + 1 columns: S = 4G + 1 takes G four-column groups and one light
+// lane per block (unless `groups_only`); S = 4G - 1 takes G groups, the last
+// with one dead column.
+MePlan me_plan(int B, int R, bool groups_only) {
     MePlan p;
-    p.G = umi::ceil_div(2 * R + 1, 4);
-    const int max_threads = B == 8 ? me_max_threads<8>() : me_max_threads<16>();
     const int S = 2 * R + 1;
+    p.GL = (S % 4 == 1 && !groups_only) ? 1 : 0;
+    p.G = p.GL ? S / 4 : umi::ceil_div(S, 4);
+    const int max_threads = B == 8 ? me_max_threads<8>() : me_max_threads<16>();
     p.idx_bits = umi::bits_for((uint32_t)(S * S - 1));
     int nblk = 1;
-    while (nblk * 2 * p.G <= max_threads && nblk * 2 <= 256) nblk *= 2;
+    while (nblk * 2 * (p.G + p.GL) <= max_threads && nblk * 2 <= 256) nblk *= 2;
     // shrink the tile until window + current tile + keys fit 64 KiB of LDS
     for (;; nblk /= 2) {
         p.tbx = nblk < 16 ? nblk : 16;
         p.tby = nblk / p.tbx;
-        p.threads = umi::round_up(nblk * p.G, 64);
+        p.threads = umi::round_up(nblk * (p.G + p.GL), 64);
         p.ws4 = umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
         p.wrows = p.tby * B + 2 * R;
         p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
@@ -322,22 +359,24 @@ MePlan me_plan(int B, int R) {
     return p;
 }
 
-int g_me_variant = 1;  // 1 = v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32
+// bit 0: 1 = v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32;
+// bit 1: four-column groups only (no light lanes)
+int g_me_variant = 1;
 
 template <int B, bool QSAD>
 umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R,
                      int32_t *mv_sad, int32_t *volume, hipStream_t st) {
-    const MePlan p = me_plan(B, R);
+    const MePlan p = me_plan(B, R, (g_me_variant & 2) != 0);
     const int nby = H / B, nbx = W / B;
     dim3 grid(umi::ceil_div(nbx, p.tbx), umi::ceil_div(nby, p.tby));
     if (volume)
         hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, true>), grid, dim3(p.threads), p.lds, st, cur, ref,
-                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.ws4, p.wrows, p.idx_bits, mv_sad,
-                           volume);
+                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.GL, p.ws4, p.wrows, p.idx_bits,
+                           mv_sad, volume);
     else
         hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false>), grid, dim3(p.threads), p.lds, st, cur, ref,
-                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.ws4, p.wrows, p.idx_bits, mv_sad,
-                           volume);
+                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.GL, p.ws4, p.wrows, p.idx_bits,
+                           mv_sad, volume);
     return umi::launch_check("me_full_search_kernel");
 }
 
@@ -345,11 +384,12 @@ umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int p
 
 extern "C" {
 
-// Selects the SAD instruction form for 8x8 blocks (1 = v_qsad_pk_u16_u8,
-// 0 = v_sad_u8 + v_alignbyte_b32).  Both give identical results; exported
-// for measurement.
+// Selects the full search's SAD instruction form (bit 0: 1 =
+// v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32) and, with bit 1, the
+// layout without light lanes.  All give identical results; exported for
+// measurement and tests.
 umi_status umi_me_set_variant(int variant) {
-    if (variant != 0 && variant != 1) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
+    if (variant < 0 || variant > 3) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
     g_me_variant = variant;
     return UMI_OK;
 }
@@ -364,13 +404,13 @@ umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, 
         return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: H=%d W=%d pitch=%d", H, W, pitch);
     if (H / B == 0 || W / B == 0) return UMI_OK;  // no whole block
     if (!mv_sad && !volume) return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: no output");
-    const MePlan p = me_plan(B, R);
+    const MePlan p = me_plan(B, R, (g_me_variant & 2) != 0);
     if (p.lds > 64 * 1024) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: LDS %zu B", p.lds);
     const int sad_bits = umi::bits_for(255u * B * B);
     if (sad_bits + p.idx_bits > 32) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: key overflow");
     hipStream_t st = umi::as_stream(s);
     if (B == 8)
-        return g_me_variant ? me_launch<8, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
+        return (g_me_variant & 1) ? me_launch<8, true>(cur, ref, H, W, pitch, R, mv_sad, volume, st)
                             : me_launch<8, false>(cur, ref, H, W, pitch, R, mv_sad, volume, st);
     // B = 16 always uses v_qsad_pk_u16_u8: the v_sad_u8 form needs 64 more
     // accumulator registers than fit without spilling.
