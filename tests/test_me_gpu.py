@@ -13,8 +13,11 @@ pytestmark = pytest.mark.gpu
 
 
 # bit 0: v_qsad_pk_u16_u8 (1) or v_sad_u8 (0); bit 1: four-column groups only,
-# without the light lane that takes the last column when S = 4G + 1
-@pytest.fixture(params=[1, 0, 3, 2], ids=["qsad", "sad", "qsad_groups", "sad_groups"])
+# without the light lane that takes the last column when S = 4G + 1; bit 2:
+# dword staging even where frames allow 16-byte staging; bit 3: runtime skips
+# in the last row chunk even where 8x8 blocks with R % 4 == 0 make them static
+@pytest.fixture(params=[1, 0, 3, 2, 5, 9], ids=["qsad", "sad", "qsad_groups", "sad_groups", "qsad_dword",
+                                                "qsad_runtime"])
 def variant(request, gpu):
     call("umi_me_set_variant", request.param)
     yield request.param
@@ -31,7 +34,7 @@ def test_me_golden(variant, name):
 
 
 @pytest.mark.parametrize("H,W,B,R", [(64, 96, 8, 16), (72, 80, 16, 7), (100, 130, 8, 32), (37, 53, 8, 1),
-                                     (96, 160, 16, 16), (80, 80, 8, 64)])
+                                     (96, 160, 16, 16), (80, 80, 8, 64), (72, 112, 8, 8)])
 def test_me_random_vs_oracle(variant, H, W, B, R):
     g = workloads.rng(H * 1000 + W + R)
     cur = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
@@ -44,7 +47,8 @@ def test_me_random_vs_oracle(variant, H, W, B, R):
 
 
 @pytest.mark.parametrize("H,W,B,R", [(64, 96, 8, 16), (100, 130, 8, 32), (37, 53, 8, 1), (80, 80, 8, 64),
-                                     (72, 88, 8, 7), (48, 64, 8, 2), (72, 80, 16, 7)])
+                                     (72, 88, 8, 7), (48, 64, 8, 2), (72, 80, 16, 7), (72, 112, 8, 8),
+                                     (56, 144, 8, 12), (64, 208, 8, 4), (48, 176, 16, 12)])
 def test_me_mv_only_vs_oracle(variant, H, W, B, R):
     """Motion vectors without the volume (the kernel form the bench and the
     proposer's judge run), on frames with many equal SADs."""

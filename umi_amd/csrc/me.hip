@@ -41,8 +41,13 @@ constexpr int me_max_threads() { return B == 8 ? 1024 : 512; }
 // (SAD << idx_bits) | (dyi * S + dxi).  NQ = 4: v_qsad_pk_u16_u8 (or four
 // v_sad_u8 over v_alignbyte_b32-shifted words) per cur dword.  NQ = 1: the
 // single column 4g, whose window dwords are dword-aligned, by v_sad_u8.
-// `wbase` points at the block's first window row, dword g.
-template <int B, bool QSAD, bool VOLUME, int NQ>
+// `wbase` points at the block's first window row, dword g; rows are ws4
+// dwords apart.  LFULL says that the 2R + B rows are whole chunks of B rows
+// (2R a multiple of B), which makes every skip in the last chunk static.
+// (The stride stays a runtime value: with a compile-time stride the compiler
+// pairs dwords of consecutive rows in one ds_read2_b32, whose two halves then
+// hit overlapping LDS banks.)
+template <int B, bool QSAD, bool VOLUME, int NQ, bool LFULL>
 __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t *wbase, int ws4, int R, int g,
                                    int idx_bits, bool valid, size_t blk, int32_t *__restrict__ volume) {
     constexpr int B4 = B / 4;
@@ -65,13 +70,15 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
     // 2^14, so (SAD << 2 | q) fits 16 bits and orders (SAD, column) as the
     // first minimum; columns past the window are forced to 0xffff.  The thread
     // keeps (SAD << 10 | dyi << 2 | q).  PACKED1 (one column): the thread keeps
-    // (SAD << 8 | dyi); dyi <= 2R <= 128.  Both form the raster key once at
-    // the end.
+    // (SAD << 8 | dyi); dyi <= 2R <= 128.  Inside a chunk of B rows the keys
+    // carry the row's index j in the chunk (a constant after unrolling) in
+    // place of dyi, and the chunk's base r0 - (B-1) is added once per chunk.
+    // Both form the raster key once at the end.
     constexpr bool PACKED4 = B == 8 && Q && !VOLUME;
     constexpr bool PACKED1 = NQ == 1 && !VOLUME;
     const uint32_t inv_lo = (4 * g + 0 >= S ? 0x0000ffffu : 0u) | (4 * g + 1 >= S ? 0xffff0000u : 0u);
     const uint32_t inv_hi = (4 * g + 2 >= S ? 0x0000ffffu : 0u) | (4 * g + 3 >= S ? 0xffff0000u : 0u);
-    uint32_t bestp = 0xffffffffu;
+    uint32_t bestp = 0xffffffffu, bestc = 0xffffffffu;
 
     // One window row r (= r0 + j): add its contribution to the open vertical
     // offsets dyi = r - py, then close dyi = r - (B-1).  FIRST (r0 == 0)
@@ -81,7 +88,7 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
         constexpr bool FIRST = decltype(first_tag)::value;
         constexpr bool LAST = decltype(last_tag)::value;
         const int r = r0 + j;
-        if (LAST && r >= nrows) return;
+        if (LAST && !LFULL && r >= nrows) return;
         const uint32_t *wr = wbase + r * ws4;
         uint32_t w[NW];
 #pragma unroll
@@ -97,7 +104,7 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
 #pragma unroll
             for (int py = 0; py < B; ++py) {
                 if (FIRST && py > j) continue;
-                if (LAST && r - py > 2 * R) continue;
+                if (LAST && (LFULL ? py < j : r - py > 2 * R)) continue;
                 const int slot = (j - py + B) % B;
 #pragma unroll
                 for (int q = 0; q < NQ; ++q)
@@ -109,7 +116,7 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
 #pragma unroll
             for (int py = 0; py < B; ++py) {
                 if (FIRST && py > j) continue;
-                if (LAST && r - py > 2 * R) continue;
+                if (LAST && (LFULL ? py < j : r - py > 2 * R)) continue;
                 const int slot = (j - py + B) % B;
 #pragma unroll
                 for (int k = 0; k < B4; ++k) {
@@ -122,15 +129,20 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
         const int dyi = r - (B - 1);
         const bool closes = (!FIRST || j == B - 1) && dyi >= 0;
         if (PACKED4 && closes) {
-            const uint32_t lo = (uint32_t)acc[fs], hi = (uint32_t)(acc[fs] >> 32);
-            const uint32_t klo = ((lo << 2) & 0xfffcfffcu) | 0x00010000u | inv_lo;  // q = 0, 1
-            const uint32_t khi = ((hi << 2) & 0xfffcfffcu) | 0x00030002u | inv_hi;  // q = 2, 3
+            const uint2 a2 = __builtin_bit_cast(uint2, acc[fs]);
+            uint32_t lo = a2.x, hi = a2.y;
+            // keeps the shift below a 32-bit v_lshl_or_b32 (not a 64-bit funnel shift plus mask)
+            asm("" : "+v"(hi));
+            // both u16 halves are < 2^14, so one 32-bit shift moves them without carry
+            const uint32_t klo = (lo << 2) | (0x00010000u | inv_lo);  // q = 0, 1
+            const uint32_t khi = (hi << 2) | (0x00030002u | inv_hi);  // q = 2, 3
             const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, klo),
                                                                                        __builtin_bit_cast(u16x2, khi)));
             const uint32_t k16 = min(m & 0xffffu, m >> 16);
-            bestp = min(bestp, ((k16 & ~3u) << 8) | ((uint32_t)dyi << 2) | (k16 & 3u));
+            // (SAD << 5 | j << 2 | q): SAD, then the row, then the column
+            bestc = min(bestc, ((k16 << 3) & ~0x1fu) | ((uint32_t)j << 2) | (k16 & 3u));
         } else if (PACKED1 && closes) {
-            bestp = min(bestp, (acc32[fs][0] << 8) | (uint32_t)dyi);
+            bestc = min(bestc, (acc32[fs][0] << 8) | (uint32_t)j);
         } else if (closes) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
@@ -148,15 +160,21 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
     };
     using T = std::true_type;
     using F = std::false_type;
+    // Every chunk closes at least one offset (the first at j = B-1, the last
+    // at its first row), so bestc is a real key when it is folded.
+    auto chunk = [&](const int r0, auto first_tag, auto last_tag) {
+        bestc = 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < B; ++j) row(r0, j, first_tag, last_tag);
+        // chunk key (SAD << 5 | j << 2 | q) -> (SAD << 10 | dyi << 2 | q), dyi = r0 - (B-1) + j
+        if constexpr (PACKED4)
+            bestp = min(bestp, ((bestc & ~0x1fu) << 5) + (bestc & 0x1fu) + ((uint32_t)(r0 - (B - 1)) << 2));
+        if constexpr (PACKED1) bestp = min(bestp, bestc + (uint32_t)(r0 - (B - 1)));
+    };
     const int nchunks = (nrows + B - 1) / B;  // >= 2 since R >= 1
-#pragma unroll
-    for (int j = 0; j < B; ++j) row(0, j, T{}, F{});
-    for (int ch = 1; ch < nchunks - 1; ++ch) {
-#pragma unroll
-        for (int j = 0; j < B; ++j) row(ch * B, j, F{}, F{});
-    }
-#pragma unroll
-    for (int j = 0; j < B; ++j) row((nchunks - 1) * B, j, F{}, T{});
+    chunk(0, T{}, F{});
+    for (int ch = 1; ch < nchunks - 1; ++ch) chunk(ch * B, F{}, F{});
+    chunk((nchunks - 1) * B, F{}, T{});
     if constexpr (PACKED4) {
         const uint32_t sad = bestp >> 10, dyi = (bestp >> 2) & 0xffu, q = bestp & 3u;
         bestk = (sad << idx_bits) | (dyi * (uint32_t)S + 4u * g + q);
@@ -173,11 +191,11 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
 // own (block, the last column 4G): one light lane per block instead of a
 // fifth group with three dead columns, so that a 64-block tile is 8 waves of
 // four-column work plus one light wave.
-template <int B, bool QSAD, bool VOLUME>
+template <int B, bool QSAD, bool VOLUME, bool LFULL>
 __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H,
                                int W, int pitch, int R, int nby, int nbx, int tbx, int tby, int G, int GL, int ws4,
-                               int wrows, int idx_bits, int32_t *__restrict__ mv_sad, int32_t *__restrict__ volume,
-                               int bx0, int by0) {
+                               int wrows, int fast16, int idx_bits, int32_t *__restrict__ mv_sad,
+                               int32_t *__restrict__ volume, int bx0, int by0) {
     constexpr int B4 = B / 4;
     const int nblk = tbx * tby;
     const int cs4 = tbx * B4;             // cur tile row stride (dwords)
@@ -189,19 +207,30 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
     const int nthr = blockDim.x;
     const int x0 = bx0 * B, y0 = by0 * B;
 
-    // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(x0-R+c)].
-    stage_clamped<8>(win, wrows * ws4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
-        const int r = i / ws4;
-        row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
-        gx = x0 - R + 4 * (i - r * ws4);
-    });
-    // Current tile.  Blocks past the frame edge read clamped pixels and are
-    // never written out.
-    stage_clamped<4>(ctile, tby * B * cs4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
-        const int r = i / cs4;
-        row = cur + (size_t)min(y0 + r, H - 1) * pitch;
-        gx = x0 + 4 * (i - r * cs4);
-    });
+    // Reference window, clamp-to-edge: window (r, c) <- ref[cl(y0-R+r)][cl(xs+c)]
+    // and current tile (blocks past the frame edge read clamped pixels and are
+    // never written out).  fast16 (16-byte aligned frames and rows, R % 4 == 0,
+    // tile width a multiple of 16): the window starts at xs = floor16(x0 - R)
+    // and both are staged in 16-byte chunks; otherwise xs = x0 - R, by dwords.
+    const int xr = x0 - R, xs = fast16 ? (xr & ~15) : xr;
+    const auto ref_row = [&](int r) { return ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch; };
+    const auto cur_row = [&](int r) { return cur + (size_t)min(y0 + r, H - 1) * pitch; };
+    if (fast16) {
+        const int need16 = ((xr - xs) + tbx * B + 2 * R + 4 + 15) >> 4;  // <= ws4 / 4
+        stage_rows16<2>(win, wrows, need16, ws4 / 4, xs, W, tid, nthr, ref_row);
+        stage_rows16<1>(ctile, tby * B, cs4 / 4, cs4 / 4, x0, W, tid, nthr, cur_row);
+    } else {
+        stage_clamped<8>(win, wrows * ws4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
+            const int r = i / ws4;
+            row = ref_row(r);
+            gx = xs + 4 * (i - r * ws4);
+        });
+        stage_clamped<4>(ctile, tby * B * cs4, tid, nthr, W, [&](int i, const uint8_t *&row, int &gx) {
+            const int r = i / cs4;
+            row = cur_row(r);
+            gx = x0 + 4 * (i - r * cs4);
+        });
+    }
     for (int i = tid; i < nblk; i += nthr) best[i] = 0xffffffffu;
     __syncthreads();
 
@@ -221,10 +250,10 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
 #pragma unroll
             for (int k = 0; k < B4; ++k) c[py][k] = ctile[(byl * B + py) * cs4 + bxl * B4 + k];
 
-        const uint32_t *wbase = win + (byl * B) * ws4 + bxl * B4 + g;
+        const uint32_t *wbase = win + (byl * B) * ws4 + ((xr - xs) >> 2) + bxl * B4 + g;
         const uint32_t bestk =
-            light ? me_walk<B, QSAD, VOLUME, 1>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume)
-                  : me_walk<B, QSAD, VOLUME, 4>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume);
+            light ? me_walk<B, QSAD, VOLUME, 1, LFULL>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume)
+                  : me_walk<B, QSAD, VOLUME, 4, LFULL>(c, wbase, ws4, R, g, idx_bits, valid, blk, volume);
         atomicMin(&best[bl], bestk);
     }
     __syncthreads();
@@ -244,14 +273,14 @@ __device__ inline void me_tile(uint32_t *lds, const uint8_t *__restrict__ cur, c
     }
 }
 
-template <int B, bool QSAD, bool VOLUME>
+template <int B, bool QSAD, bool VOLUME, bool LFULL>
 __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
     const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch, int R, int nby, int nbx,
-    int tbx, int tby, int G, int GL, int ws4, int wrows, int idx_bits, int32_t *__restrict__ mv_sad,
+    int tbx, int tby, int G, int GL, int ws4, int wrows, int fast16, int idx_bits, int32_t *__restrict__ mv_sad,
     int32_t *__restrict__ volume) {
-    extern __shared__ uint32_t lds[];
-    me_tile<B, QSAD, VOLUME>(lds, cur, ref, H, W, pitch, R, nby, nbx, tbx, tby, G, GL, ws4, wrows, idx_bits, mv_sad,
-                             volume, blockIdx.x * tbx, blockIdx.y * tby);
+    extern __shared__ __align__(16) uint32_t lds[];
+    me_tile<B, QSAD, VOLUME, LFULL>(lds, cur, ref, H, W, pitch, R, nby, nbx, tbx, tby, G, GL, ws4, wrows, fast16, idx_bits,
+                             mv_sad, volume, blockIdx.x * tbx, blockIdx.y * tby);
 }
 
 // Listed blocks: a persistent grid walks list[0 .. *count) (block indices
@@ -330,14 +359,19 @@ __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restric
 }
 
 struct MePlan {
-    int G, GL, tbx, tby, threads, ws4, wrows, idx_bits;
+    int G, GL, tbx, tby, threads, ws4, wrows, fast16, stat, idx_bits;
     size_t lds;
 };
 
 // S = 2R + 1 columns: S = 4G + 1 takes G four-column groups and one light
 // lane per block (unless `groups_only`); S = 4G - 1 takes G groups, the last
-// with one dead column.
-MePlan me_plan(int B, int R, bool groups_only) {
+// with one dead column.  `aligned16`: both frames and the pitch are 16-byte
+// aligned and W >= 16, which with R % 4 == 0 allows 16-byte staging; the
+// window row then starts at floor16(x0 - R) and spans whole 16-byte chunks.
+//
+// `stat`: 8x8 blocks, R % 4 == 0 (the 2R + 8 window rows are whole chunks of
+// 8) and no SAD volume: the kernel is compiled with a static last chunk.
+MePlan me_plan(int B, int R, bool groups_only, bool aligned16, bool volume, bool allow_static) {
     MePlan p;
     const int S = 2 * R + 1;
     p.GL = (S % 4 == 1 && !groups_only) ? 1 : 0;
@@ -351,7 +385,11 @@ MePlan me_plan(int B, int R, bool groups_only) {
         p.tbx = nblk < 16 ? nblk : 16;
         p.tby = nblk / p.tbx;
         p.threads = umi::round_up(nblk * (p.G + p.GL), 64);
-        p.ws4 = umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
+        p.fast16 = aligned16 && R % 4 == 0 && (p.tbx * B) % 16 == 0;
+        // x0 is a multiple of 16, so x0 - R sits (-R) mod 16 bytes past xs
+        p.ws4 = p.fast16 ? 4 * umi::ceil_div((16 - R % 16) % 16 + p.tbx * B + 2 * R + 4, 16)
+                         : umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
+        p.stat = allow_static && B == 8 && R % 4 == 0 && !volume;
         p.wrows = p.tby * B + 2 * R;
         p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
         if (p.lds <= 64 * 1024 || nblk == 1) break;
@@ -360,23 +398,36 @@ MePlan me_plan(int B, int R, bool groups_only) {
 }
 
 // bit 0: 1 = v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32;
-// bit 1: four-column groups only (no light lanes)
+// bit 1: four-column groups only (no light lanes); bit 2: dword staging only;
+// bit 3: no static last chunk
 int g_me_variant = 1;
+
+MePlan me_plan_for(int B, int R, const uint8_t *cur, const uint8_t *ref, int W, int pitch, bool volume) {
+    const bool aligned16 =
+        ((reinterpret_cast<uintptr_t>(cur) | reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 15) == 0 &&
+        W >= 16 && (g_me_variant & 4) == 0;
+    return me_plan(B, R, (g_me_variant & 2) != 0, aligned16, volume, (g_me_variant & 8) == 0);
+}
 
 template <int B, bool QSAD>
 umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R,
                      int32_t *mv_sad, int32_t *volume, hipStream_t st) {
-    const MePlan p = me_plan(B, R, (g_me_variant & 2) != 0);
+    const MePlan p = me_plan_for(B, R, cur, ref, W, pitch, volume != nullptr);
     const int nby = H / B, nbx = W / B;
     dim3 grid(umi::ceil_div(nbx, p.tbx), umi::ceil_div(nby, p.tby));
+#define UMI_ME_ARGS                                                                                                  \
+    grid, dim3(p.threads), p.lds, st, cur, ref, H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.GL, p.ws4, p.wrows,    \
+        p.fast16, p.idx_bits, mv_sad, volume
     if (volume)
-        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, true>), grid, dim3(p.threads), p.lds, st, cur, ref,
-                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.GL, p.ws4, p.wrows, p.idx_bits,
-                           mv_sad, volume);
-    else
-        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false>), grid, dim3(p.threads), p.lds, st, cur, ref,
-                           H, W, pitch, R, nby, nbx, p.tbx, p.tby, p.G, p.GL, p.ws4, p.wrows, p.idx_bits,
-                           mv_sad, volume);
+        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, true, false>), UMI_ME_ARGS);
+    else if constexpr (B == 8) {
+        if (p.stat)
+            hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false, true>), UMI_ME_ARGS);
+        else
+            hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false, false>), UMI_ME_ARGS);
+    } else
+        hipLaunchKernelGGL((me_full_search_kernel<B, QSAD, false, false>), UMI_ME_ARGS);
+#undef UMI_ME_ARGS
     return umi::launch_check("me_full_search_kernel");
 }
 
@@ -385,11 +436,12 @@ umi_status me_launch(const uint8_t *cur, const uint8_t *ref, int H, int W, int p
 extern "C" {
 
 // Selects the full search's SAD instruction form (bit 0: 1 =
-// v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32) and, with bit 1, the
-// layout without light lanes.  All give identical results; exported for
+// v_qsad_pk_u16_u8, 0 = v_sad_u8 + v_alignbyte_b32), with bit 1 the layout
+// without light lanes, with bit 2 dword staging of aligned frames and with
+// bit 3 no static last chunk.  All give identical results; exported for
 // measurement and tests.
 umi_status umi_me_set_variant(int variant) {
-    if (variant < 0 || variant > 3) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
+    if (variant < 0 || variant > 15) return umi::set_error(UMI_ERR_INVALID, "umi_me_set_variant: %d", variant);
     g_me_variant = variant;
     return UMI_OK;
 }
@@ -404,7 +456,7 @@ umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, 
         return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: H=%d W=%d pitch=%d", H, W, pitch);
     if (H / B == 0 || W / B == 0) return UMI_OK;  // no whole block
     if (!mv_sad && !volume) return set_error(UMI_ERR_INVALID, "umi_me_full_search_u8: no output");
-    const MePlan p = me_plan(B, R, (g_me_variant & 2) != 0);
+    const MePlan p = me_plan_for(B, R, cur, ref, W, pitch, volume != nullptr);
     if (p.lds > 64 * 1024) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: LDS %zu B", p.lds);
     const int sad_bits = umi::bits_for(255u * B * B);
     if (sad_bits + p.idx_bits > 32) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_u8: key overflow");

@@ -50,4 +50,45 @@ __device__ inline void stage_clamped(uint32_t *dst, int n, int tid, int nthr, in
     }
 }
 
+// Sixteen clamped bytes of `row` from column gx, as four dwords.
+__device__ inline uint4 load16_clamped(const uint8_t *row, int gx, int W) {
+    return make_uint4(load4_clamped(row, gx, W), load4_clamped(row, gx + 4, W), load4_clamped(row, gx + 8, W),
+                      load4_clamped(row, gx + 12, W));
+}
+
+// dst holds `rows` rows of s16 16-byte chunks at a stride of d16 >= s16
+// chunks; chunk (r, c) = the clamped bytes rowp(r)[xs + 16 c .. + 16).
+// rowp(r) must be 16-byte aligned with
+// W >= 16 bytes, xs a multiple of 16 and dst 16-byte aligned.  A chunk
+// inside [0, W) is one 16-byte load; the others take load16_clamped.  As in
+// stage_clamped, all U loads are issued before the first store: indices past
+// the end repeat the last chunk (same bytes, same slot) and non-fast lanes
+// first load the row's first 16 bytes.
+template <int U, class RowPtr>
+__device__ inline void stage_rows16(uint32_t *dst, int rows, int s16, int d16, int xs, int W, int tid, int nthr,
+                                    RowPtr rowp) {
+    const int n = rows * s16;
+    for (int base = 0; base < n; base += U * nthr) {
+        uint4 v[U];
+        int idx[U], slot[U], gx[U];
+        const uint8_t *row[U];
+        bool fast[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            idx[u] = min(base + u * nthr + tid, n - 1);
+            const int r = idx[u] / s16, c = idx[u] - r * s16;
+            row[u] = rowp(r);
+            gx[u] = xs + 16 * c;
+            slot[u] = r * d16 + c;
+            fast[u] = gx[u] >= 0 && gx[u] + 16 <= W;
+            v[u] = *reinterpret_cast<const uint4 *>(row[u] + (fast[u] ? gx[u] : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (!fast[u]) v[u] = load16_clamped(row[u], gx[u], W);
+#pragma unroll
+        for (int u = 0; u < U; ++u) reinterpret_cast<uint4 *>(dst)[slot[u]] = v[u];
+    }
+}
+
 }  // namespace umi
