@@ -172,10 +172,13 @@ umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int ra
  * Integral image out[y][x] = sum_{y'<=y, x'<=x} in[y'][x'].  Replaces the two
  * scan launches of main/iimg.cu:37-82 (row scan into a transposed
  * temporary, then a scan of it).  Forms: uint8 -> uint32 and int32 -> int32
- * (exact; int32 wraps), float -> float with an fp64 column pass.
+ * (exact; int32 wraps), float -> float with row sums inside 16 x 256 tiles
+ * in fp32 and every vertical sum and carry in fp64.  Three launches (tile
+ * totals, carries, tile output); the input is read twice, the output
+ * written once.
  * workspace: device memory of umi_iimg_workspace_size(H, W, kind) bytes
  * (kind 0 = u8, 1 = i32, 2 = f32); NULL makes the call allocate and free it
- * itself, which synchronises the stream.  W <= 16000.
+ * itself, which synchronises the stream.
  * ------------------------------------------------------------------------- */
 umi_status umi_iimg_workspace_size(int H, int W, int kind, size_t *bytes);
 umi_status umi_iimg_u8_u32(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
@@ -232,8 +235,8 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
  * main/bilateral.cu:54, main/iimg.cu:37,60); here a batch object describes
  * n images inside one input and one output buffer (element offsets and
  * pitches), and each batched operator is one persistent launch over the
- * tiles of all images (the integral image: three launches, band / carry
- * scan / carry add).  Creation uploads the work tables once; the operator
+ * tiles of all images (the integral image: three launches, tile totals /
+ * carries / tile output).  Creation uploads the work tables once; the operator
  * calls allocate nothing and can be captured in a graph.  `config` selects
  * a tile shape (0 .. umi_batch_configs(op)-1, -1 = default); op 0 = 3x3
  * filter, 1 = bilateral.
@@ -247,7 +250,7 @@ umi_status umi_batch_wfilter3x3_f32(umi_batch b, const float *in, const float *w
                                     int mode, float *out, int config, umi_stream s);
 umi_status umi_batch_bilateral_f32(umi_batch b, const float *in, int radius, const float *spatial, float norm,
                                    float pad, int mode, float *out, int config, umi_stream s);
-/* kind 0: uint8 -> uint32, kind 2: float -> float (fp64 column pass) */
+/* kind 0: uint8 -> uint32, kind 2: float -> float (fp64 vertical sums and carries) */
 umi_status umi_batch_iimg_workspace_size(umi_batch b, int kind, size_t *bytes);
 umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, void *workspace,
                                  size_t workspace_bytes, umi_stream s);

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from umi_amd import ops, workloads
+from umi_amd import DeviceBuffer, ops, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -101,6 +101,23 @@ def test_iimg_max_value_and_ones(gpu):
     ones = np.ones((300, 500), np.int32)  # main/iimg.cu's input
     yy, xx = np.mgrid[0:300, 0:500]
     np.testing.assert_array_equal(ops.iimg_np(ones), ((yy + 1) * (xx + 1)).astype(np.int32))
+
+
+@pytest.mark.parametrize("H,W,pad", [(37, 300, 3), (5, 20000, 0), (300, 513, 13), (16, 256, 16)])
+def test_iimg_pitched_and_wide(gpu, H, W, pad):
+    # rows `pad` elements past W (pad 3 and 13 leave rows unaligned, so the tiles take the
+    # element path), a width of 79 tiles, and a single whole tile
+    g = workloads.rng(H * 7 + W + pad)
+    u8 = g.integers(0, 256, size=(H, W + pad), dtype=np.int64).astype(np.uint8)
+    f32 = g.random((H, W + pad), dtype=np.float32)
+    print(f"[iimg] pitched {H}x{W} pitch {W + pad}", flush=True)
+    o = DeviceBuffer(H * (W + 5) * 4)
+    ops.iimg(DeviceBuffer.from_array(u8), H, W, W + pad, o, W + 5, np.uint8)
+    np.testing.assert_array_equal(o.download((H, W + 5), np.uint32)[:, :W], oracle.iimg(u8[:, :W]))
+    ops.iimg(DeviceBuffer.from_array(f32), H, W, W + pad, o, W + 5, np.float32)
+    ref = oracle.iimg(np.ascontiguousarray(f32[:, :W]))
+    got = o.download((H, W + 5), np.float32)[:, :W]
+    assert np.max(np.abs(got.astype(np.float64) - ref) / ref) <= 4e-6
 
 
 @pytest.mark.parametrize("H,W", [(31, 65), (1024, 1024), (1080, 1920), (100, 4000)])

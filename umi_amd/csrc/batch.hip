@@ -84,53 +84,58 @@ __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *
     }
 }
 
-// Integral image, phase 1: one workgroup per band of one image.
+// Integral image (the passes of iimg_dev.h).  Passes A and C: one workgroup
+// per tile of every image; pass B: one thread per carry item (rows, columns,
+// tile columns) of every image.
 template <class T>
-__global__ __launch_bounds__(IM_THREADS) void batch_iimg_band_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                                     const int *__restrict__ band_prefix,
-                                                                     const long long *__restrict__ ws_off,
-                                                                     const typename T::In *__restrict__ in,
-                                                                     typename T::Out *__restrict__ out,
-                                                                     typename T::Col *__restrict__ ws) {
-    extern __shared__ uint32_t im_lds[];
-    const int total = band_prefix[n];
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                                       const int *__restrict__ tile_prefix,
+                                                                       const long long *__restrict__ ws_off,
+                                                                       const typename T::In *__restrict__ in,
+                                                                       typename T::Col *__restrict__ ws) {
+    __shared__ typename T::Row lds[IM_TH * IM_TW];
+    __shared__ typename T::Col tmp[4];
+    const int total = tile_prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(band_prefix, n, t);
+        const int i = find_item(tile_prefix, n, t);
         const BatchImage im = imgs[i];
-        const IimgPlan p = iimg_plan(im.H, im.W);
-        iimg_band<T>(reinterpret_cast<typename T::Row *>(im_lds), in + im.in_off, im.H, im.W, im.in_pitch, p,
-                     t - band_prefix[i], out + im.out_off, im.out_pitch, ws + ws_off[i]);
+        const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
+        const typename T::In *src = in + im.in_off;
+        iimg_tile_totals<T>(lds, tmp, src, im.H, im.W, im.in_pitch, iimg_rows16(src, im.in_pitch), local % ntx,
+                            local / ntx, iimg_ws<T>(ws + ws_off[i], im.H, im.W));
     }
 }
 
-// Phase 2: one thread per column of every image with more than one band.
 template <class T>
-__global__ void batch_iimg_scan_kernel(const BatchImage *__restrict__ imgs, int n, const int *__restrict__ col_prefix,
-                                       const long long *__restrict__ ws_off, typename T::Col *__restrict__ ws) {
-    const int total = col_prefix[n];
+__global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                               const int *__restrict__ item_prefix,
+                                                               const long long *__restrict__ ws_off,
+                                                               typename T::Col *__restrict__ ws) {
+    const int total = item_prefix[n];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        const int i = find_item(col_prefix, n, t);
+        const int i = find_item(item_prefix, n, t);
         const BatchImage im = imgs[i];
-        const IimgPlan p = iimg_plan(im.H, im.W);
-        iimg_carry_scan<T>(im.W, p.nb, t - col_prefix[i], ws + ws_off[i]);
+        iimg_carry_item<T>(im.H, im.W, t - item_prefix[i], iimg_ws<T>(ws + ws_off[i], im.H, im.W));
     }
 }
 
-// Phase 3: one workgroup per row (at or below the second band) of every image.
 template <class T>
-__global__ __launch_bounds__(256) void batch_iimg_add_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                             const int *__restrict__ row_prefix,
-                                                             const long long *__restrict__ ws_off,
-                                                             typename T::Out *__restrict__ out,
-                                                             const typename T::Col *__restrict__ ws) {
-    const int total = row_prefix[n];
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                                    const int *__restrict__ tile_prefix,
+                                                                    const long long *__restrict__ ws_off,
+                                                                    const typename T::In *__restrict__ in,
+                                                                    typename T::Col *__restrict__ ws,
+                                                                    typename T::Out *__restrict__ out) {
+    __shared__ typename T::Row lds[IM_TH * IM_TW];
+    __shared__ typename T::Col rl[IM_TH];
+    const int total = tile_prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(row_prefix, n, t);
+        const int i = find_item(tile_prefix, n, t);
         const BatchImage im = imgs[i];
-        const IimgPlan p = iimg_plan(im.H, im.W);
-        const int y = p.TB + (t - row_prefix[i]);
-        for (int x = threadIdx.x; x < im.W; x += blockDim.x)
-            iimg_carry_add<T>(im.W, p.TB, y, x, ws + ws_off[i], out + im.out_off, im.out_pitch);
+        const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
+        const typename T::In *src = in + im.in_off;
+        iimg_tile_out<T>(lds, rl, src, im.H, im.W, im.in_pitch, iimg_rows16(src, im.in_pitch), local % ntx,
+                         local / ntx, iimg_ws<T>(ws + ws_off[i], im.H, im.W), out + im.out_off, im.out_pitch);
     }
 }
 
@@ -138,14 +143,13 @@ __global__ __launch_bounds__(256) void batch_iimg_add_kernel(const BatchImage *_
 
 struct umi_batch_s {
     int n = 0;
-    int max_lds_iimg = 0;
     std::vector<BatchImage> host;
     BatchImage *imgs = nullptr;
     // device prefix arrays (n + 1 ints each) and their totals
     int *ftiles[NF] = {}, *btiles[NB] = {};
     int ftotal[NF] = {}, btotal[NB] = {};
-    int *bands = nullptr, *cols = nullptr, *rows = nullptr;
-    int nbands = 0, ncols = 0, nrows = 0;
+    int *itiles = nullptr, *icarry = nullptr;  // integral image tiles and carry items
+    int nitiles = 0, nicarry = 0;
     long long *ws_off = nullptr;
     long long ws_elems = 0;
     std::vector<void *> allocs;
@@ -195,17 +199,15 @@ umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *o
     const size_t need = (size_t)b->ws_elems * sizeof(typename T::Col);
     if (need && (!ws || ws_bytes < need)) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
     auto *w = static_cast<typename T::Col *>(ws);
-    hipLaunchKernelGGL(batch_iimg_band_kernel<T>, dim3(grid_for(b->nbands)), dim3(IM_THREADS), (size_t)b->max_lds_iimg,
-                       st, b->imgs, b->n, b->bands, b->ws_off, in, out, w);
-    UMI_TRY(launch_check("batch_iimg_band_kernel"));
-    if (b->ncols > 0) {
-        hipLaunchKernelGGL(batch_iimg_scan_kernel<T>, dim3(grid_for(ceil_div(b->ncols, 256))), dim3(256), 0, st,
-                           b->imgs, b->n, b->cols, b->ws_off, w);
-        UMI_TRY(launch_check("batch_iimg_scan_kernel"));
-        hipLaunchKernelGGL(batch_iimg_add_kernel<T>, dim3(grid_for(b->nrows)), dim3(256), 0, st, b->imgs, b->n,
-                           b->rows, b->ws_off, out, w);
-        UMI_TRY(launch_check("batch_iimg_add_kernel"));
-    }
+    hipLaunchKernelGGL(batch_iimg_totals_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs,
+                       b->n, b->itiles, b->ws_off, in, w);
+    UMI_TRY(launch_check("batch_iimg_totals_kernel"));
+    hipLaunchKernelGGL(batch_iimg_carry_kernel<T>, dim3(grid_for(ceil_div(b->nicarry, 256))), dim3(256), 0, st,
+                       b->imgs, b->n, b->icarry, b->ws_off, w);
+    UMI_TRY(launch_check("batch_iimg_carry_kernel"));
+    hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs, b->n,
+                       b->itiles, b->ws_off, in, w, out);
+    UMI_TRY(launch_check("batch_iimg_out_kernel"));
     return UMI_OK;
 }
 
@@ -221,7 +223,7 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         return set_error(UMI_ERR_INVALID, "umi_batch_create: n=%d or null arrays", n);
     auto *b = new umi_batch_s;
     b->n = n;
-    std::vector<int> fp[NF], bp[NB], bands(n + 1, 0), cols(n + 1, 0), rows(n + 1, 0);
+    std::vector<int> fp[NF], bp[NB], itiles(n + 1, 0), icarry(n + 1, 0);
     for (int c = 0; c < NF; ++c) fp[c].assign(n + 1, 0);
     for (int c = 0; c < NB; ++c) bp[c].assign(n + 1, 0);
     std::vector<long long> wso(n, 0);
@@ -232,20 +234,14 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
             return set_error(UMI_ERR_INVALID, "umi_batch_create: image %d: H=%d W=%d pitches %d %d", i, H[i], W[i],
                              in_pitch[i], out_pitch[i]);
         }
-        const IimgPlan p = iimg_plan(H[i], W[i]);
-        if (p.TB < 1) {
-            delete b;
-            return set_error(UMI_ERR_UNSUPPORTED, "umi_batch_create: image %d width %d", i, W[i]);
-        }
+        const IimgGeom g = iimg_geom(H[i], W[i]);
         b->host.push_back({in_off[i], out_off[i], H[i], W[i], in_pitch[i], out_pitch[i]});
         for (int c = 0; c < NF; ++c) fp[c][i + 1] = fp[c][i] + ceil_div(W[i], FTX[c]) * ceil_div(H[i], FTY[c]);
         for (int c = 0; c < NB; ++c) bp[c][i + 1] = bp[c][i] + ceil_div(W[i], BTX[c]) * ceil_div(H[i], BTY[c]);
-        bands[i + 1] = bands[i] + p.nb;
-        cols[i + 1] = cols[i] + (p.nb > 1 ? W[i] : 0);
-        rows[i + 1] = rows[i] + (p.nb > 1 ? H[i] - p.TB : 0);
+        itiles[i + 1] = itiles[i] + g.ntx * g.nty;
+        icarry[i + 1] = icarry[i] + iimg_carry_items(H[i], W[i]);
         wso[i] = ws;
-        ws += (long long)p.nb * W[i];
-        b->max_lds_iimg = std::max(b->max_lds_iimg, 4 * p.LS * p.TB);
+        ws += iimg_ws_elems(H[i], W[i]);
     }
     umi_status st = upload(b, b->host, &b->imgs);
     for (int c = 0; c < NF && st == UMI_OK; ++c) {
@@ -256,13 +252,11 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         st = upload(b, bp[c], &b->btiles[c]);
         b->btotal[c] = bp[c][n];
     }
-    if (st == UMI_OK) st = upload(b, bands, &b->bands);
-    if (st == UMI_OK) st = upload(b, cols, &b->cols);
-    if (st == UMI_OK) st = upload(b, rows, &b->rows);
+    if (st == UMI_OK) st = upload(b, itiles, &b->itiles);
+    if (st == UMI_OK) st = upload(b, icarry, &b->icarry);
     if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
-    b->nbands = bands[n];
-    b->ncols = cols[n];
-    b->nrows = rows[n];
+    b->nitiles = itiles[n];
+    b->nicarry = icarry[n];
     b->ws_elems = ws;
     if (st != UMI_OK) {
         for (void *p : b->allocs) hipFree(p);

@@ -6,18 +6,12 @@
 // (row scan into a transposed temporary, then a scan of that temporary;
 // main/iimg.cu:11-18,37-82, check loop main/iimg.cu:84-112, johnjohnlin/UMI).
 // Element forms here: uint8 -> uint32 and int32 -> int32 (exact, wrapping),
-// float32 -> float32 with the row pass in fp32 and the column pass in fp64.
+// float32 -> float32 with row sums inside a tile in fp32 and every vertical
+// sum and carry in fp64.
 //
-// Design (three launches, HBM-streaming):
-//  K1 band kernel: a workgroup loads a band of TB rows into LDS (coalesced),
-//     each wave scans whole rows (per-lane contiguous chunks + a DPP wave
-//     scan of the chunk totals), then each thread runs down one column of the
-//     band (column accumulator in Tcol), writing the band-local table and the
-//     band's bottom row (its column totals) to the workspace;
-//  K2 per column, an exclusive scan of the band totals (nb values);
-//  K3 adds each band's carry to its rows (bands >= 1).
-// LDS rows use the padded index e + e / C (C = per-lane chunk) so the
-// chunk-strided accesses of a wave fall in distinct banks.
+// Design: three launches of the tiled reduce-then-scan in iimg_dev.h (tile
+// totals, carries, tile output).  The input is read twice and the output
+// written once; the carries are small (row, column and tile prefix sums).
 #include "common.h"
 #include "iimg_dev.h"
 
@@ -25,31 +19,32 @@ namespace {
 using namespace umi;
 
 template <class T>
-__global__ __launch_bounds__(IM_THREADS) void iimg_band_kernel(const typename T::In *__restrict__ in, int H, int W,
-                                                               int ip, IimgPlan p, typename T::Out *__restrict__ out,
-                                                               int op, typename T::Col *__restrict__ totals) {
-    extern __shared__ uint32_t im_lds[];
-    iimg_band<T>(reinterpret_cast<typename T::Row *>(im_lds), in, H, W, ip, p, blockIdx.x, out, op, totals);
+__global__ __launch_bounds__(IM_THREADS) void iimg_totals_kernel(const typename T::In *__restrict__ in, int H, int W,
+                                                                 int ip, typename T::Col *__restrict__ ws) {
+    __shared__ typename T::Row lds[IM_TH * IM_TW];
+    __shared__ typename T::Col tmp[4];
+    iimg_tile_totals<T>(lds, tmp, in, H, W, ip, iimg_rows16(in, ip), blockIdx.x, blockIdx.y, iimg_ws<T>(ws, H, W));
 }
 
 template <class T>
-__global__ void iimg_carry_scan_kernel(int W, int nb, typename T::Col *__restrict__ totals) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < W) iimg_carry_scan<T>(W, nb, x, totals);
+__global__ __launch_bounds__(256) void iimg_carry_kernel(int H, int W, typename T::Col *__restrict__ ws) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // whole waves: the item count is a multiple of 64
+    if (k < iimg_carry_items(H, W)) iimg_carry_item<T>(H, W, k, iimg_ws<T>(ws, H, W));
 }
 
 template <class T>
-__global__ void iimg_carry_add_kernel(int H, int W, int TB, const typename T::Col *__restrict__ carry,
-                                      typename T::Out *__restrict__ out, int op) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = TB + blockIdx.y;  // band 0 has no carry
-    if (x < W && y < H) iimg_carry_add<T>(W, TB, y, x, carry, out, op);
+__global__ __launch_bounds__(IM_THREADS) void iimg_out_kernel(const typename T::In *__restrict__ in, int H, int W,
+                                                              int ip, typename T::Col *__restrict__ ws,
+                                                              typename T::Out *__restrict__ out, int op) {
+    __shared__ typename T::Row lds[IM_TH * IM_TW];
+    __shared__ typename T::Col rl[IM_TH];
+    iimg_tile_out<T>(lds, rl, in, H, W, ip, iimg_rows16(in, ip), blockIdx.x, blockIdx.y, iimg_ws<T>(ws, H, W), out,
+                     op);
 }
 
 template <class T>
 size_t iimg_ws_bytes(int H, int W) {
-    const IimgPlan p = iimg_plan(H, W);
-    return (size_t)p.nb * W * sizeof(typename T::Col);
+    return (size_t)iimg_ws_elems(H, W) * sizeof(typename T::Col);
 }
 
 template <class T>
@@ -59,8 +54,7 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     if (H < 0 || W < 0 || ip < W || op < W)
         return set_error(UMI_ERR_INVALID, "%s: H=%d W=%d in_pitch=%d out_pitch=%d", name, H, W, ip, op);
     if (H == 0 || W == 0) return UMI_OK;
-    const IimgPlan p = iimg_plan(H, W);
-    if (p.TB < 1) return set_error(UMI_ERR_UNSUPPORTED, "%s: width %d exceeds one LDS row band", name, W);
+    const IimgGeom g = iimg_geom(H, W);
     const size_t need = iimg_ws_bytes<T>(H, W);
     hipStream_t st = as_stream(s);
     void *tmp = ws;
@@ -71,18 +65,17 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     } else if (ws_bytes < need) {
         return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", name, ws_bytes, need);
     }
-    auto *tot = static_cast<typename T::Col *>(tmp);
-    hipLaunchKernelGGL(iimg_band_kernel<T>, dim3(p.nb), dim3(IM_THREADS), (size_t)4 * p.LS * p.TB, st, in, H, W,
-                       ip, p, out, op, tot);
+    auto *w = static_cast<typename T::Col *>(tmp);
+    hipLaunchKernelGGL(iimg_totals_kernel<T>, dim3(g.ntx, g.nty), dim3(IM_THREADS), 0, st, in, H, W, ip, w);
     umi_status rc = launch_check(name);
-    if (rc == UMI_OK && p.nb > 1) {
-        hipLaunchKernelGGL(iimg_carry_scan_kernel<T>, dim3(ceil_div(W, 256)), dim3(256), 0, st, W, p.nb, tot);
+    if (rc == UMI_OK) {
+        hipLaunchKernelGGL(iimg_carry_kernel<T>, dim3(ceil_div(iimg_carry_items(H, W), 256)), dim3(256), 0, st, H, W,
+                           w);
         rc = launch_check(name);
-        if (rc == UMI_OK) {
-            hipLaunchKernelGGL(iimg_carry_add_kernel<T>, dim3(ceil_div(W, 256), H - p.TB), dim3(256), 0, st, H, W,
-                               p.TB, tot, out, op);
-            rc = launch_check(name);
-        }
+    }
+    if (rc == UMI_OK) {
+        hipLaunchKernelGGL(iimg_out_kernel<T>, dim3(g.ntx, g.nty), dim3(IM_THREADS), 0, st, in, H, W, ip, w, out, op);
+        rc = launch_check(name);
     }
     if (own) {
         hipError_t e = hipStreamSynchronize(st);

@@ -1,9 +1,28 @@
-// Device-side bodies of the integral image (band scan, carry scan, carry
-// add), shared by the one-image kernels (iimg.hip) and the batch kernels
-// (batch.hip).  Semantics: main/iimg.cu:11-18,37-82,84-112 (johnjohnlin/UMI).
+// Device-side bodies of the integral image, shared by the one-image kernels
+// (iimg.hip) and the batch kernels (batch.hip).  Semantics:
+// main/iimg.cu:11-18,37-82,84-112 (johnjohnlin/UMI): out[y][x] is the sum of
+// in[y'][x'] over y' <= y, x' <= x.
+//
+// Tiled reduce-then-scan, three passes over tiles of IM_TH x IM_TW:
+//  A (totals)  per tile: the row totals of its IM_TH rows, L[y][tx]; the
+//              within-tile prefix over x of its column totals, C[ty][x]; the
+//              tile total, T[ty][tx].  Reads the input once.
+//  B (carries) per row y: L becomes the exclusive prefix over tx (the sum of
+//              the row left of the tile); per column x: C becomes the
+//              exclusive prefix over ty (the sum, above the tile, of the
+//              tile-local row prefixes); per tile column tx: K[ty][tx] = the
+//              sum of T over tiles above and to the left.
+//  C (output)  per tile: the within-tile row prefix lrp(y, x), then down each
+//              column acc = K + C + sum over the tile's rows so far of
+//              (L[y] + lrp(y, x)).  Reads the input again, writes the output
+//              once.
+// out(y, x) = K(ty, tx) + C(ty, x) + sum_{y' = ty0 .. y} (L(y', tx) + lrp(y', x))
+// is the summed-area table split into the part above-left of the tile, above
+// it, left of it and inside it.  Integer forms are exact (mod 2^32) in any
+// order; the float form sums rows within a tile in fp32 and everything
+// vertical and every carry in fp64.
 #pragma once
 #include "common.h"
-#include "wave.h"
 
 namespace umi {
 
@@ -19,80 +38,219 @@ using II32 = IimgTypes<int32_t, uint32_t, uint32_t, int32_t>;
 using IF32 = IimgTypes<float, float, double, float>;
 
 constexpr int IM_THREADS = 256;
-constexpr int IM_LDS = 64 * 1024;
+constexpr int IM_TW = 256, IM_TH = 16;  // thread (r = t / 16, s = t % 16) holds row r, columns 16 s .. 16 s + 15
 
-struct IimgPlan {
-    int C, LS, TB, nb;
+struct IimgGeom {
+    int ntx, nty;
 };
-__host__ __device__ inline IimgPlan iimg_plan(int H, int W) {
-    IimgPlan p;
-    p.C = ceil_div(W, 64);
-    p.LS = W + ceil_div(W, p.C) + 1;
-    p.TB = IM_LDS / (4 * p.LS);
-    if (p.TB > 32) p.TB = 32;
-    p.nb = p.TB > 0 ? ceil_div(H, p.TB) : 0;
-    return p;
+__host__ __device__ inline IimgGeom iimg_geom(int H, int W) { return {ceil_div(W, IM_TW), ceil_div(H, IM_TH)}; }
+
+// Workspace of one image, in Col elements: L[H][ntx], C[nty][W], T[nty][ntx], K[nty][ntx].
+__host__ __device__ inline long long iimg_ws_elems(int H, int W) {
+    const IimgGeom g = iimg_geom(H, W);
+    return (long long)H * g.ntx + (long long)g.nty * W + 2LL * g.nty * g.ntx;
 }
 
-// Band b (rows b*TB ..) of one image by 256 threads: band-local summed-area
-// table into `out`, the band's column totals into totals[b][x].  `rows` is
-// LDS of TB x LS row elements.  Ends with a barrier.
 template <class T>
-__device__ inline void iimg_band(typename T::Row *rows, const typename T::In *__restrict__ in, int H, int W, int ip,
-                                 const IimgPlan &p, int b, typename T::Out *__restrict__ out, int op,
-                                 typename T::Col *__restrict__ totals) {
+struct IimgWs {
+    typename T::Col *L, *C, *T_, *K;
+};
+template <class T>
+__device__ inline IimgWs<T> iimg_ws(typename T::Col *ws, int H, int W) {
+    const IimgGeom g = iimg_geom(H, W);
+    IimgWs<T> w;
+    w.L = ws;
+    w.C = w.L + (size_t)H * g.ntx;
+    w.T_ = w.C + (size_t)g.nty * W;
+    w.K = w.T_ + (size_t)g.nty * g.ntx;
+    return w;
+}
+
+// True when every row of `in` starts 16-byte aligned.
+template <class In>
+__device__ inline bool iimg_rows16(const In *in, int pitch) {
+    return ((reinterpret_cast<uintptr_t>(in) | ((uintptr_t)pitch * sizeof(In))) & 15) == 0;
+}
+
+// The thread's 16 elements of row y (zeros past the image).
+template <class T>
+__device__ inline void iimg_load16(typename T::Row (&v)[16], const typename T::In *__restrict__ in, int H, int W,
+                                   int ip, bool rows16, int y, int x) {
+    using In = typename T::In;
+    using Row = typename T::Row;
+    if (y < H && rows16 && x + 16 <= W) {
+        constexpr int NQ = 16 * (int)sizeof(In) / 16;
+        uint4 q[NQ];
+        const uint4 *src = reinterpret_cast<const uint4 *>(in + (size_t)y * ip + x);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) q[i] = src[i];
+        const In *e = reinterpret_cast<const In *>(q);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = (Row)e[j];
+    } else {
+        // clamped indices keep every load inside the image; the select zeroes the rest
+        const In *row = in + (size_t)min(y, H - 1) * ip;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const Row e = (Row)row[min(x + j, W - 1)];
+            v[j] = (y < H && x + j < W) ? e : (Row)0;
+        }
+    }
+}
+
+// Inclusive prefix over the 256 threads of the workgroup (tmp: 4 values of
+// LDS).  Contains a barrier; the caller separates reuses of tmp by another.
+template <class V>
+__device__ inline V iimg_block_scan(V v, V *tmp) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const V n = __shfl_up(v, d);
+        if (lane >= d) v += n;
+    }
+    if (lane == 63) tmp[wv] = v;
+    __syncthreads();
+    V off = 0;
+    for (int i = 0; i < wv; ++i) off += tmp[i];
+    return v + off;
+}
+
+// Pass A for tile (tx, ty).  lds: IM_TH x IM_TW Row values; tmp: 4 Col.
+template <class T>
+__device__ inline void iimg_tile_totals(typename T::Row *lds, typename T::Col *tmp,
+                                        const typename T::In *__restrict__ in, int H, int W, int ip, bool rows16,
+                                        int tx, int ty, const IimgWs<T> &w) {
     using Row = typename T::Row;
     using Col = typename T::Col;
-    const int C = p.C, LS = p.LS;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int y0 = b * p.TB;
-    const int nr = min(p.TB, H - y0);
-    for (int r = 0; r < nr; ++r) {
-        const typename T::In *src = in + (size_t)(y0 + r) * ip;
-        for (int x = tid; x < W; x += IM_THREADS) rows[r * LS + x + x / C] = (Row)src[x];
-    }
+    const int t = threadIdx.x, r = t >> 4, s = t & 15;
+    const int ntx = iimg_geom(H, W).ntx;
+    const int y = ty * IM_TH + r, x = tx * IM_TW + 16 * s;
+    Row v[16];
+    iimg_load16<T>(v, in, H, W, ip, rows16, y, x);
+    Row rs = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) rs += v[j];
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) rs += __shfl_xor(rs, m, 16);
+    if (s == 0 && y < H) w.L[(size_t)y * ntx + tx] = (Col)rs;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) lds[r * IM_TW + 16 * s + j] = v[j];
     __syncthreads();
-    for (int r = wv; r < nr; r += IM_THREADS / 64) {
-        Row *row = rows + r * LS + lane * (C + 1);
-        const int n = min(C, W - lane * C);
-        Row run = 0;
-        for (int k = 0; k < n; ++k) {
-            run = run + row[k];
-            row[k] = run;
-        }
-        const Row incl = wave_inclusive_scan(run, [](Row a, Row c) { return a + c; });
-        const Row excl = incl - run;
-        for (int k = 0; k < n; ++k) row[k] = row[k] + excl;
-    }
-    __syncthreads();
-    for (int x = tid; x < W; x += IM_THREADS) {
-        Col acc = 0;
-        for (int r = 0; r < nr; ++r) {
-            acc = acc + (Col)rows[r * LS + x + x / C];
-            out[(size_t)(y0 + r) * op + x] = (typename T::Out)acc;
-        }
-        totals[(size_t)b * W + x] = acc;
-    }
+    Col c = 0;
+#pragma unroll
+    for (int rr = 0; rr < IM_TH; ++rr) c += (Col)lds[rr * IM_TW + t];
+    const Col incl = iimg_block_scan(c, tmp);
+    const int xc = tx * IM_TW + t;
+    if (xc < W) w.C[(size_t)ty * W + xc] = incl;
+    if (t == IM_THREADS - 1) w.T_[ty * ntx + tx] = incl;  // columns past W hold zeros
     __syncthreads();
 }
 
-// Exclusive scan over the nb band totals of column x (in place).
+// Pass B items of one image: 64 * ntx corner items (one wave per tile
+// column; lanes over tile rows), then H row items, then W column items,
+// padded to a multiple of 64 so that a wave never mixes images or kinds.
+__host__ __device__ inline int iimg_carry_items(int H, int W) {
+    return round_up(64 * iimg_geom(H, W).ntx + H + W, 64);
+}
+
+// Pass B, item k of [0, iimg_carry_items(H, W)).  Loads are issued in
+// batches ahead of the dependent sums.
 template <class T>
-__device__ inline void iimg_carry_scan(int W, int nb, int x, typename T::Col *__restrict__ totals) {
-    typename T::Col run = 0;
-    for (int b = 0; b < nb; ++b) {
-        const typename T::Col t = totals[(size_t)b * W + x];
-        totals[(size_t)b * W + x] = run;
-        run = run + t;
+__device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) {
+    using Col = typename T::Col;
+    const IimgGeom g = iimg_geom(H, W);
+    constexpr int U = 16;
+    const int nc = 64 * g.ntx;
+    if (k < nc) {
+        // K[ty][tx] = sum over ty' < ty of P(ty'), P(ty') = sum over tx' < tx of T[ty'][tx']
+        const int tx = k >> 6, lane = k & 63;
+        const Col *__restrict__ Tp = w.T_;
+        Col *__restrict__ Kp = w.K;
+        Col off = 0;
+        for (int ty0 = 0; ty0 < g.nty; ty0 += 64) {
+            const int ty = ty0 + lane;
+            Col p = 0;
+            if (ty < g.nty)
+                for (int i = 0; i < tx; ++i) p += Tp[ty * g.ntx + i];
+            Col incl = p;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const Col n = __shfl_up(incl, d);
+                if (lane >= d) incl += n;
+            }
+            Col excl = __shfl_up(incl, 1);
+            if (lane == 0) excl = 0;
+            if (ty < g.nty) Kp[ty * g.ntx + tx] = off + excl;
+            off += __shfl(incl, 63);
+        }
+    } else if (k < nc + H) {
+        Col *__restrict__ p = w.L + (size_t)(k - nc) * g.ntx;
+        Col run = 0;
+        for (int i0 = 0; i0 < g.ntx; i0 += U) {
+            Col v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = i0 + u < g.ntx ? p[i0 + u] : (Col)0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (i0 + u < g.ntx) {
+                    p[i0 + u] = run;
+                    run += v[u];
+                }
+        }
+    } else if (k < nc + H + W) {
+        Col *__restrict__ p = w.C + (k - nc - H);
+        Col run = 0;
+        for (int j0 = 0; j0 < g.nty; j0 += U) {
+            Col v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = j0 + u < g.nty ? p[(size_t)(j0 + u) * W] : (Col)0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j0 + u < g.nty) {
+                    p[(size_t)(j0 + u) * W] = run;
+                    run += v[u];
+                }
+        }
     }
 }
 
-// out[y][x] += carry of y's band.
+// Pass C for tile (tx, ty).  lds: IM_TH x IM_TW Row values; rl: IM_TH Col.
 template <class T>
-__device__ inline void iimg_carry_add(int W, int TB, int y, int x, const typename T::Col *__restrict__ carry,
-                                      typename T::Out *__restrict__ out, int op) {
-    typename T::Out *o = out + (size_t)y * op + x;
-    *o = (typename T::Out)((typename T::Col)*o + carry[(size_t)(y / TB) * W + x]);
+__device__ inline void iimg_tile_out(typename T::Row *lds, typename T::Col *rl, const typename T::In *__restrict__ in,
+                                     int H, int W, int ip, bool rows16, int tx, int ty, const IimgWs<T> &w,
+                                     typename T::Out *__restrict__ out, int op) {
+    using Row = typename T::Row;
+    using Col = typename T::Col;
+    const int t = threadIdx.x, r = t >> 4, s = t & 15;
+    const int ntx = iimg_geom(H, W).ntx;
+    const int y = ty * IM_TH + r, x = tx * IM_TW + 16 * s;
+    Row v[16];
+    iimg_load16<T>(v, in, H, W, ip, rows16, y, x);
+#pragma unroll
+    for (int j = 1; j < 16; ++j) v[j] += v[j - 1];
+    Row incl = v[15];
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        const Row n = __shfl_up(incl, d, 16);
+        if (s >= d) incl += n;
+    }
+    Row excl = __shfl_up(incl, 1, 16);
+    if (s == 0) excl = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) lds[r * IM_TW + 16 * s + j] = excl + v[j];
+    if (t < IM_TH) {
+        const int yy = ty * IM_TH + t;
+        rl[t] = yy < H ? w.L[(size_t)yy * ntx + tx] : (Col)0;
+    }
+    __syncthreads();
+    const int xc = tx * IM_TW + t;
+    Col acc = w.K[ty * ntx + tx] + (xc < W ? w.C[(size_t)ty * W + xc] : (Col)0);
+    const int nr = min(IM_TH, H - ty * IM_TH);
+    for (int rr = 0; rr < nr; ++rr) {
+        acc += rl[rr] + (Col)lds[rr * IM_TW + t];
+        if (xc < W) out[(size_t)(ty * IM_TH + rr) * op + xc] = (typename T::Out)acc;
+    }
+    __syncthreads();
 }
 
 }  // namespace umi
