@@ -146,11 +146,12 @@ __device__ inline void iimg_tile_totals(typename T::Row *lds, typename T::Col *t
     __syncthreads();
 }
 
-// Pass B items of one image: 64 * ntx corner items (one wave per tile
-// column; lanes over tile rows), then H row items, then W column items,
-// padded to a multiple of 64 so that a wave never mixes images or kinds.
+// Pass B items of one image, in whole waves: 64 * ntx corner items (a wave
+// per tile column; lanes over tile rows), H row items padded to a multiple
+// of 64, then 64 * ceil(W / 8) column items (a wave per 8 columns; lane =
+// column c + 8 * segment q, the 8 segments splitting the tile rows).
 __host__ __device__ inline int iimg_carry_items(int H, int W) {
-    return round_up(64 * iimg_geom(H, W).ntx + H + W, 64);
+    return 64 * iimg_geom(H, W).ntx + round_up(H, 64) + 64 * ceil_div(W, 8);
 }
 
 // Pass B, item k of [0, iimg_carry_items(H, W)).  Loads are issued in
@@ -160,10 +161,11 @@ __device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) 
     using Col = typename T::Col;
     const IimgGeom g = iimg_geom(H, W);
     constexpr int U = 16;
-    const int nc = 64 * g.ntx;
+    const int nc = 64 * g.ntx, nr = round_up(H, 64);
+    const int lane = k & 63;
     if (k < nc) {
         // K[ty][tx] = sum over ty' < ty of P(ty'), P(ty') = sum over tx' < tx of T[ty'][tx']
-        const int tx = k >> 6, lane = k & 63;
+        const int tx = k >> 6;
         const Col *__restrict__ Tp = w.T_;
         Col *__restrict__ Kp = w.K;
         Col off = 0;
@@ -183,8 +185,10 @@ __device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) 
             if (ty < g.nty) Kp[ty * g.ntx + tx] = off + excl;
             off += __shfl(incl, 63);
         }
-    } else if (k < nc + H) {
-        Col *__restrict__ p = w.L + (size_t)(k - nc) * g.ntx;
+    } else if (k < nc + nr) {
+        const int y = k - nc;
+        if (y >= H) return;
+        Col *__restrict__ p = w.L + (size_t)y * g.ntx;
         Col run = 0;
         for (int i0 = 0; i0 < g.ntx; i0 += U) {
             Col v[U];
@@ -197,17 +201,37 @@ __device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) 
                     run += v[u];
                 }
         }
-    } else if (k < nc + H + W) {
-        Col *__restrict__ p = w.C + (k - nc - H);
-        Col run = 0;
-        for (int j0 = 0; j0 < g.nty; j0 += U) {
+    } else {
+        // C[ty][x] becomes the exclusive prefix over ty: segment totals, a
+        // scan of the 8 segments across lanes, then the segments rewritten
+        const int q = lane >> 3, x = ((k - nc - nr) >> 6) * 8 + (lane & 7);
+        const int seg = ceil_div(g.nty, 8), j0 = q * seg, j1 = min(g.nty, j0 + seg);
+        Col *__restrict__ p = w.C + min(x, W - 1);
+        const bool own = x < W;
+        Col tot = 0;
+        for (int j = j0; j < j1; j += U) {
             Col v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = j0 + u < g.nty ? p[(size_t)(j0 + u) * W] : (Col)0;
+            for (int u = 0; u < U; ++u) v[u] = own && j + u < j1 ? p[(size_t)(j + u) * W] : (Col)0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) tot += v[u];
+        }
+        Col incl = tot;
+#pragma unroll
+        for (int d = 8; d < 64; d <<= 1) {
+            const Col n = __shfl_up(incl, d);
+            if (lane >= d) incl += n;
+        }
+        Col run = __shfl_up(incl, 8);
+        if (q == 0) run = 0;
+        for (int j = j0; j < j1; j += U) {
+            Col v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = own && j + u < j1 ? p[(size_t)(j + u) * W] : (Col)0;
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (j0 + u < g.nty) {
-                    p[(size_t)(j0 + u) * W] = run;
+                if (own && j + u < j1) {
+                    p[(size_t)(j + u) * W] = run;
                     run += v[u];
                 }
         }
