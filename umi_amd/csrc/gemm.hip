@@ -30,9 +30,21 @@
 #include "common.h"
 
 #include <algorithm>
+#include <type_traits>
+#include <utility>
 
 namespace {
 using namespace umi;
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>), in order.
+template <class F, int... I>
+__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ inline void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
@@ -98,10 +110,16 @@ struct DenseSrc {
     __device__ Kp kprep(int kb, int /*esz*/) const { return {kb}; }
     __device__ const uint8_t *at(const P &p, const Kp &k) const { return p.p ? p.p + k.kb : nullptr; }
 };
+// floor(x / d) = umulhi(x, recip32(d)) for d >= 2 and x * d < 2^32: the
+// multiplier is ceil(2^32 / d) rounded from below, so x * m / 2^32 exceeds
+// x / d by less than x / 2^32 <= 1 / d.
+inline uint32_t recip32(int d) { return d > 1 ? 0xffffffffu / (uint32_t)d + 1u : 0u; }
+
 struct ConvSrc {
     const uint8_t *base;  // NHWC input
     int N, H, W, C;       // C in elements
     int OH, OW, R, S, stride, pad, dil;
+    uint32_t mC, mS;      // recip32(C), recip32(S): the per-stage divisions by multiply-high
     struct P {
         const uint8_t *img;  // start of image n, or null past the last pixel
         int iy0, ix0;        // oy * stride - pad, ox * stride - pad
@@ -119,8 +137,8 @@ struct ConvSrc {
     }
     __device__ Kp kprep(int kbyte, int esz) const {
         const int k = kbyte / esz;
-        const int rs = k / C, c = k - rs * C;
-        const int r = rs / S, s = rs - r * S;
+        const int rs = (int)__umulhi((uint32_t)k, mC), c = k - rs * C;  // C >= 4
+        const int r = S == 1 ? rs : (int)__umulhi((uint32_t)rs, mS), s = rs - r * S;
         return {r * dil, s * dil, c * esz, C * esz};
     }
     __device__ const uint8_t *at(const P &p, const Kp &k) const {
@@ -137,8 +155,13 @@ struct DenseDst {
     float *base;
     long ldc;
     int M, N;
-    __device__ void put(int m, int n, float v) const {
-        if (m < M && n < N) base[(long)m * ldc + n] = v;
+    struct Col {
+        int n;
+        bool ok;
+    };
+    __device__ Col col(int n) const { return {n, n < N}; }
+    __device__ void put(int m, const Col &c, float v) const {
+        if (m < M && c.ok) base[(long)m * ldc + c.n] = v;
     }
 };
 // NHWC bf16 output with per-channel bias and optional ReLU (the layer
@@ -146,29 +169,44 @@ struct DenseDst {
 struct ConvDst {
     float *base;  // [N][K][OH][OW]
     int K, npix, total;
-    __device__ void put(int m, int j, float v) const {
-        if (m < K && j < total) {
-            const int n = j / npix, p = j - n * npix;
-            base[((long)n * K + m) * npix + p] = v;
-        }
+    struct Col {
+        long off;  // element offset of (image n, channel 0, pixel p)
+        bool ok;
+    };
+    __device__ Col col(int j) const {
+        const int n = j / npix, p = j - n * npix;
+        return {(long)n * K * npix + p, j < total};
+    }
+    __device__ void put(int m, const Col &c, float v) const {
+        if (m < K && c.ok) base[c.off + (long)m * npix] = v;
     }
 };
 
-// 16 bytes of one panel row (zeros where the source row is absent).
-__device__ inline i32x4 fetch16(const uint8_t *p) {
-    return p ? *(const i32x4 *)p : i32x4{0, 0, 0, 0};
+// 16 bytes of one panel row.  ZERO_NOW: a load only for present rows, zeros
+// otherwise (the form for one stage in flight).  Otherwise the load is
+// unconditional (`safe`, the operand's own base, stands in for an absent
+// row) and the caller zeroes absent rows only when it stores the chunk to
+// LDS: a branch or a select right after the load makes the wait counters
+// drain every load in flight, which defeats a deeper prefetch.
+template <bool ZERO_NOW>
+__device__ inline i32x4 fetch16(const uint8_t *p, const uint8_t *safe) {
+    if constexpr (ZERO_NOW) return p ? *(const i32x4 *)p : i32x4{0, 0, 0, 0};
+    return *(const i32x4 *)(p ? p : safe);
 }
 
-// Workgroup tile: WGM x WGN waves, each wave TM x TN MFMA blocks of 32 x 32.
-template <int WGM_, int WGN_, int TM_, int TN_>
+// Workgroup tile: WGM x WGN waves, each wave TM x TN MFMA blocks of 32 x 32;
+// PF K stages of global loads are kept in flight in registers (a stage's
+// MFMA work is short next to a global load's latency, so one stage of
+// prefetch leaves the loop waiting on every load).
+template <int WGM_, int WGN_, int TM_, int TN_, int PF_>
 struct Tile {
-    static constexpr int WGM = WGM_, WGN = WGN_, TM = TM_, TN = TN_;
+    static constexpr int WGM = WGM_, WGN = WGN_, TM = TM_, TN = TN_, PF = PF_;
     static constexpr int BM = WGM * TM * 32, BN = WGN * TN * 32;
     static_assert(WGM * WGN * 64 == NTHREADS, "four waves");
 };
-using TileGemm = Tile<2, 2, 4, 4>;   // 256 x 256: large GEMMs
-using TileMid = Tile<2, 2, 2, 2>;    // 128 x 128
-using TileNarrow = Tile<1, 4, 2, 1>; // 64 x 128: few output rows (conv with K <= 64)
+using TileGemm = Tile<2, 2, 4, 4, 1>;   // 256 x 256: large GEMMs (the accumulators fill the register file)
+using TileMid = Tile<2, 2, 2, 2, 2>;    // 128 x 128
+using TileNarrow = Tile<1, 4, 2, 1, 4>; // 64 x 128: few output rows (conv with K <= 64)
 
 template <class TL, int MODE, class ASrc, class BSrc, class Dst>
 __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BSrc b_hi, BSrc b_lo, Dst dst, int K,
@@ -197,9 +235,16 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
     const int m0 = tm * BMt, n0 = tn * BNt;
 
     const int nstages = K * OP::ESZ / ROWB;
-    // global -> LDS assignment: thread t copies 16 B chunk (t & 3) of rows t/4 + 64 i
+    // global -> LDS assignment: thread t copies 16 B chunk (t & 3) of rows t/4 + 64 i;
+    // stage s travels in register slot s % PF (at most 2 for the split forms,
+    // whose hi and lo panels double the ring's registers)
+    constexpr int PF = SPLIT && TL::PF > 2 ? 2 : TL::PF;
     const int lrow = tid >> 2, lchunk = (tid & 3) * 16;
-    i32x4 ra[SPLIT ? 2 : 1][RA], rb[SPLIT ? 2 : 1][RB];
+    i32x4 ra[PF][SPLIT ? 2 : 1][RA], rb[PF][SPLIT ? 2 : 1][RB];
+    // false: absent row, stored as zeros (PF == 1 zeroes at load time: the
+    // 256 x 256 tile has no registers to spare for the flags)
+    constexpr bool ZN = PF == 1;
+    bool oka[PF][SPLIT ? 2 : 1][RA], okb[PF][SPLIT ? 2 : 1][RB];
     typename ASrc::P pa_[RA], pal_[RA];
     typename BSrc::P pb_[RB], pbl_[RB];
 #pragma unroll
@@ -212,31 +257,47 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
         pb_[i] = b_hi.prep(n0 + lrow + 64 * i, OP::ESZ);
         if constexpr (SPLIT) pbl_[i] = b_lo.prep(n0 + lrow + 64 * i, OP::ESZ);
     }
-    auto gload = [&](int st) {
+    auto gload = [&](int st, auto slot_tag) {
+        constexpr int SL = decltype(slot_tag)::value;
         const int kb = st * ROWB + lchunk;
         const auto ka = a_hi.kprep(kb, OP::ESZ);
         const auto kbp = b_hi.kprep(kb, OP::ESZ);
 #pragma unroll
         for (int i = 0; i < RA; ++i) {
-            ra[0][i] = fetch16(a_hi.at(pa_[i], ka));
-            if constexpr (SPLIT) ra[1][i] = fetch16(a_lo.at(pal_[i], ka));
+            const uint8_t *p = a_hi.at(pa_[i], ka);
+            ra[SL][0][i] = fetch16<ZN>(p, a_hi.base);
+            oka[SL][0][i] = ZN || p != nullptr;
+            if constexpr (SPLIT) {
+                const uint8_t *q = a_lo.at(pal_[i], ka);
+                ra[SL][1][i] = fetch16<ZN>(q, a_lo.base);
+                oka[SL][1][i] = ZN || q != nullptr;
+            }
         }
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
-            rb[0][i] = fetch16(b_hi.at(pb_[i], kbp));
-            if constexpr (SPLIT) rb[1][i] = fetch16(b_lo.at(pbl_[i], kbp));
+            const uint8_t *p = b_hi.at(pb_[i], kbp);
+            rb[SL][0][i] = fetch16<ZN>(p, b_hi.base);
+            okb[SL][0][i] = ZN || p != nullptr;
+            if constexpr (SPLIT) {
+                const uint8_t *q = b_lo.at(pbl_[i], kbp);
+                rb[SL][1][i] = fetch16<ZN>(q, b_lo.base);
+                okb[SL][1][i] = ZN || q != nullptr;
+            }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, auto slot_tag) {
+        constexpr int SL = decltype(slot_tag)::value;
         uint8_t *base = smem + buf * STAGE;
 #pragma unroll
         for (int h = 0; h < (SPLIT ? 2 : 1); ++h) {
 #pragma unroll
             for (int i = 0; i < RA; ++i)
-                *(i32x4 *)(base + h * OFF_LO + (lrow + 64 * i) * LDSROW + lchunk) = ra[h][i];
+                *(i32x4 *)(base + h * OFF_LO + (lrow + 64 * i) * LDSROW + lchunk) =
+                    oka[SL][h][i] ? ra[SL][h][i] : i32x4{0, 0, 0, 0};
 #pragma unroll
             for (int i = 0; i < RB; ++i)
-                *(i32x4 *)(base + h * OFF_LO + OFF_B + (lrow + 64 * i) * LDSROW + lchunk) = rb[h][i];
+                *(i32x4 *)(base + h * OFF_LO + OFF_B + (lrow + 64 * i) * LDSROW + lchunk) =
+                    okb[SL][h][i] ? rb[SL][h][i] : i32x4{0, 0, 0, 0};
         }
     };
 
@@ -250,15 +311,8 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
 
     constexpr int UNIT = 127;            // E8M0 exponent of 1.0
     constexpr int LO = 127 - 4;          // 2^-4: weight of the fp8 residual terms
-    if (nstages > 0) {
-        gload(0);
-        sstore(0);
-    }
-    __syncthreads();
     const int lr = lane & 31, lk = (lane >> 5) * OP::LANEB;
-    for (int st = 0; st < nstages; ++st) {
-        const int buf = st & 1;
-        if (st + 1 < nstages) gload(st + 1);
+    auto compute = [&](int buf) {
         const uint8_t *base = smem + buf * STAGE;
         const uint8_t *pa = base + (wm * TM * 32 + lr) * LDSROW + lk;
         const uint8_t *pb = base + OFF_B + (wn * TN * 32 + lr) * LDSROW + lk;
@@ -279,24 +333,67 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
                     }
                 }
         }
-        if (st + 1 < nstages) {
-            __syncthreads();  // everyone is done reading buffer buf^1 (stage st-1)
-            sstore(buf ^ 1);
-        }
+    };
+    // Prologue: stages 0 .. PF-1 in flight (every slot filled, repeating the
+    // last stage when there are fewer), stage 0 in LDS buffer 0, stage PF
+    // into the freed slot 0.
+    if (nstages > 0)
+        static_for<PF>([&](auto d) { gload(min((int)decltype(d)::value, nstages - 1), d); });
+    if (nstages > 0) {
+        sstore(0, std::integral_constant<int, 0>{});
+        if (PF > 1 || PF < nstages) gload(min(PF, nstages - 1), std::integral_constant<int, 0>{});
+    }
+    __syncthreads();
+    // Step st: store stage st+1 (slot (st+1) % PF) into the other buffer, which
+    // every wave finished reading before the previous barrier; refill the slot
+    // with stage st+1+PF; multiply stage st; one barrier.  Whole groups of PF
+    // steps run without branches around the loads (a refill past the last
+    // stage repeats the last stage, a store past it fills a buffer nobody
+    // reads), so the wait counters keep PF stages in flight; the last,
+    // partial group checks each step.
+    auto step = [&](int st, auto slot_tag) {
+        const int buf = st & 1;
+        sstore(buf ^ 1, slot_tag);
+        gload(min(st + 1 + PF, nstages - 1), slot_tag);
+        compute(buf);
         __syncthreads();
+    };
+    if constexpr (PF == 1) {
+        // one stage in flight: a plain loop (the 256 x 256 tile's registers
+        // are full, and the grouped form below makes it spill)
+        for (int st = 0; st < nstages; ++st) {
+            const int buf = st & 1;
+            if (st + 1 < nstages) {
+                sstore(buf ^ 1, std::integral_constant<int, 0>{});
+                if (st + 2 < nstages) gload(st + 2, std::integral_constant<int, 0>{});
+            }
+            compute(buf);
+            __syncthreads();
+        }
+    } else {
+        int st0 = 0;
+        for (; st0 + PF <= nstages; st0 += PF)
+            static_for<PF>([&](auto d) {
+                step(st0 + decltype(d)::value, std::integral_constant<int, (decltype(d)::value + 1) % PF>{});
+            });
+        static_for<PF>([&](auto d) {
+            if (st0 + decltype(d)::value < nstages)
+                step(st0 + decltype(d)::value, std::integral_constant<int, (decltype(d)::value + 1) % PF>{});
+        });
     }
 
     const float alpha = (qa ? 1.0f / *qa : 1.0f) * (qb ? 1.0f / *qb : 1.0f);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int j = 0; j < TN; ++j) {
+        const auto cj = dst.col(n0 + wn * TN * 32 + j * 32 + lr);  // one column per lane and block
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int m = m0 + wm * TM * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int n = n0 + wn * TN * 32 + j * 32 + lr;
-                dst.put(m, n, acc[i][j][r] * alpha);
+                dst.put(m, cj, acc[i][j][r] * alpha);
             }
+    }
 }
 
 template <class TL, int MODE, class ASrc, class BSrc, class Dst>
@@ -522,8 +619,11 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
     if ((mode == GM_FP8 || mode == GM_FP8X3) && (!qi || !qw))
         return set_error(UMI_ERR_INVALID, "umi_conv_fprop: fp8 modes need the quantisation scales");
     const DenseSrc ah{(const uint8_t *)w_hi, (long)KG * e, K}, al{(const uint8_t *)w_lo, (long)KG * e, K};
-    const ConvSrc bh{(const uint8_t *)in_hi, N, H, W, C, OH, OW, R, S, stride, pad, 1};
-    const ConvSrc bl{(const uint8_t *)in_lo, N, H, W, C, OH, OW, R, S, stride, pad, 1};
+    // the stage prep divides k < R*S*C by C and rs < R*S by S by multiply-high (exact while x * d < 2^32)
+    if ((uint64_t)KG * (uint64_t)C >= (1ull << 32))
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_conv_fprop: R*S*C*C = %llu >= 2^32", (unsigned long long)KG * C);
+    const ConvSrc bh{(const uint8_t *)in_hi, N, H, W, C, OH, OW, R, S, stride, pad, 1, recip32(C), recip32(S)};
+    const ConvSrc bl{(const uint8_t *)in_lo, N, H, W, C, OH, OW, R, S, stride, pad, 1, recip32(C), recip32(S)};
     const ConvDst d{out, K, OH * OW, N * OH * OW};
     const int NP = N * OH * OW;
     hipStream_t st = as_stream(s);
