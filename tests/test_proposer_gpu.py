@@ -280,3 +280,16 @@ def test_trained_proposer_quality_and_gain(gpu):
     print(f"[proposer] with network {with_net}\n[proposer] without {without}", flush=True)
     assert with_net["at_min_sad"] >= 0.99
     assert without["at_min_sad"] < with_net["at_min_sad"]
+
+
+def test_concurrent_graph_pipelines(gpu):
+    """Two held-out pairs in flight on two streams, each pipeline captured in a
+    graph: the first replay reproduces the eager pipeline bit for bit (checked
+    inside `throughput`) and both pairs stay >= 99 % at the minimum SAD."""
+    if not proposer.CHECKPOINT.exists():
+        pytest.skip("no trained checkpoint committed yet")
+    net = proposer.Network.load(proposer.CHECKPOINT)
+    r = proposer.throughput(net, 2, 2_000_000, reps=2)
+    print(f"[proposer] two streams {r}", flush=True)
+    assert r["min_at_min_sad"] >= 0.99
+    assert r["us_per_pair_pipeline"] > 0 and r["us_per_pair_full_search"] > 0

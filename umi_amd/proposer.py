@@ -366,6 +366,68 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
     return out
 
 
+def throughput(net: "Network", pairs: int, seed0: int, reps: int = 30, R: int = 32) -> dict:
+    """Blocks per second with `pairs` independent pairs in flight at once (as
+    consecutive pairs of a video would be), one stream and one graph-captured
+    pipeline per pair, against the plain full search given the same streams
+    and graphs.  The first replay of every pipeline graph must reproduce its
+    eager result bit for bit.  Host-timed: `reps` rounds of one launch per
+    stream between two device synchronisations."""
+    import time
+    from .device import Graph, synchronize
+
+    H, W = 1080, 1920
+    pm = PairMaker(H, W)
+    refs = [DeviceBuffer(H * W) for _ in range(pairs)]
+    curs = [DeviceBuffer(H * W) for _ in range(pairs)]
+    streams = [Stream() for _ in range(pairs)]
+    props = [Proposer(net, H, W, R=R) for _ in range(pairs)]
+    nblk = props[0].nblk
+    mvs = [DeviceBuffer(nblk * 12) for _ in range(pairs)]
+    judges = [DeviceBuffer(nblk * 12) for _ in range(pairs)]
+    at_min, gp, gf = [], [], []
+    for i in range(pairs):
+        pm.make(seed0 + i, refs[i], curs[i], streams[i])
+        ops.me_full_search(curs[i], refs[i], H, W, W, 8, R, judges[i], stream=streams[i])
+        props[i].run(curs[i], refs[i], mvs[i], streams[i])
+        streams[i].synchronize()
+        eager = mvs[i].download((nblk, 3), np.int32)
+        at_min.append(float(np.mean(eager[:, 2] == judges[i].download((nblk, 3), np.int32)[:, 2])))
+        g = Graph(streams[i])
+        g.begin()
+        props[i].run(curs[i], refs[i], mvs[i], streams[i])
+        g.end()
+        mvs[i].fill(0)
+        g.launch()
+        streams[i].synchronize()
+        if not np.array_equal(mvs[i].download((nblk, 3), np.int32), eager):
+            raise RuntimeError("proposer: graph replay differs from the eager pipeline")
+        gp.append(g)
+        g = Graph(streams[i])
+        g.begin()
+        ops.me_full_search(curs[i], refs[i], H, W, W, 8, R, judges[i], stream=streams[i])
+        g.end()
+        gf.append(g)
+
+    def timed(graphs):
+        for _ in range(3):
+            for g in graphs:
+                g.launch()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for g in graphs:
+                g.launch()
+        synchronize()
+        return (time.perf_counter() - t0) / (reps * len(graphs))
+
+    tp, tf = timed(gp), timed(gf)
+    return {"streams": pairs, "first_seed": seed0, "at_min_sad": float(np.mean(at_min)),
+            "min_at_min_sad": float(np.min(at_min)), "us_per_pair_pipeline": tp * 1e6,
+            "us_per_pair_full_search": tf * 1e6, "blocks_per_s_pipeline": nblk / tp,
+            "blocks_per_s_full_search": nblk / tf, "speedup": tf / tp}
+
+
 def main(argv=None):
     """python -m umi_amd.proposer train --pairs 2000 --epochs 12 --out ckpt.bin
        python -m umi_amd.proposer eval --ckpt ckpt.bin --pairs 4"""
