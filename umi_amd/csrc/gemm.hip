@@ -53,24 +53,34 @@ typedef __attribute__((ext_vector_type(4))) int i32x4;
 
 enum GemmMode { GM_F32 = 0, GM_BF16 = 1, GM_BF16X3 = 2, GM_FP8 = 3, GM_FP8X3 = 4 };
 
-constexpr int ROWB = 64;          // bytes of one operand row per stage
-constexpr int LDSROW = ROWB + 16; // padded LDS row stride
+constexpr int ROWB = 64;    // bytes of one operand row per stage
+constexpr int LDSROW = ROWB; // LDS row stride; the 16-byte chunks are swizzled instead of padded
 constexpr int NTHREADS = 256;
+
+// LDS position of byte `off` of a stage row whose swizzle is sw = (row >> 2) & 3:
+// 16-byte chunk c lives at chunk c ^ sw.  Sixteen lanes that write 4 rows x
+// 4 chunks, or read one chunk of 16 consecutive rows, then touch 16 distinct
+// groups of 4 banks.
+__device__ inline int swz(int off, int sw) { return (((off >> 4) ^ sw) << 4) | (off & 15); }
 
 template <int MODE>
 struct Mfma;
 template <>
 struct Mfma<GM_F32> {
     static constexpr int ESZ = 4, LANEB = 4, CH = 8, NOP = 2;
-    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, f32x16 c, int, int) {
-        return __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float *)a, *(const float *)b, c, 0, 0, 0);
+    // a, b: the lane's LDS rows; off: byte offset in the row; sw: the rows' swizzle
+    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int, int) {
+        const int o = swz(off, sw);
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float *)(a + o), *(const float *)(b + o), c, 0, 0, 0);
     }
 };
 template <>
 struct Mfma<GM_BF16> {
     static constexpr int ESZ = 2, LANEB = 16, CH = 2, NOP = 2;
-    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, f32x16 c, int, int) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8 *)a, *(const bf16x8 *)b, c, 0, 0, 0);
+    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int, int) {
+        const int o = swz(off, sw);
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8 *)(a + o), *(const bf16x8 *)(b + o), c, 0, 0,
+                                                       0);
     }
 };
 template <>
@@ -80,9 +90,11 @@ struct Mfma<GM_BF16X3> : Mfma<GM_BF16> {
 template <>
 struct Mfma<GM_FP8> {
     static constexpr int ESZ = 1, LANEB = 32, CH = 1, NOP = 2;
-    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, f32x16 c, int sa, int sb) {
-        const i32x4 *a4 = (const i32x4 *)a, *b4 = (const i32x4 *)b;
-        const i32x4 a0 = a4[0], a1 = a4[1], b0 = b4[0], b1 = b4[1];
+    // 32 bytes = two chunks, which the swizzle may place in either order
+    static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int sa, int sb) {
+        const int o0 = swz(off, sw), o1 = swz(off + 16, sw);
+        const i32x4 a0 = *(const i32x4 *)(a + o0), a1 = *(const i32x4 *)(a + o1);
+        const i32x4 b0 = *(const i32x4 *)(b + o0), b1 = *(const i32x4 *)(b + o1);
         const i32x8 av = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
         const i32x8 bv = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
         return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
@@ -285,6 +297,8 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
             }
         }
     };
+    // the thread's chunk, swizzled by its rows (lrow + 64 i share (row >> 2) & 3)
+    const int schunk = swz(lchunk, (lrow >> 2) & 3);
     auto sstore = [&](int buf, auto slot_tag) {
         constexpr int SL = decltype(slot_tag)::value;
         uint8_t *base = smem + buf * STAGE;
@@ -292,11 +306,11 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
         for (int h = 0; h < (SPLIT ? 2 : 1); ++h) {
 #pragma unroll
             for (int i = 0; i < RA; ++i)
-                *(i32x4 *)(base + h * OFF_LO + (lrow + 64 * i) * LDSROW + lchunk) =
+                *(i32x4 *)(base + h * OFF_LO + (lrow + 64 * i) * LDSROW + schunk) =
                     oka[SL][h][i] ? ra[SL][h][i] : i32x4{0, 0, 0, 0};
 #pragma unroll
             for (int i = 0; i < RB; ++i)
-                *(i32x4 *)(base + h * OFF_LO + OFF_B + (lrow + 64 * i) * LDSROW + lchunk) =
+                *(i32x4 *)(base + h * OFF_LO + OFF_B + (lrow + 64 * i) * LDSROW + schunk) =
                     okb[SL][h][i] ? rb[SL][h][i] : i32x4{0, 0, 0, 0};
         }
     };
@@ -312,24 +326,25 @@ __global__ __launch_bounds__(NTHREADS) void gemm_kernel(ASrc a_hi, ASrc a_lo, BS
     constexpr int UNIT = 127;            // E8M0 exponent of 1.0
     constexpr int LO = 127 - 4;          // 2^-4: weight of the fp8 residual terms
     const int lr = lane & 31, lk = (lane >> 5) * OP::LANEB;
+    const int rsw = (lr >> 2) & 3;  // swizzle of the lane's rows (tile rows are multiples of 32 apart)
     auto compute = [&](int buf) {
         const uint8_t *base = smem + buf * STAGE;
-        const uint8_t *pa = base + (wm * TM * 32 + lr) * LDSROW + lk;
-        const uint8_t *pb = base + OFF_B + (wn * TN * 32 + lr) * LDSROW + lk;
+        const uint8_t *pa = base + (wm * TM * 32 + lr) * LDSROW;
+        const uint8_t *pb = base + OFF_B + (wn * TN * 32 + lr) * LDSROW;
 #pragma unroll
         for (int kc = 0; kc < OP::CH; ++kc) {
-            const int ko = kc * 2 * OP::LANEB;
+            const int off = kc * 2 * OP::LANEB + lk;
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j) {
-                    const uint8_t *ai = pa + i * 32 * LDSROW + ko;
-                    const uint8_t *bj = pb + j * 32 * LDSROW + ko;
-                    acc[i][j] = OP::run(ai, bj, acc[i][j], UNIT, UNIT);
+                    const uint8_t *ai = pa + i * 32 * LDSROW;
+                    const uint8_t *bj = pb + j * 32 * LDSROW;
+                    acc[i][j] = OP::run(ai, bj, off, rsw, acc[i][j], UNIT, UNIT);
                     if constexpr (SPLIT) {
                         constexpr int S1 = (MODE == GM_FP8X3) ? LO : UNIT;
-                        acc[i][j] = OP::run(ai + OFF_LO, bj, acc[i][j], S1, UNIT);  // A lo * B hi
-                        acc[i][j] = OP::run(ai, bj + OFF_LO, acc[i][j], UNIT, S1);  // A hi * B lo
+                        acc[i][j] = OP::run(ai + OFF_LO, bj, off, rsw, acc[i][j], S1, UNIT);  // A lo * B hi
+                        acc[i][j] = OP::run(ai, bj + OFF_LO, off, rsw, acc[i][j], UNIT, S1);  // A hi * B lo
                     }
                 }
         }
