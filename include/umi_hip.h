@@ -58,6 +58,12 @@ umi_status umi_copy_d2d(void *dst, const void *src, size_t bytes, umi_stream s);
 umi_status umi_stream_create(umi_stream *s);
 umi_status umi_stream_destroy(umi_stream s);
 umi_status umi_stream_synchronize(umi_stream s);
+/* Work enqueued on `waiter` after this call starts only when the work
+ * enqueued on `signaller` so far has finished (an event recorded on one
+ * stream and waited on by the other: fork and join of streams, also under
+ * graph capture).  No reference counterpart: the reference uses the legacy
+ * default stream only (src/UMI_gpu.h:610). */
+umi_status umi_stream_wait(umi_stream waiter, umi_stream signaller);
 /* Launch a kernel compiled in the caller's translation unit (the generic
  * Execute<> engine instantiates its kernel from the user's strategy struct;
  * src/UMI_gpu.h:610 is the launch it replaces). */

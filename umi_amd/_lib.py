@@ -31,6 +31,7 @@ SIGNATURES = {
     "umi_stream_create": [C.POINTER(c_void_p)],
     "umi_stream_destroy": [P],
     "umi_stream_synchronize": [P],
+    "umi_stream_wait": [P, P],
     "umi_launch_kernel": [P, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint, c_size_t,
                           C.POINTER(c_void_p), P],
     "umi_timer_create": [C.POINTER(c_void_p)],

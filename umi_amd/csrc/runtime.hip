@@ -140,6 +140,15 @@ umi_status umi_stream_synchronize(umi_stream s) {
     return hip_check(hipStreamSynchronize(as_stream(s)), "hipStreamSynchronize");
 }
 
+umi_status umi_stream_wait(umi_stream waiter, umi_stream signaller) {
+    hipEvent_t e;
+    UMI_TRY(hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"));
+    umi_status rc = hip_check(hipEventRecord(e, as_stream(signaller)), "hipEventRecord");
+    if (rc == UMI_OK) rc = hip_check(hipStreamWaitEvent(as_stream(waiter), e, 0), "hipStreamWaitEvent");
+    const umi_status rd = hip_check(hipEventDestroy(e), "hipEventDestroy");  // the enqueued wait keeps its event
+    return rc != UMI_OK ? rc : rd;
+}
+
 umi_status umi_launch_kernel(const void *func, unsigned gx, unsigned gy, unsigned gz, unsigned bx,
                              unsigned by, unsigned bz, size_t shmem, void **args, umi_stream s) {
     if (!func) return set_error(UMI_ERR_INVALID, "umi_launch_kernel: null kernel");

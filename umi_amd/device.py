@@ -77,6 +77,10 @@ class Stream:
         call("umi_stream_create", C.byref(p))
         self.handle = p.value
 
+    def wait(self, other: "Stream") -> None:
+        """Work enqueued here from now on waits for the work enqueued on `other` so far."""
+        call("umi_stream_wait", self.handle, other.handle)
+
     def synchronize(self) -> None:
         call("umi_stream_synchronize", self.handle)
 
