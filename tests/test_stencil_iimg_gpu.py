@@ -61,7 +61,7 @@ def test_wfilter_1024_config(gpu):
                                          (5, 9, 0, 1.0, 0.1), (129, 67, 5, 3.0, 0.2)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_bilateral_vs_oracle(gpu, H, W, r, ss, sr, mode):
-    # tolerance: device expf vs host expf differ by an ulp or two -> 2e-6 relative
+    # tolerance: the device weight 2^(d^2 norm log2 e) on v_exp_f32 vs host expf differ by a few ulp -> 2e-6 relative
     g = workloads.rng(H + W + r)
     img = g.random((H, W), dtype=np.float32)
     print(f"[bilateral] {H}x{W} r={r} mode={mode}", flush=True)

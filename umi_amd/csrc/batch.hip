@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void batch_wfilter_kernel(const BatchImage *__
     }
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int RT>
 __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *__restrict__ imgs, int n,
                                                               const int *__restrict__ prefix,
                                                               const float *__restrict__ in, int r,
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *
         const BatchImage im = imgs[i];
         const int local = t - prefix[i], tiles_x = ceil_div(im.W, TX);
         const int ty = local / tiles_x, tx = local - ty * tiles_x;
-        bilateral_tile<TX, TY>(bl_lds + 32, wsh, in + im.in_off, im.H, im.W, im.in_pitch, r, norm, pad, mode,
+        bilateral_tile<TX, TY, RT>(bl_lds + 32, wsh, in + im.in_off, im.H, im.W, im.in_pitch, r, norm, pad, mode,
                                out + im.out_off, im.out_pitch, tx * TX, ty * TY);
     }
 }
@@ -187,8 +187,12 @@ template <int C>
 umi_status launch_bilateral(umi_batch b, const float *in, int r, const float *ws, float norm, float pad, int mode,
                             float *out, hipStream_t st) {
     const size_t lds = sizeof(float) * ((size_t)(BTX[C] + 2 * r) * (BTY[C] + 2 * r) + 32);
-    hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C]>), dim3(grid_for(b->btotal[C])), dim3(256), lds, st,
-                       b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
+    if (r == 2)  // the 5 x 5 window: unrolled
+        hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C], 2>), dim3(grid_for(b->btotal[C])), dim3(256), lds,
+                           st, b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
+    else
+        hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C], 0>), dim3(grid_for(b->btotal[C])), dim3(256), lds,
+                           st, b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
     return launch_check("batch_bilateral_kernel");
 }
 

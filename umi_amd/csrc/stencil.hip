@@ -35,14 +35,15 @@ __global__ __launch_bounds__(256) void filter3x3_kernel(const T *__restrict__ in
                                             blockIdx.y * F_TY);
 }
 
+template <int RT>
 __global__ __launch_bounds__(256) void bilateral_kernel(const float *__restrict__ in, int H, int W, int ip, int r,
                                                         const float *__restrict__ ws, float norm, float pad,
                                                         int mode, float *__restrict__ out, int op) {
     extern __shared__ float bl_lds[];
     float *wsh = bl_lds;
     if ((int)threadIdx.x < 2 * r + 1) wsh[threadIdx.x] = ws[threadIdx.x];
-    bilateral_tile<BL_TX, BL_TY>(bl_lds + 32, wsh, in, H, W, ip, r, norm, pad, mode, out, op, blockIdx.x * BL_TX,
-                                 blockIdx.y * BL_TY);
+    bilateral_tile<BL_TX, BL_TY, RT>(bl_lds + 32, wsh, in, H, W, ip, r, norm, pad, mode, out, op, blockIdx.x * BL_TX,
+                                     blockIdx.y * BL_TY);
 }
 
 template <typename T, bool WEIGHTED>
@@ -98,8 +99,12 @@ umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int ra
     if (H == 0 || W == 0) return UMI_OK;
     const size_t lds = sizeof(float) * ((size_t)(BL_TX + 2 * radius) * (BL_TY + 2 * radius) + 32);
     dim3 grid(ceil_div(W, BL_TX), ceil_div(H, BL_TY));
-    hipLaunchKernelGGL(bilateral_kernel, grid, dim3(256), lds, as_stream(s), in, H, W, in_pitch, radius, spatial,
-                       norm, pad, mode, out, out_pitch);
+    if (radius == 2)  // the 5 x 5 window: unrolled
+        hipLaunchKernelGGL(bilateral_kernel<2>, grid, dim3(256), lds, as_stream(s), in, H, W, in_pitch, radius,
+                           spatial, norm, pad, mode, out, out_pitch);
+    else
+        hipLaunchKernelGGL(bilateral_kernel<0>, grid, dim3(256), lds, as_stream(s), in, H, W, in_pitch, radius,
+                           spatial, norm, pad, mode, out, out_pitch);
     return launch_check("umi_bilateral_f32");
 }
 

@@ -69,12 +69,15 @@ __device__ inline void filter3x3_tile(T *tile, const T *__restrict__ in, int H, 
 
 // One TX x TY tile of the bilateral filter (one output per thread, 256
 // threads): halo tile of (TY+2r) x (TX+2r) in `tile`, spatial weights in
-// `wsh` (2r+1 values, already staged).  Ends with a barrier.
-template <int TX, int TY>
+// `wsh` (2r+1 values, already staged).  RT > 0 is the radius known at compile
+// time (r == RT): the window loops unroll and the spatial weights stay in
+// registers.  Ends with a barrier.
+template <int TX, int TY, int RT = 0>
 __device__ inline void bilateral_tile(float *tile, const float *wsh, const float *__restrict__ in, int H, int W, int ip,
-                                      int r, float norm, float pad, int mode, float *__restrict__ out, int op, int x0,
+                                      int r_, float norm, float pad, int mode, float *__restrict__ out, int op, int x0,
                                       int y0) {
     static_assert(TX * TY == 256, "one output per thread");
+    const int r = RT > 0 ? RT : r_;
     const int LW = TX + 2 * r, LH = TY + 2 * r;
     const int tid = threadIdx.x;
     for (int e = tid; e < LH * LW; e += 256) {
@@ -84,18 +87,33 @@ __device__ inline void bilateral_tile(float *tile, const float *wsh, const float
     __syncthreads();
     const int tx = tid % TX, ty = tid / TX;
     const int x = x0 + tx, y = y0 + ty;
+    // exp(v) = 2^(v log2 e) on v_exp_f32 (about 1 ulp): the range weight
+    // without the library exp's range reduction and special cases
+    const float norm2 = norm * 1.44269504088896340736f;
     if (x < W && y < H) {
         const float center = tile[(ty + r) * LW + tx + r];
         float wsum = 0.f, wxsum = 0.f;
-        for (int dy = 0; dy <= 2 * r; ++dy) {
-            const float wy = wsh[dy];
-            const float *row = tile + (ty + dy) * LW + tx;
-            for (int dx = 0; dx <= 2 * r; ++dx) {
-                const float n = row[dx];
-                const float d = n - center;
-                const float wgt = expf(d * d * norm) * wy * wsh[dx];
-                wsum += wgt;
-                wxsum = __builtin_fmaf(wgt, n, wxsum);
+        auto tap = [&](float n, float wy, float wx) {
+            const float d = n - center;
+            const float wgt = __builtin_amdgcn_exp2f(d * d * norm2) * wy * wx;
+            wsum += wgt;
+            wxsum = __builtin_fmaf(wgt, n, wxsum);
+        };
+        if constexpr (RT > 0) {
+            float wv[2 * RT + 1];
+#pragma unroll
+            for (int k = 0; k <= 2 * RT; ++k) wv[k] = wsh[k];
+#pragma unroll
+            for (int dy = 0; dy <= 2 * RT; ++dy) {
+                const float *row = tile + (ty + dy) * LW + tx;
+#pragma unroll
+                for (int dx = 0; dx <= 2 * RT; ++dx) tap(row[dx], wv[dy], wv[dx]);
+            }
+        } else {
+            for (int dy = 0; dy <= 2 * r; ++dy) {
+                const float wy = wsh[dy];
+                const float *row = tile + (ty + dy) * LW + tx;
+                for (int dx = 0; dx <= 2 * r; ++dx) tap(row[dx], wy, wsh[dx]);
             }
         }
         out[(size_t)y * op + x] = wxsum / wsum;
