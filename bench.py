@@ -7,7 +7,7 @@ int32 SAD, MV + SAD per block (32,400 blocks) — the plain full-search ME
 configuration.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line (metric, value, roofline, cpu_baseline, ...).
-`--gpus N` with N > 1 exits: this build runs on a single GPU.
+`--gpus N` with N > 1 exits: the library and this benchmark use a single GPU.
 """
 from __future__ import annotations
 
