@@ -22,11 +22,31 @@ struct Arith<int32_t> {  // two's-complement wrap, as the reference's int32 exam
     static __device__ int32_t mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
 };
 
-template <typename T>
-__device__ inline T load_bounded(const T *in, int H, int W, int ip, int gy, int gx, int mode, T pad) {
-    if (gy >= 0 && gy < H && gx >= 0 && gx < W) return in[(size_t)gy * ip + gx];
-    if (mode == 0) return pad;
-    return in[(size_t)clampi(gy, 0, H - 1) * ip + clampi(gx, 0, W - 1)];
+// Halo tile staging: tile[e] = the image at (gy0 + e / LW, gx0 + e % LW) for
+// e < n, out-of-image positions by `mode` (0: pad, 1: clamp to the edge).
+// Every load is unconditional (clamped indices) and a thread's U loads are
+// all issued before its first LDS store, so a tile costs one load latency
+// rather than one per element; indices past n repeat element n - 1 (same
+// value, same slot).  All 256 threads call it; the caller synchronises.
+template <int U, typename T>
+__device__ inline void stage_halo(T *tile, int n, int LW, const T *__restrict__ in, int H, int W, int ip, int gy0,
+                                  int gx0, int mode, T pad) {
+    const int tid = threadIdx.x;
+    for (int base = 0; base < n; base += U * 256) {
+        T v[U];
+        int idx[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            idx[u] = min(base + u * 256 + tid, n - 1);
+            const int r = idx[u] / LW, c = idx[u] - r * LW;
+            const int gy = gy0 + r, gx = gx0 + c;
+            ok[u] = mode != 0 || (gy >= 0 && gy < H && gx >= 0 && gx < W);
+            v[u] = in[(size_t)clampi(gy, 0, H - 1) * ip + clampi(gx, 0, W - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) tile[idx[u]] = ok[u] ? v[u] : pad;
+    }
 }
 
 // One TX x TY output tile of a 3x3 filter by 256 threads: stage the
@@ -39,10 +59,7 @@ __device__ inline void filter3x3_tile(T *tile, const T *__restrict__ in, int H, 
     constexpr int LW = TX + 2, LH = TY + 2, ROWS = TX * TY / 256;
     static_assert(TX * TY % 256 == 0 && 256 % TX == 0, "tile shape");
     const int tid = threadIdx.x;
-    for (int e = tid; e < LH * LW; e += 256) {
-        const int r = e / LW, c = e - r * LW;
-        tile[e] = load_bounded(in, H, W, ip, y0 - 1 + r, x0 - 1 + c, mode, pad);
-    }
+    stage_halo<(LH * LW + 255) / 256>(tile, LH * LW, LW, in, H, W, ip, y0 - 1, x0 - 1, mode, pad);
     __syncthreads();
     const int tx = tid % TX, ty = tid / TX;
     const int x = x0 + tx;
@@ -80,10 +97,7 @@ __device__ inline void bilateral_tile(float *tile, const float *wsh, const float
     const int r = RT > 0 ? RT : r_;
     const int LW = TX + 2 * r, LH = TY + 2 * r;
     const int tid = threadIdx.x;
-    for (int e = tid; e < LH * LW; e += 256) {
-        const int rr = e / LW, c = e - rr * LW;
-        tile[e] = load_bounded(in, H, W, ip, y0 - r + rr, x0 - r + c, mode, pad);
-    }
+    stage_halo<4>(tile, LH * LW, LW, in, H, W, ip, y0 - r, x0 - r, mode, pad);
     __syncthreads();
     const int tx = tid % TX, ty = tid / TX;
     const int x = x0 + tx, y = y0 + ty;
