@@ -81,14 +81,16 @@ def learned_proposer(st):
     pairs, seed0 = 3, 2_000_000
     with_net = proposer.evaluate(prop, pairs, seed0, proposals=True, reps=20)
     without = proposer.evaluate(prop, pairs, seed0, proposals=False, reps=20)
-    # four pairs in flight on four streams (graph-captured), both methods alike
-    concurrent = proposer.throughput(net, 4, seed0, reps=30)
+    # four and six pairs in flight, one stream each (graph-captured), both methods alike
+    concurrent = proposer.throughput(net, 4, seed0, reps=20, gate=150)
+    concurrent6 = proposer.throughput(net, 6, seed0, reps=20, gate=150)
     return {"pairs": pairs, "first_seed": seed0, "search": "+-32, 8x8 blocks, 32400 blocks per pair",
             "settings": {"iters": prop.iters, "local_rounds": prop.local_rounds, "rounds": prop.rounds,
                          "slack": prop.slack},
             "blocks_per_s_pipeline": prop.nblk / (with_net["ms_pipeline"] * 1e-3),
             "blocks_per_s_full_search": prop.nblk / (with_net["ms_full_search"] * 1e-3),
-            "with_network": with_net, "without_network": without, "four_streams": concurrent}
+            "with_network": with_net, "without_network": without, "four_streams": concurrent,
+            "six_streams": concurrent6}
 
 
 def extra_configs(st):
