@@ -121,6 +121,13 @@ umi_status umi_graph_destroy(umi_graph g);
  * ------------------------------------------------------------------------- */
 umi_status umi_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
                                  int B, int R, int32_t *mv_sad, int32_t *volume, umi_stream s);
+/* Kernel form of the full search for the following calls (process-wide), for
+ * measurement and tests; every form gives identical results.  bit 0: 1 =
+ * v_qsad_pk_u16_u8 (default), 0 = v_sad_u8; bit 1: four-column groups only
+ * (no light lane for the last column); bit 2: dword staging even for
+ * 16-byte aligned frames; bit 3: runtime skips in the last row chunk.
+ * 0 <= variant <= 15.  No reference counterpart. */
+umi_status umi_me_set_variant(int variant);
 
 /* ---------------------------------------------------------------------------
  * SAD cost volume, uint8 pair.  Replaces the MotionEst launch of
