@@ -217,6 +217,7 @@ def _refine_gpu(cur, ref, prop, R, iters, local_rounds, rounds, slack, pitch=Non
     dict(H=64, W=64, R=8, iters=1, local=0, rounds=0, slack=1.1, kind="smooth"),       # no rounds, one step
     dict(H=64, W=72, R=8, iters=4, local=2, rounds=3, slack=0.0, kind="smooth"),       # slack 0 forces fallbacks
     dict(H=200, W=264, R=64, iters=8, local=8, rounds=4, slack=1.1, kind="smooth"),    # largest radius
+    dict(H=200, W=264, R=32, iters=4, local=4, rounds=4, slack=1.1, kind="tiles"),     # the Proposer defaults
 ])
 def test_refine_matches_oracle(gpu, case):
     H, W, R = case["H"], case["W"], case["R"]

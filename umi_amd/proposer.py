@@ -263,7 +263,7 @@ def pack_layer(net: Network, li: int, stream=None) -> DeviceBuffer:
 class Proposer:
     """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
 
-    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=8, local_rounds=8, rounds=4):
+    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=4, local_rounds=4, rounds=4):
         self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
         self.iters, self.local_rounds, self.rounds = iters, local_rounds, rounds
         self.H4, self.W4 = H // 4, W // 4
@@ -472,7 +472,8 @@ def main(argv=None):
     ap.add_argument("--eval-seed0", type=int, default=1_000_000, help="first held-out pair seed")
     ap.add_argument("--ckpt", default=str(CHECKPOINT))
     ap.add_argument("--rounds", type=int, default=4, help="global propagation rounds of the refinement")
-    ap.add_argument("--local-rounds", type=int, default=8, help="in-tile propagation rounds of the refinement")
+    ap.add_argument("--local-rounds", type=int, default=4, help="in-tile propagation rounds of the refinement")
+    ap.add_argument("--iters", type=int, default=4, help="descent steps per block of the refinement")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
@@ -495,7 +496,7 @@ def main(argv=None):
         print(json.dumps({"loss_history": hist, "checkpoint": out}), flush=True)
     else:
         net = Network.load(a.ckpt)
-        prop = Proposer(net, local_rounds=a.local_rounds, rounds=a.rounds)
+        prop = Proposer(net, local_rounds=a.local_rounds, rounds=a.rounds, iters=a.iters)
         with_net = evaluate(prop, a.pairs, a.eval_seed0, proposals=True)
         without = evaluate(prop, a.pairs, a.eval_seed0, proposals=False)
         long = Proposer(net, rounds=32)  # the same refinement with a larger budget of global rounds
