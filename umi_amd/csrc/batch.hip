@@ -5,10 +5,12 @@
 // main/iimg.cu:37,60, johnjohnlin/UMI).  Here a batch object holds, per
 // image, its size and its offsets into one input and one output buffer, plus
 // prefix sums of the work items (tiles, bands, columns, rows) of every image
-// for every tile configuration, all resident in device memory.  A batched
-// operator is then one persistent kernel whose workgroups stride over the
-// concatenated work list, find their image by binary search over the
-// prefix array and run the same tile body as the one-image kernel.  No
+// for every tile configuration and an owner table giving the image of every
+// work item, all resident in device memory.  A batched operator is then one
+// persistent kernel whose workgroups stride over the concatenated work list,
+// read their image from the owner table (one load, where a binary search
+// over the prefix array took log2(n) dependent ones) and run the same tile
+// body as the one-image kernel.  No
 // launch allocates or copies, so the launches can be captured in a hipGraph.
 // Tile configurations are selectable so a tuner can pick the fastest.
 #include "common.h"
@@ -32,20 +34,10 @@ constexpr int BTX[NB] = {32, 16, 64, 128}, BTY[NB] = {8, 16, 4, 2};
 
 constexpr int PERSIST_WG = 256 * 8;  // workgroups of a persistent launch (8 per CU)
 
-// index i with prefix[i] <= t < prefix[i+1]
-__device__ inline int find_item(const int *__restrict__ prefix, int n, int t) {
-    int lo = 0, hi = n;  // prefix[lo] <= t < prefix[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 template <int TX, int TY>
 __global__ __launch_bounds__(256) void batch_wfilter_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                            const int *__restrict__ prefix, const float *__restrict__ in,
+                                                            const int *__restrict__ prefix,
+                                                            const int *__restrict__ owner, const float *__restrict__ in,
                                                             const float *__restrict__ wts, float init, float pad,
                                                             int mode, float *__restrict__ out) {
     __shared__ float tile[(TY + 2) * (TX + 2)];
@@ -54,7 +46,7 @@ __global__ __launch_bounds__(256) void batch_wfilter_kernel(const BatchImage *__
     for (int k = 0; k < 9; ++k) w[k] = wts[k];
     const int total = prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(prefix, n, t);
+        const int i = owner[t];
         const BatchImage im = imgs[i];
         const int local = t - prefix[i], tiles_x = ceil_div(im.W, TX);
         const int ty = local / tiles_x, tx = local - ty * tiles_x;
@@ -66,6 +58,7 @@ __global__ __launch_bounds__(256) void batch_wfilter_kernel(const BatchImage *__
 template <int TX, int TY, int RT>
 __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *__restrict__ imgs, int n,
                                                               const int *__restrict__ prefix,
+                                                              const int *__restrict__ owner,
                                                               const float *__restrict__ in, int r,
                                                               const float *__restrict__ ws, float norm, float pad,
                                                               int mode, float *__restrict__ out) {
@@ -75,7 +68,7 @@ __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *
     __syncthreads();
     const int total = prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(prefix, n, t);
+        const int i = owner[t];
         const BatchImage im = imgs[i];
         const int local = t - prefix[i], tiles_x = ceil_div(im.W, TX);
         const int ty = local / tiles_x, tx = local - ty * tiles_x;
@@ -90,6 +83,7 @@ __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *
 template <class T>
 __global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const BatchImage *__restrict__ imgs, int n,
                                                                        const int *__restrict__ tile_prefix,
+                                                                       const int *__restrict__ owner,
                                                                        const long long *__restrict__ ws_off,
                                                                        const typename T::In *__restrict__ in,
                                                                        typename T::Col *__restrict__ ws) {
@@ -97,7 +91,7 @@ __global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const Bat
     __shared__ typename T::Col tmp[4];
     const int total = tile_prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(tile_prefix, n, t);
+        const int i = owner[t];
         const BatchImage im = imgs[i];
         const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
         const typename T::In *src = in + im.in_off;
@@ -109,11 +103,12 @@ __global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const Bat
 template <class T>
 __global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const BatchImage *__restrict__ imgs, int n,
                                                                const int *__restrict__ item_prefix,
+                                                               const int *__restrict__ owner,
                                                                const long long *__restrict__ ws_off,
                                                                typename T::Col *__restrict__ ws) {
     const int total = item_prefix[n];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        const int i = find_item(item_prefix, n, t);
+        const int i = owner[t >> 6];  // every image's carry items are whole waves of 64
         const BatchImage im = imgs[i];
         iimg_carry_item<T>(im.H, im.W, t - item_prefix[i], iimg_ws<T>(ws + ws_off[i], im.H, im.W));
     }
@@ -122,6 +117,7 @@ __global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const BatchImage 
 template <class T>
 __global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const BatchImage *__restrict__ imgs, int n,
                                                                     const int *__restrict__ tile_prefix,
+                                                                    const int *__restrict__ owner,
                                                                     const long long *__restrict__ ws_off,
                                                                     const typename T::In *__restrict__ in,
                                                                     typename T::Col *__restrict__ ws,
@@ -130,7 +126,7 @@ __global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const BatchI
     __shared__ typename T::Col rl[IM_TH];
     const int total = tile_prefix[n];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = find_item(tile_prefix, n, t);
+        const int i = owner[t];
         const BatchImage im = imgs[i];
         const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
         const typename T::In *src = in + im.in_off;
@@ -147,8 +143,10 @@ struct umi_batch_s {
     BatchImage *imgs = nullptr;
     // device prefix arrays (n + 1 ints each) and their totals
     int *ftiles[NF] = {}, *btiles[NB] = {};
+    int *fown[NF] = {}, *bown[NB] = {};  // owner tables: the image of every tile
     int ftotal[NF] = {}, btotal[NB] = {};
     int *itiles = nullptr, *icarry = nullptr;  // integral image tiles and carry items
+    int *iown = nullptr, *cown = nullptr;      // image of every tile, of every 64 carry items
     int nitiles = 0, nicarry = 0;
     long long *ws_off = nullptr;
     long long ws_elems = 0;
@@ -179,7 +177,7 @@ template <int C>
 umi_status launch_wfilter(umi_batch b, const float *in, const float *w, float init, float pad, int mode, float *out,
                           hipStream_t st) {
     hipLaunchKernelGGL((batch_wfilter_kernel<FTX[C], FTY[C]>), dim3(grid_for(b->ftotal[C])), dim3(256), 0, st, b->imgs,
-                       b->n, b->ftiles[C], in, w, init, pad, mode, out);
+                       b->n, b->ftiles[C], b->fown[C], in, w, init, pad, mode, out);
     return launch_check("batch_wfilter_kernel");
 }
 
@@ -189,10 +187,10 @@ umi_status launch_bilateral(umi_batch b, const float *in, int r, const float *ws
     const size_t lds = sizeof(float) * ((size_t)(BTX[C] + 2 * r) * (BTY[C] + 2 * r) + 32);
     if (r == 2)  // the 5 x 5 window: unrolled
         hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C], 2>), dim3(grid_for(b->btotal[C])), dim3(256), lds,
-                           st, b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
+                           st, b->imgs, b->n, b->btiles[C], b->bown[C], in, r, ws, norm, pad, mode, out);
     else
         hipLaunchKernelGGL((batch_bilateral_kernel<BTX[C], BTY[C], 0>), dim3(grid_for(b->btotal[C])), dim3(256), lds,
-                           st, b->imgs, b->n, b->btiles[C], in, r, ws, norm, pad, mode, out);
+                           st, b->imgs, b->n, b->btiles[C], b->bown[C], in, r, ws, norm, pad, mode, out);
     return launch_check("batch_bilateral_kernel");
 }
 
@@ -204,13 +202,13 @@ umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *o
     if (need && (!ws || ws_bytes < need)) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
     auto *w = static_cast<typename T::Col *>(ws);
     hipLaunchKernelGGL(batch_iimg_totals_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs,
-                       b->n, b->itiles, b->ws_off, in, w);
+                       b->n, b->itiles, b->iown, b->ws_off, in, w);
     UMI_TRY(launch_check("batch_iimg_totals_kernel"));
     hipLaunchKernelGGL(batch_iimg_carry_kernel<T>, dim3(grid_for(ceil_div(b->nicarry, 256))), dim3(256), 0, st,
-                       b->imgs, b->n, b->icarry, b->ws_off, w);
+                       b->imgs, b->n, b->icarry, b->cown, b->ws_off, w);
     UMI_TRY(launch_check("batch_iimg_carry_kernel"));
     hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs, b->n,
-                       b->itiles, b->ws_off, in, w, out);
+                       b->itiles, b->iown, b->ws_off, in, w, out);
     UMI_TRY(launch_check("batch_iimg_out_kernel"));
     return UMI_OK;
 }
@@ -258,6 +256,17 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     }
     if (st == UMI_OK) st = upload(b, itiles, &b->itiles);
     if (st == UMI_OK) st = upload(b, icarry, &b->icarry);
+    // owner tables: item t of a list belongs to image owner[t]
+    auto owners = [&](const std::vector<int> &pre, int per) {
+        std::vector<int> o((size_t)pre[n] / per + 1, 0);
+        for (int i = 0; i < n; ++i)
+            for (int t = pre[i] / per; t < pre[i + 1] / per; ++t) o[t] = i;
+        return o;
+    };
+    for (int c = 0; c < NF && st == UMI_OK; ++c) st = upload(b, owners(fp[c], 1), &b->fown[c]);
+    for (int c = 0; c < NB && st == UMI_OK; ++c) st = upload(b, owners(bp[c], 1), &b->bown[c]);
+    if (st == UMI_OK) st = upload(b, owners(itiles, 1), &b->iown);
+    if (st == UMI_OK) st = upload(b, owners(icarry, 64), &b->cown);
     if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
     b->nitiles = itiles[n];
     b->nicarry = icarry[n];
