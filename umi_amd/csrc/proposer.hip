@@ -655,7 +655,7 @@ struct RefineArgs {
     int H, W, pitch, R, iters;
     int tiles_x, ntiles;  // tile grid
     int stride4, rows;    // LDS ref region: rows x stride4 dwords
-    int ref16, cur16;     // frame base and pitch are 16-byte aligned
+    int ref16, cur16;     // frame base and pitch are 16-byte aligned and W >= 16
 };
 
 // LDS geometry of a tile's ref region: rows 8 RT_Y + 2R; a row starts at the
@@ -685,50 +685,32 @@ __device__ inline int floor16(int x) { return x >= 0 ? x & ~15 : -((-x + 15) & ~
 
 // Stages the tile's ref region (columns from xs = floor16(x0 - R), rows from
 // y0 - R, coordinates clamped to the frame) and cur tile.  Returns x0 - R - xs.
-// Interior tiles of 16-byte-aligned frames load 16-byte chunks, all issued
-// before the first LDS store; other tiles assemble clamped bytes.
+// Frames with 16-byte-aligned base and pitch (and W >= 16) load 16-byte
+// chunks, all issued before the first LDS store (stage_rows16); others
+// assemble clamped dwords.
 __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs &a, int bx0, int by0) {
-    constexpr int U = 4;
     const int xr = bx0 * 8 - a.R, y0 = by0 * 8 - a.R, xs = floor16(xr);
-    const int s16 = a.stride4 / 4, n16 = a.rows * s16;
-    if (a.ref16 && xs >= 0 && xs + 16 * s16 <= a.W) {
-        // No branches around the loads or stores, so that all U loads are in
-        // flight before the first store: indices past the region repeat its
-        // last chunk (the same bytes stored to the same slot twice).
-        for (int base = 0; base < n16; base += U * RT_THREADS) {
-            uint4 v[U];
-            int idx[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                idx[u] = min(base + u * RT_THREADS + (int)threadIdx.x, n16 - 1);
-                const int r = idx[u] / s16, c = idx[u] - r * s16;
-                v[u] = *reinterpret_cast<const uint4 *>(a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch + xs +
-                                                       16 * c);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) reinterpret_cast<uint4 *>(reg)[idx[u]] = v[u];
-        }
+    const auto ref_row = [&](int r) { return a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch; };
+    // cur tile: 8 RT_Y rows of 8 RT_X bytes; rows past the frame belong to no block
+    const auto cur_row = [&](int r) { return a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch; };
+    if (a.ref16) {
+        stage_rows16<4>(reg, a.rows, a.stride4 / 4, a.stride4 / 4, xs, a.W, threadIdx.x, RT_THREADS, ref_row);
     } else {
         stage_clamped<8>(reg, a.rows * a.stride4, threadIdx.x, RT_THREADS, a.W,
                          [&](int i, const uint8_t *&row, int &gx) {
                              const int r = i / a.stride4;
-                             row = a.ref + (size_t)clampi(y0 + r, 0, a.H - 1) * a.pitch;
+                             row = ref_row(r);
                              gx = xs + 4 * (i - r * a.stride4);
                          });
     }
-    // cur tile: 8 RT_Y rows of 8 RT_X bytes; rows past the frame belong to no block
-    constexpr int C16 = 8 * RT_Y * RT_X / 2;  // 16-byte chunks
-    static_assert(C16 <= RT_THREADS, "one chunk per thread");
-    if (a.cur16 && bx0 * 8 + 8 * RT_X <= a.W) {
-        // threads past C16 repeat chunk t % C16 (same bytes, same slot)
-        const int i = threadIdx.x % C16, r = i / (RT_X / 2), c = i - r * (RT_X / 2);
-        reinterpret_cast<uint4 *>(ctile)[i] =
-            *reinterpret_cast<const uint4 *>(a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch + bx0 * 8 + 16 * c);
+    static_assert(8 * RT_Y * RT_X / 16 <= RT_THREADS, "one cur chunk per thread");
+    if (a.cur16) {
+        stage_rows16<1>(ctile, 8 * RT_Y, RT_X / 2, RT_X / 2, bx0 * 8, a.W, threadIdx.x, RT_THREADS, cur_row);
     } else {
         stage_clamped<2>(ctile, 8 * RT_Y * 2 * RT_X, threadIdx.x, RT_THREADS, a.W,
                          [&](int i, const uint8_t *&row, int &gx) {
                              const int r = i / (2 * RT_X);
-                             row = a.cur + (size_t)min(by0 * 8 + r, a.H - 1) * a.pitch;
+                             row = cur_row(r);
                              gx = bx0 * 8 + 4 * (i - r * (2 * RT_X));
                          });
     }
@@ -1403,8 +1385,8 @@ umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, i
     RefState *buf[2] = {(RefState *)ws, (RefState *)((char *)ws + sb)};
     int32_t *changed = (int32_t *)((char *)ws + 2 * sb);
     RefineArgs a{cur, ref, H, W, pitch, R, iters, ceil_div(nbx, RT_X), 0, 0, 0, 0, 0};
-    a.ref16 = ((reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 15) == 0;
-    a.cur16 = ((reinterpret_cast<uintptr_t>(cur) | (uintptr_t)pitch) & 15) == 0;
+    a.ref16 = ((reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 15) == 0 && W >= 16;
+    a.cur16 = ((reinterpret_cast<uintptr_t>(cur) | (uintptr_t)pitch) & 15) == 0 && W >= 16;
     a.ntiles = a.tiles_x * ceil_div(nby, RT_Y);
     region_shape(R, a.stride4, a.rows);
     const size_t lds = region_lds_bytes(R);

@@ -58,21 +58,24 @@ __device__ inline uint4 load16_clamped(const uint8_t *row, int gx, int W) {
 
 // dst holds `rows` rows of s16 16-byte chunks at a stride of d16 >= s16
 // chunks; chunk (r, c) = the clamped bytes rowp(r)[xs + 16 c .. + 16).
-// rowp(r) must be 16-byte aligned with
-// W >= 16 bytes, xs a multiple of 16 and dst 16-byte aligned.  A chunk
-// inside [0, W) is one 16-byte load; the others take load16_clamped.  As in
-// stage_clamped, all U loads are issued before the first store: indices past
-// the end repeat the last chunk (same bytes, same slot) and non-fast lanes
-// first load the row's first 16 bytes.
+// rowp(r) must be 16-byte aligned with W >= 16 bytes, xs a multiple of 16
+// and dst 16-byte aligned.  Every chunk takes one 16-byte load: a chunk
+// inside [0, W) loads itself; a chunk left of the row loads the row's first
+// chunk and repeats its byte 0; a chunk right of a row whose width is a
+// multiple of 16 loads the row's last chunk and repeats its byte 15.  Only a
+// chunk across W, or right of a row of another width, is assembled by
+// load16_clamped after the loads.  As in stage_clamped, all U loads are
+// issued before the first store: indices past the end repeat the last chunk
+// (same bytes, same slot).
 template <int U, class RowPtr>
 __device__ inline void stage_rows16(uint32_t *dst, int rows, int s16, int d16, int xs, int W, int tid, int nthr,
                                     RowPtr rowp) {
     const int n = rows * s16;
+    const bool w16 = (W & 15) == 0;
     for (int base = 0; base < n; base += U * nthr) {
         uint4 v[U];
-        int idx[U], slot[U], gx[U];
+        int idx[U], slot[U], gx[U], kind[U];  // kind: 0 inside, 1 left, 2 right, 3 assembled
         const uint8_t *row[U];
-        bool fast[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             idx[u] = min(base + u * nthr + tid, n - 1);
@@ -80,12 +83,16 @@ __device__ inline void stage_rows16(uint32_t *dst, int rows, int s16, int d16, i
             row[u] = rowp(r);
             gx[u] = xs + 16 * c;
             slot[u] = r * d16 + c;
-            fast[u] = gx[u] >= 0 && gx[u] + 16 <= W;
-            v[u] = *reinterpret_cast<const uint4 *>(row[u] + (fast[u] ? gx[u] : 0));
+            kind[u] = gx[u] >= 0 && gx[u] + 16 <= W ? 0 : gx[u] < 0 ? 1 : gx[u] >= W && w16 ? 2 : 3;
+            const int at = kind[u] == 0 ? gx[u] : kind[u] == 2 ? W - 16 : 0;
+            v[u] = *reinterpret_cast<const uint4 *>(row[u] + at);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (!fast[u]) v[u] = load16_clamped(row[u], gx[u], W);
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = (kind[u] == 1 ? v[u].x & 0xffu : v[u].w >> 24) * 0x01010101u;
+            if (kind[u] == 1 || kind[u] == 2) v[u] = make_uint4(e, e, e, e);
+            if (kind[u] == 3) v[u] = load16_clamped(row[u], gx[u], W);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) reinterpret_cast<uint4 *>(dst)[slot[u]] = v[u];
     }
