@@ -13,6 +13,7 @@
 //    propagation rounds, fallback check) and the fallback list.
 #include "common.h"
 #include "stage.h"
+#include "wave.h"
 
 #include <algorithm>
 
@@ -869,12 +870,8 @@ __device__ inline void global_cur(uint32_t (&cw)[16], const uint8_t *cur, int pi
     }
 }
 
-// Minimum over the 8 lanes of a block.
-__device__ inline uint32_t group_min(uint32_t k) {
-#pragma unroll
-    for (int m = 4; m > 0; m >>= 1) k = min(k, (uint32_t)__shfl_xor((int)k, m, 8));
-    return k;
-}
+// Minimum over the 8 lanes of a block (DPP, no LDS round trip).
+__device__ inline uint32_t group_min(uint32_t k) { return min8_u32(k); }
 
 // Iterated +-2 descent from (cy, cx).  Each step takes the 25 positions
 // clamp(c + (i / 5 - 2, i % 5 - 2), -R, R), i = 0..24, and moves to the
@@ -1073,7 +1070,8 @@ __global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, 
     extern __shared__ uint32_t lds[];
     const int tile = tile_of_workgroup();
     if (tile >= a.ntiles) return;
-    if (prev_changed && *prev_changed == 0) return;
+    // read now, tested after the loads below are in flight (most rounds run)
+    const int32_t run = prev_changed ? *prev_changed : 1;
     uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
     const BlockCtx c = block_ctx(a, tile);
     const int nby = a.H / 8, nbx = a.W / 8;
@@ -1090,6 +1088,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_round_kernel(RefineArgs a, 
         mx[j] = n.dx;
     }
     const int xoff = fill_tile(reg, ctile, a, (tile % a.tiles_x) * RT_X, (tile / a.tiles_x) * RT_Y);
+    if (run == 0) return;  // whole workgroup
     __syncthreads();
     bool improved = false;
     if (c.valid) {

@@ -1,4 +1,4 @@
-// Wave64 cross-lane scans built on DPP (gfx9 row_shr / row_bcast controls).
+// Wave64 cross-lane scans and reductions built on DPP (gfx9 controls).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -8,10 +8,22 @@ namespace umi {
 // DPP control words (GFX9 encoding).
 constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
 constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+constexpr int DPP_QUAD_XOR1 = 0xb1, DPP_QUAD_XOR2 = 0x4e;  // quad_perm [1,0,3,2], [2,3,0,1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;                 // lane i <- lane 7 - i within 8
 
 template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
 __device__ inline uint32_t dpp_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, BOUND_ZERO);
+}
+
+// Minimum over each aligned group of 8 lanes, all 8 active: the quad_perm
+// swaps give every lane its quad's minimum, row_half_mirror pairs each lane
+// with one of the other quad.
+__device__ inline uint32_t min8_u32(uint32_t k) {
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_QUAD_XOR1, 0xf, 0xf, false));
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_QUAD_XOR2, 0xf, 0xf, false));
+    k = min(k, (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_HALF_MIRROR, 0xf, 0xf, false));
+    return k;
 }
 
 // Inclusive prefix sum across the 64 lanes of a wave (lane order), with
