@@ -328,6 +328,12 @@ umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, in
  * block proposals of umi_me_proposals into prop when prop is not NULL. */
 umi_status umi_cnn_first_layer_frames(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
                                       const void *packed, const float *bias, void *out_nhwc, umi_stream s);
+/* The first two layers in one launch: umi_cnn_first_layer_frames followed by
+ * layer kind 1 of dilation `dil` (packed1, bias1), identical results, without
+ * the first layer's output in device memory. */
+umi_status umi_cnn_first_two_layers_frames(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                          const void *packed0, const float *bias0, int dil, const void *packed1,
+                                          const float *bias1, void *out_nhwc, umi_stream s);
 umi_status umi_cnn_last_layer_proposals(const void *in_nhwc, int H4, int W4, int dil, const void *packed,
                                         const float *bias, int R, float *flow, int32_t *prop, umi_stream s);
 /* Block proposals from the quarter-resolution flow: round(4 x mean of the

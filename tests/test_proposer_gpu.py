@@ -156,6 +156,29 @@ def test_fused_first_layer_equals_downsample_path(gpu, H, W):
     np.testing.assert_array_equal(a.download((H4 * W4 * 32,), np.uint16), b.download((H4 * W4 * 32,), np.uint16))
 
 
+@pytest.mark.parametrize("H,W,d,pitch", [(1080, 1920, 2, 1920), (72, 136, 2, 136), (44, 70, 2, 70),
+                                         (60, 100, 3, 101), (36, 200, 1, 200)])
+def test_fused_first_two_layers_equal_two_launches(gpu, H, W, d, pitch):
+    """umi_cnn_first_two_layers_frames = umi_cnn_first_layer_frames + layer
+    kind 1 of dilation d, bit for bit (pitch 101: the byte-load path)."""
+    g = workloads.rng(H * W + d)
+    cur = g.integers(0, 256, size=(H, pitch)).astype(np.uint8)
+    ref = g.integers(0, 256, size=(H, pitch)).astype(np.uint8)
+    net = proposer.Network(seed=7)
+    p0, p1 = proposer.pack_layer(net, 0), proposer.pack_layer(net, 1)
+    H4, W4 = H // 4, W // 4
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    a, b, c = (DeviceBuffer(H4 * W4 * 64) for _ in range(3))
+    print(f"[fused] first two layers H={H} W={W} d={d} pitch={pitch}", flush=True)
+    call("umi_cnn_first_layer_frames", dc.ptr, dr.ptr, H, W, pitch, p0.ptr, net.b[0].ptr, a.ptr, None)
+    call("umi_cnn_infer_layer", 1, a.ptr, 1, H4, W4, d, p1.ptr, net.b[1].ptr, b.ptr, None)
+    call("umi_cnn_first_two_layers_frames", dc.ptr, dr.ptr, H, W, pitch, p0.ptr, net.b[0].ptr, d, p1.ptr,
+         net.b[1].ptr, c.ptr, None)
+    want = b.download((H4 * W4 * 32,), np.uint16)
+    assert np.count_nonzero(want) > want.size // 4  # ReLU leaves a good share of the outputs
+    np.testing.assert_array_equal(c.download((H4 * W4 * 32,), np.uint16), want)
+
+
 @pytest.mark.parametrize("H4,W4,R", [(270, 480, 32), (19, 37, 5), (8, 16, 32)])
 def test_fused_last_layer_equals_flow_and_proposals(gpu, H4, W4, R):
     """umi_cnn_last_layer_proposals writes layer kind 2's flow and

@@ -84,6 +84,7 @@ SIGNATURES = {
     "umi_cnn_pack_layer": [c_int, P, P, P],
     "umi_cnn_infer_layer": [c_int, P, c_int, c_int, c_int, c_int, P, P, P, P],
     "umi_cnn_first_layer_frames": [P, P, c_int, c_int, c_int, P, P, P, P],
+    "umi_cnn_first_two_layers_frames": [P, P, c_int, c_int, c_int, P, P, c_int, P, P, P, P],
     "umi_cnn_last_layer_proposals": [P, c_int, c_int, c_int, P, P, c_int, P, P, P],
     "umi_me_proposals": [P, c_int, c_int, c_int, P, P],
     "umi_me_refine_workspace": [c_int, c_int, c_int, P],
@@ -124,4 +125,8 @@ def check(status: int, what: str = "") -> None:
 
 
 def call(name: str, *args) -> None:
+    """Call entry point `name`; only entry points with declared argument
+    types (without them ctypes would pass 64-bit pointers as 32-bit ints)."""
+    if name not in SIGNATURES:
+        raise UmiError(f"{name}: no argument types declared in umi_amd._lib.SIGNATURES")
     check(getattr(lib(), name)(*args), name)

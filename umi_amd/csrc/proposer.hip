@@ -424,6 +424,56 @@ __global__ __launch_bounds__(256) void cnn_layer_kernel(const void *__restrict__
 // each).  Same operands, order and results as cnn_layer_kernel<CNN_MID>.
 constexpr int CT_H = 8, CT_W = 32, CT_PX = 80;
 
+// The stored form of a layer output: bias, ReLU, bf16 (RNE); lane (l & 31,
+// l >> 5) writes channels 8 q + 4 (l >> 5) .. + 3, q = 0..3, of its pixel.
+__device__ inline void put_relu_bf16(const f32x16 &acc, const float (&bv)[16], int lh, uint16_t *px) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t h[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
+            const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
+            h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+        }
+        *(uint2 *)(px + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
+    }
+}
+
+// A operands (18 chunks) and bias of a 32 -> 32 layer, per lane.
+__device__ inline void mid_operands(const bf16x8 *__restrict__ wp, const float *__restrict__ bias, bf16x8 (&A)[18],
+                                    float (&bv)[16]) {
+    const int lane = threadIdx.x & 63, lh = lane >> 5;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) A[c] = wp[c * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bv[r] = bias[8 * (r >> 2) + 4 * lh + (r & 3)];
+}
+
+// A 32 -> 32 layer of dilation d on the CT_H x CT_W output tile at (y0, x0)
+// of an H x W image, from its (CT_H + 2d) x (CT_W + 2d) input pixels in LDS
+// (hw = CT_W + 2d per row, CT_PX bytes per pixel, zero outside the image).
+// Wave w computes rows 2w and 2w + 1; bf16 NHWC output.
+__device__ inline void mid_from_tile(const uint8_t *tile, int hw, int d, int y0, int x0, int H, int W,
+                                     const bf16x8 (&A)[18], const float (&bv)[16], uint16_t *__restrict__ out) {
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int r = 2 * w + rr, y = y0 + r, x = x0 + lr;
+        f32x16 acc = {};
+#pragma unroll
+        for (int tp = 0; tp < 9; ++tp) {
+            const int tr = r + d + (tp / 3 - 1) * d, tc = lr + d + (tp % 3 - 1) * d;
+            const uint8_t *src = tile + (tr * hw + tc) * CT_PX + 16 * lh;
+            const bf16x8 b0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)src);
+            const bf16x8 b1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(src + 32));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp], b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp + 1], b1, acc, 0, 0, 0);
+        }
+        if (y < H && x < W) put_relu_bf16(acc, bv, lh, out + ((long)y * W + x) * 32);
+    }
+}
+
 __global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__restrict__ in, int H, int W, int d,
                                                            const bf16x8 *__restrict__ wp,
                                                            const float *__restrict__ bias,
@@ -455,42 +505,11 @@ __global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__res
             *(u32x4 *)(tile + px * CT_PX + 16 * q) = v[u];
         }
     }
-    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5, w = threadIdx.x >> 6;
     bf16x8 A[18];
-#pragma unroll
-    for (int c = 0; c < 18; ++c) A[c] = wp[c * 64 + lane];
     float bv[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) bv[r] = bias[8 * (r >> 2) + 4 * lh + (r & 3)];
+    mid_operands(wp, bias, A, bv);
     __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-        const int r = 2 * w + rr, y = y0 + r, x = x0 + lr;
-        f32x16 acc = {};
-#pragma unroll
-        for (int tp = 0; tp < 9; ++tp) {
-            const int tr = r + d + (tp / 3 - 1) * d, tc = lr + d + (tp % 3 - 1) * d;
-            const uint8_t *src = tile + (tr * hw + tc) * CT_PX + 16 * lh;
-            const bf16x8 b0 = __builtin_bit_cast(bf16x8, *(const u32x4 *)src);
-            const bf16x8 b1 = __builtin_bit_cast(bf16x8, *(const u32x4 *)(src + 32));
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp], b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2 * tp + 1], b1, acc, 0, 0, 0);
-        }
-        if (y < H && x < W) {
-            uint16_t *o = out + ((long)n * plane + (long)y * W + x) * 32;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t h[2];
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
-                    const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
-                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
-                }
-                *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
-            }
-        }
-    }
+    mid_from_tile(tile, hw, d, y0, x0, H, W, A, bv, out + (long)n * plane * 32);
 }
 
 // The first layer read straight from the frames: the workgroup computes its
@@ -562,6 +581,102 @@ __global__ __launch_bounds__(256) void cnn_first_frames_kernel(const uint8_t *__
             }
         }
     }
+}
+
+// The first two layers in one launch, for the second layer's CT_H x CT_W
+// output tile at (y0, x0): the workgroup computes the quarter-resolution
+// inputs of the (CT_H + 2d + 2) x (CT_W + 2d + 2) pixels around it from the
+// frames (as cnn_first_frames_kernel: dword row loads when A4, the frames
+// and pitch being 4-byte aligned), the first layer's outputs on the
+// (CT_H + 2d) x (CT_W + 2d) pixels into LDS (zero outside the image: the
+// second layer's padding), and the second layer (dilation d) from there as
+// cnn_mid_tile_kernel.  Same operands, order and bf16 roundings as those two
+// launches; one image (N = 1).
+template <bool A4>
+__global__ __launch_bounds__(256) void cnn_first_mid_frames_kernel(
+    const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch,
+    const bf16x8 *__restrict__ wp0, const float *__restrict__ bias0, int d, const bf16x8 *__restrict__ wp1,
+    const float *__restrict__ bias1, uint16_t *__restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t tile[];
+    const int H4 = H / 4, W4 = W / 4;
+    const int tiles_x = (W4 + CT_W - 1) / CT_W;
+    const int y0 = (blockIdx.x / tiles_x) * CT_H, x0 = (blockIdx.x % tiles_x) * CT_W;
+    const int hw = CT_W + 2 * d, hh = CT_H + 2 * d;  // first-layer outputs
+    const int iw = hw + 2, ih = hh + 2;              // their inputs
+    uint16_t(*xin)[2] = reinterpret_cast<uint16_t(*)[2]>(tile + hh * hw * CT_PX);
+    // item i: quarter-resolution pixel i / 2 of cur (i even) or ref (i odd);
+    // all row loads of a pass are issued before the first sum
+    constexpr int U = 6;
+    const int nitem = ih * iw * 2;
+    for (int base = 0; base < nitem; base += U * 256) {
+        uint32_t v[U][4];
+        int idx[U];
+        bool inside[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            idx[u] = min(base + u * 256 + (int)threadIdx.x, nitem - 1);
+            const int px = idx[u] >> 1, r = px / iw, c = px - r * iw;
+            const int Y = y0 - d - 1 + r, X = x0 - d - 1 + c;
+            inside[u] = Y >= 0 && Y < H4 && X >= 0 && X < W4;
+            const uint8_t *f =
+                ((idx[u] & 1) ? ref : cur) + (size_t)(4 * clampi(Y, 0, H4 - 1)) * pitch + 4 * clampi(X, 0, W4 - 1);
+#pragma unroll
+            for (int dy = 0; dy < 4; ++dy) {
+                const uint8_t *q = f + (size_t)dy * pitch;
+                if constexpr (A4)
+                    v[u][dy] = *reinterpret_cast<const uint32_t *>(q);
+                else
+                    v[u][dy] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int dy = 0; dy < 4; ++dy) sum = __builtin_amdgcn_sad_u8(v[u][dy], 0u, sum);  // byte sum
+            const float val = inside[u] ? ((int)sum * (1.f / 16.f) - 128.f) * 0.125f : 0.f;
+            xin[idx[u] >> 1][idx[u] & 1] = bf16_rne(val);
+        }
+    }
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5, w = threadIdx.x >> 6;
+    const bf16x8 A0 = wp0[lane], A1 = wp0[64 + lane];
+    float b0v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b0v[r] = bias0[8 * (r >> 2) + 4 * lh + (r & 3)];
+    __syncthreads();
+    // first layer: 32-pixel groups of the hh x hw region in raster order;
+    // lanes past its end repeat its last pixel (same value, same slot)
+    const int npx = hh * hw;
+    for (int g = w; g < (npx + 31) / 32; g += 4) {
+        const int p = min(32 * g + lr, npx - 1), r = p / hw, c = p - r * hw;
+        bf16x8 B[2];
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            uint16_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 16 * ch + 8 * lh + j, t = k >> 1;
+                v[j] = k < 18 ? xin[(r + t / 3) * iw + c + t % 3][k & 1] : (uint16_t)0;
+            }
+            B[ch] = __builtin_bit_cast(bf16x8, v);
+        }
+        f32x16 acc = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0, B[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, B[1], acc, 0, 0, 0);
+        const int Y = y0 - d + r, X = x0 - d + c;
+        uint16_t *dst = reinterpret_cast<uint16_t *>(tile + p * CT_PX);
+        if (Y >= 0 && Y < H4 && X >= 0 && X < W4) {
+            put_relu_bf16(acc, b0v, lh, dst);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) *(uint2 *)(dst + 8 * q + 4 * lh) = make_uint2(0, 0);
+        }
+    }
+    bf16x8 A[18];
+    float bv[16];
+    mid_operands(wp1, bias1, A, bv);
+    __syncthreads();
+    mid_from_tile(tile, hw, d, y0, x0, H4, W4, A, bv, out);
 }
 
 // The last layer with the block proposals: a 32-pixel group is 2 rows x 16
@@ -1336,6 +1451,29 @@ umi_status umi_cnn_first_layer_frames(const uint8_t *cur, const uint8_t *ref, in
     hipLaunchKernelGGL(cnn_first_frames_kernel, dim3(tiles), dim3(256), 0, as_stream(s), cur, ref, H, W, pitch,
                        (const bf16x8 *)packed, bias, (uint16_t *)out_nhwc);
     return launch_check("cnn_first_frames_kernel");
+}
+
+umi_status umi_cnn_first_two_layers_frames(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                          const void *packed0, const float *bias0, int dil, const void *packed1,
+                                          const float *bias1, void *out_nhwc, umi_stream s) {
+    if (!cur || !ref || !packed0 || !bias0 || !packed1 || !bias1 || !out_nhwc)
+        return set_error(UMI_ERR_INVALID, "umi_cnn_first_two_layers_frames: null pointer");
+    if (H < 4 || W < 4 || pitch < W || dil < 1)
+        return set_error(UMI_ERR_INVALID, "umi_cnn_first_two_layers_frames: bad frame or dilation");
+    const size_t lds = (size_t)(CT_H + 2 * dil) * (CT_W + 2 * dil) * CT_PX +
+                       (size_t)(CT_H + 2 * dil + 2) * (CT_W + 2 * dil + 2) * 4;
+    if (lds > 64 * 1024) return set_error(UMI_ERR_UNSUPPORTED, "umi_cnn_first_two_layers_frames: dilation %d", dil);
+    const int tiles = ceil_div(H / 4, CT_H) * ceil_div(W / 4, CT_W);
+    const bool a4 = ((reinterpret_cast<uintptr_t>(cur) | reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 3) == 0;
+    if (a4)
+        hipLaunchKernelGGL(cnn_first_mid_frames_kernel<true>, dim3(tiles), dim3(256), lds, as_stream(s), cur, ref, H, W,
+                           pitch, (const bf16x8 *)packed0, bias0, dil, (const bf16x8 *)packed1, bias1,
+                           (uint16_t *)out_nhwc);
+    else
+        hipLaunchKernelGGL(cnn_first_mid_frames_kernel<false>, dim3(tiles), dim3(256), lds, as_stream(s), cur, ref, H,
+                           W, pitch, (const bf16x8 *)packed0, bias0, dil, (const bf16x8 *)packed1, bias1,
+                           (uint16_t *)out_nhwc);
+    return launch_check("cnn_first_mid_frames_kernel");
 }
 
 umi_status umi_cnn_last_layer_proposals(const void *in_nhwc, int H4, int W4, int dil, const void *packed,

@@ -281,14 +281,13 @@ class Proposer:
         self.ws = DeviceBuffer(n.value)
 
     def propose(self, cur: DeviceBuffer, ref: DeviceBuffer, stream=None):
-        """Network and block proposals: the first layer reads the frames, the
-        last one writes the proposals (4 launches)."""
+        """Network and block proposals: the first two layers read the frames
+        in one launch, the last one writes the proposals (3 launches)."""
         s = _s(stream)
-        call("umi_cnn_first_layer_frames", cur.ptr, ref.ptr, self.H, self.W, self.W, self.packed[0].ptr,
-             self.net.b[0].ptr, self.a[0].ptr, s)
-        for li in (1, 2):
-            call("umi_cnn_infer_layer", 1, self.a[li - 1].ptr, 1, self.H4, self.W4, LAYERS[li][2], self.packed[li].ptr,
-                 self.net.b[li].ptr, self.a[li].ptr, s)
+        call("umi_cnn_first_two_layers_frames", cur.ptr, ref.ptr, self.H, self.W, self.W, self.packed[0].ptr,
+             self.net.b[0].ptr, LAYERS[1][2], self.packed[1].ptr, self.net.b[1].ptr, self.a[1].ptr, s)
+        call("umi_cnn_infer_layer", 1, self.a[1].ptr, 1, self.H4, self.W4, LAYERS[2][2], self.packed[2].ptr,
+             self.net.b[2].ptr, self.a[2].ptr, s)
         call("umi_cnn_last_layer_proposals", self.a[2].ptr, self.H4, self.W4, LAYERS[3][2], self.packed[3].ptr,
              self.net.b[3].ptr, self.R, self.flow.ptr, self.prop.ptr, s)
 
