@@ -289,6 +289,13 @@ __device__ inline uint16_t bf16_rne(float f) {
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 __device__ inline float bf16f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+// Two values rounded to bf16 (RNE) by one v_cvt_pk_bf16_f32, packed low = a;
+// for finite inputs the same bits as bf16_rne.
+__device__ inline uint32_t bf16x2_rne(float a, float b) {
+    typedef __attribute__((ext_vector_type(2))) float f32x2;
+    typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(256) void cnn_layer_kernel(const void *__restrict__
                 for (int e = 0; e < 2; ++e) {
                     const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
                     const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
-                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+                    h[e] = bf16x2_rne(a0, a1);
                 }
                 *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
             }
@@ -434,7 +441,7 @@ __device__ inline void put_relu_bf16(const f32x16 &acc, const float (&bv)[16], i
         for (int e = 0; e < 2; ++e) {
             const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
             const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
-            h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+            h[e] = bf16x2_rne(a0, a1);
         }
         *(uint2 *)(px + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
     }
@@ -474,11 +481,14 @@ __device__ inline void mid_from_tile(const uint8_t *tile, int hw, int d, int y0,
     }
 }
 
-__global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__restrict__ in, int H, int W, int d,
+// D > 0: the dilation as a compile-time constant (d is ignored); 0: d.
+template <int D>
+__global__ __launch_bounds__(256) void cnn_mid_tile_kernel(const uint16_t *__restrict__ in, int H, int W, int d_,
                                                            const bf16x8 *__restrict__ wp,
                                                            const float *__restrict__ bias,
                                                            uint16_t *__restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) uint8_t tile[];
+    const int d = D > 0 ? D : d_;
     const int tiles_x = (W + CT_W - 1) / CT_W, tiles_y = (H + CT_H - 1) / CT_H;
     const int n = blockIdx.x / (tiles_x * tiles_y), t = blockIdx.x % (tiles_x * tiles_y);
     const int y0 = (t / tiles_x) * CT_H, x0 = (t % tiles_x) * CT_W;
@@ -575,7 +585,7 @@ __global__ __launch_bounds__(256) void cnn_first_frames_kernel(const uint8_t *__
                 for (int e = 0; e < 2; ++e) {
                     const float a0 = fmaxf(acc[4 * q + 2 * e] + bv[4 * q + 2 * e], 0.f);
                     const float a1 = fmaxf(acc[4 * q + 2 * e + 1] + bv[4 * q + 2 * e + 1], 0.f);
-                    h[e] = (uint32_t)bf16_rne(a0) | ((uint32_t)bf16_rne(a1) << 16);
+                    h[e] = bf16x2_rne(a0, a1);
                 }
                 *(uint2 *)(o + 8 * q + 4 * lh) = make_uint2(h[0], h[1]);
             }
@@ -592,12 +602,13 @@ __global__ __launch_bounds__(256) void cnn_first_frames_kernel(const uint8_t *__
 // second layer's padding), and the second layer (dilation d) from there as
 // cnn_mid_tile_kernel.  Same operands, order and bf16 roundings as those two
 // launches; one image (N = 1).
-template <bool A4>
+template <bool A4, int D>
 __global__ __launch_bounds__(256) void cnn_first_mid_frames_kernel(
     const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int H, int W, int pitch,
-    const bf16x8 *__restrict__ wp0, const float *__restrict__ bias0, int d, const bf16x8 *__restrict__ wp1,
+    const bf16x8 *__restrict__ wp0, const float *__restrict__ bias0, int d_, const bf16x8 *__restrict__ wp1,
     const float *__restrict__ bias1, uint16_t *__restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) uint8_t tile[];
+    const int d = D > 0 ? D : d_;  // D > 0: compile-time dilation
     const int H4 = H / 4, W4 = W / 4;
     const int tiles_x = (W4 + CT_W - 1) / CT_W;
     const int y0 = (blockIdx.x / tiles_x) * CT_H, x0 = (blockIdx.x % tiles_x) * CT_W;
@@ -1427,8 +1438,10 @@ umi_status umi_cnn_infer_layer(int kind, const void *in, int N, int H, int W, in
             const size_t lds = (size_t)(CT_H + 2 * dil) * (CT_W + 2 * dil) * CT_PX;
             if (lds <= 64 * 1024) {  // staged tile; very large dilations read global memory per tap
                 const long tiles = (long)N * ceil_div(H, CT_H) * ceil_div(W, CT_W);
-                hipLaunchKernelGGL(cnn_mid_tile_kernel, dim3((unsigned)tiles), dim3(256), lds, st,
-                                   (const uint16_t *)in, H, W, dil, wp, bias, (uint16_t *)out);
+                const auto k = dil == 2 ? cnn_mid_tile_kernel<2> : dil == 4 ? cnn_mid_tile_kernel<4>
+                                                                             : cnn_mid_tile_kernel<0>;
+                hipLaunchKernelGGL(k, dim3((unsigned)tiles), dim3(256), lds, st, (const uint16_t *)in, H, W, dil, wp,
+                                   bias, (uint16_t *)out);
                 return launch_check("cnn_mid_tile_kernel");
             }
             hipLaunchKernelGGL(cnn_layer_kernel<CNN_MID>, dim3(grid), dim3(256), 0, st, in, N, H, W, dil, wp, bias, out);
@@ -1465,14 +1478,10 @@ umi_status umi_cnn_first_two_layers_frames(const uint8_t *cur, const uint8_t *re
     if (lds > 64 * 1024) return set_error(UMI_ERR_UNSUPPORTED, "umi_cnn_first_two_layers_frames: dilation %d", dil);
     const int tiles = ceil_div(H / 4, CT_H) * ceil_div(W / 4, CT_W);
     const bool a4 = ((reinterpret_cast<uintptr_t>(cur) | reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 3) == 0;
-    if (a4)
-        hipLaunchKernelGGL(cnn_first_mid_frames_kernel<true>, dim3(tiles), dim3(256), lds, as_stream(s), cur, ref, H, W,
-                           pitch, (const bf16x8 *)packed0, bias0, dil, (const bf16x8 *)packed1, bias1,
-                           (uint16_t *)out_nhwc);
-    else
-        hipLaunchKernelGGL(cnn_first_mid_frames_kernel<false>, dim3(tiles), dim3(256), lds, as_stream(s), cur, ref, H,
-                           W, pitch, (const bf16x8 *)packed0, bias0, dil, (const bf16x8 *)packed1, bias1,
-                           (uint16_t *)out_nhwc);
+    const auto k = a4 ? (dil == 2 ? cnn_first_mid_frames_kernel<true, 2> : cnn_first_mid_frames_kernel<true, 0>)
+                      : cnn_first_mid_frames_kernel<false, 0>;
+    hipLaunchKernelGGL(k, dim3(tiles), dim3(256), lds, as_stream(s), cur, ref, H, W, pitch, (const bf16x8 *)packed0,
+                       bias0, dil, (const bf16x8 *)packed1, bias1, (uint16_t *)out_nhwc);
     return launch_check("cnn_first_mid_frames_kernel");
 }
 
