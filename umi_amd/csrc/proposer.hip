@@ -851,42 +851,58 @@ __device__ inline int fill_tile(uint32_t *reg, uint32_t *ctile, const RefineArgs
 struct LdsSad {
     const uint32_t *reg;
     int stride4, o0;  // o0: byte offset of the block's vector (-R, -R) in the region
+    // All 24 reads are issued before the first SAD; even and odd rows sum in
+    // two interleaved chains.
     __device__ uint32_t operator()(const uint32_t (&cw)[16], int R, int dy, int dx) const {
         const int o = o0 + (dy + R) * stride4 * 4 + dx + R;
         const uint32_t *p = reg + (o >> 2);
         const uint32_t sh = (uint32_t)(o & 3);
-        uint32_t s = 0;
+        uint32_t d[8][3];
+#pragma unroll
+        for (int py = 0; py < 8; ++py)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d[py][k] = p[py * stride4 + k];
+        uint32_t s[2] = {0, 0};
 #pragma unroll
         for (int py = 0; py < 8; ++py) {
-            const uint32_t d0 = p[py * stride4], d1 = p[py * stride4 + 1], d2 = p[py * stride4 + 2];
-            s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, sh), cw[2 * py], s);
-            s = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, sh), cw[2 * py + 1], s);
+            uint32_t &a = s[py & 1];
+            a = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d[py][1], d[py][0], sh), cw[2 * py], a);
+            a = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d[py][2], d[py][1], sh), cw[2 * py + 1], a);
         }
-        return s;
+        return s[0] + s[1];
     }
     // SADs at (dy, dx0 + c), c = 0..4 (all within [-R, R]) from one pass over
     // 8 rows of 16 bytes: v_qsad_pk_u16_u8 gives the 4 horizontal offsets of
-    // c = 0..3, two v_sad_u8 the fifth.
+    // c = 0..3, two v_sad_u8 the fifth.  All 32 reads are issued first; even
+    // and odd rows accumulate in two interleaved chains (each packed u16 SAD
+    // stays below 2^15, so one 64-bit add merges them without carries between
+    // fields).
     __device__ void row5(const uint32_t (&cw)[16], int R, int dy, int dx0, uint32_t (&s)[5]) const {
         const int o = o0 + (dy + R) * stride4 * 4 + dx0 + R;
         const uint32_t *p = reg + (o >> 2);
         const uint32_t sh = (uint32_t)(o & 3);
-        uint64_t acc = 0;
-        uint32_t s4 = 0;
+        uint32_t d[8][4];
+#pragma unroll
+        for (int py = 0; py < 8; ++py)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[py][k] = p[py * stride4 + k];
+        uint64_t acc[2] = {0, 0};
+        uint32_t s4[2] = {0, 0};
 #pragma unroll
         for (int py = 0; py < 8; ++py) {
-            const uint32_t *q = p + py * stride4;
-            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
-            const uint32_t a0 = __builtin_amdgcn_alignbyte(d1, d0, sh), a1 = __builtin_amdgcn_alignbyte(d2, d1, sh),
-                           a2 = __builtin_amdgcn_alignbyte(d3, d2, sh);
-            acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a0 | ((uint64_t)a1 << 32), cw[2 * py], acc);
-            acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a1 | ((uint64_t)a2 << 32), cw[2 * py + 1], acc);
-            s4 = __builtin_amdgcn_sad_u8(a1, cw[2 * py], s4);
-            s4 = __builtin_amdgcn_sad_u8(a2, cw[2 * py + 1], s4);
+            const int h = py & 1;
+            const uint32_t a0 = __builtin_amdgcn_alignbyte(d[py][1], d[py][0], sh),
+                           a1 = __builtin_amdgcn_alignbyte(d[py][2], d[py][1], sh),
+                           a2 = __builtin_amdgcn_alignbyte(d[py][3], d[py][2], sh);
+            acc[h] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a0 | ((uint64_t)a1 << 32), cw[2 * py], acc[h]);
+            acc[h] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)a1 | ((uint64_t)a2 << 32), cw[2 * py + 1], acc[h]);
+            s4[h] = __builtin_amdgcn_sad_u8(a1, cw[2 * py], s4[h]);
+            s4[h] = __builtin_amdgcn_sad_u8(a2, cw[2 * py + 1], s4[h]);
         }
+        const uint64_t t = acc[0] + acc[1];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) s[c] = (uint32_t)(acc >> (16 * c)) & 0xffffu;
-        s[4] = s4;
+        for (int c = 0; c < 4; ++c) s[c] = (uint32_t)(t >> (16 * c)) & 0xffffu;
+        s[4] = s4[0] + s4[1];
     }
 };
 __device__ inline LdsSad lds_sad(const uint32_t *reg, const RefineArgs &a, int xoff, int ty, int tx) {
