@@ -287,12 +287,16 @@ __global__ __launch_bounds__(me_max_threads<B>()) void me_full_search_kernel(
 // by * nbx + bx), one block per iteration.  Thread t searches the 4 horizontal
 // offsets 4 (t % G) .. + 3 over the vertical offsets of chunk t / G (DCH
 // offsets each), from the block's staged window; the first minimum in raster
-// order wins, as in the full search.
+// order wins, as in the full search.  A workgroup reads its next entry
+// before searching the current one.
+// fast16 (16-byte aligned frames and pitch, W >= 16, R % 4 == 0): the window
+// rows start at xs = floor16(x0 - R) and are staged in 16-byte chunks (ws4 a
+// multiple of 4); the search then reads them (x0 - R - xs) / 4 dwords in.
 template <int B>
 __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restrict__ cur,
                                                         const uint8_t *__restrict__ ref, int H, int W, int pitch,
                                                         int R, int nbx, int G, int DCH, int ws4, int wrows,
-                                                        int idx_bits, const int32_t *__restrict__ list,
+                                                        int idx_bits, int fast16, const int32_t *__restrict__ list,
                                                         const int32_t *__restrict__ count,
                                                         int32_t *__restrict__ mv_sad) {
     constexpr int B4 = B / 4;
@@ -301,14 +305,24 @@ __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restric
     const int S = 2 * R + 1;
     const int tid = threadIdx.x, g = tid % G, ch = tid / G;
     const int n = *count;
+    int next = (int)blockIdx.x < n ? list[blockIdx.x] : 0;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int blk = list[i];
+        const int blk = next;
+        if (i + (int)gridDim.x < n) next = list[i + gridDim.x];
         const int bx = blk % nbx, by = blk / nbx, x0 = bx * B, y0 = by * B;
-        stage_clamped<8>(win, wrows * ws4, tid, blockDim.x, W, [&](int k, const uint8_t *&row, int &gx) {
-            const int r = k / ws4;
-            row = ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch;
-            gx = x0 - R + 4 * (k - r * ws4);
-        });
+        const auto ref_row = [&](int r) { return ref + (size_t)clampi(y0 - R + r, 0, H - 1) * pitch; };
+        int woff = 0;
+        if (fast16) {
+            const int xs = (x0 - R) & ~15;
+            woff = (x0 - R - xs) >> 2;
+            stage_rows16<2>(win, wrows, ws4 / 4, ws4 / 4, xs, W, tid, blockDim.x, ref_row);
+        } else {
+            stage_clamped<8>(win, wrows * ws4, tid, blockDim.x, W, [&](int k, const uint8_t *&row, int &gx) {
+                const int r = k / ws4;
+                row = ref_row(r);
+                gx = x0 - R + 4 * (k - r * ws4);
+            });
+        }
         stage_clamped<1>(ctile, B * B4, tid, blockDim.x, W, [&](int k, const uint8_t *&row, int &gx) {
             row = cur + (size_t)(y0 + k / B4) * pitch;
             gx = x0 + 4 * (k % B4);
@@ -327,7 +341,7 @@ __global__ __launch_bounds__(512) void me_listed_kernel(const uint8_t *__restric
                 uint64_t acc = 0;  // 4 x u16 SADs at horizontal offsets 4g .. 4g + 3
 #pragma unroll
                 for (int py = 0; py < B; ++py) {
-                    const uint32_t *wr = win + (dyi + py) * ws4 + g;
+                    const uint32_t *wr = win + (dyi + py) * ws4 + woff + g;
                     uint32_t w[B4 + 1];
 #pragma unroll
                     for (int k = 0; k <= B4; ++k) w[k] = wr[k];
@@ -486,7 +500,13 @@ extern "C" umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uin
     const int idx_bits = umi::bits_for((uint32_t)(S * S - 1));
     if (umi::bits_for(255u * B * B) + idx_bits > 32)
         return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_listed_u8: key overflow");
-    const int ws4 = umi::ceil_div(B + 2 * R + 4, 4), wrows = B + 2 * R;
+    // (R % 4 == 0 keeps x0 - R - floor16(x0 - R) a whole number of dwords)
+    const bool fast16 = ((reinterpret_cast<uintptr_t>(cur) | reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) &
+                         15) == 0 &&
+                        W >= 16 && R % 4 == 0;
+    // fast16: up to 12 bytes before x0 - R and whole 16-byte chunks
+    const int ws4 = fast16 ? 4 * umi::ceil_div(12 + B + 2 * R + 4, 16) : umi::ceil_div(B + 2 * R + 4, 4);
+    const int wrows = B + 2 * R;
     const size_t lds = 4 * ((size_t)wrows * ws4 + (size_t)B * (B / 4) + 1);
     // split the vertical offsets into chunks so that G x chunks threads fit 512
     // (256 for B = 16, whose cur rows take 64 registers)
@@ -497,9 +517,9 @@ extern "C" umi_status umi_me_full_search_listed_u8(const uint8_t *cur, const uin
     hipStream_t st = umi::as_stream(s);
     if (B == 8)
         hipLaunchKernelGGL((me_listed_kernel<8>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R, W / B,
-                           G, DCH, ws4, wrows, idx_bits, list, count, mv_sad);
+                           G, DCH, ws4, wrows, idx_bits, (int)fast16, list, count, mv_sad);
     else
         hipLaunchKernelGGL((me_listed_kernel<16>), dim3(grid), dim3(threads), lds, st, cur, ref, H, W, pitch, R,
-                           W / B, G, DCH, ws4, wrows, idx_bits, list, count, mv_sad);
+                           W / B, G, DCH, ws4, wrows, idx_bits, (int)fast16, list, count, mv_sad);
     return umi::launch_check("me_listed_kernel");
 }
