@@ -143,6 +143,17 @@ def extra_configs(st):
     dl, dr, cv = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R), DeviceBuffer(375 * 1242 * 128 * 4)
     ms = timed(lambda: ops.cost_volume(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, cv, stream=st), st, 20)
     out["cost_volume_1242x375_d128_5x5"] = {"ms": ms, "gbs": (375 * 1242 * 128 * 4 + 2 * 375 * 1242) / (ms * 1e-3) / 1e9}
+    # --- stereo winner-take-all fused with the cost volume (the volume never reaches HBM);
+    # VALU-bound: 25 |a-b| per (pixel, shift) counted as the 5x5 window defines them
+    wta = DeviceBuffer(375 * 1242 * 8)
+    us = timed(lambda: ops.stereo_wta(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, wta, stream=st), st, 50) * 1e3
+    ok = bool(np.array_equal(wta.download((375, 1242, 2), np.int32)[..., 0],
+                             np.argmin(cv.download((375, 1242, 128), np.int32), axis=-1)))
+    absdiff = 375 * 1242 * 128 * 25
+    out["stereo_wta_1242x375_d128_5x5"] = {
+        "us": us, "vs_volume": ms * 1e3 / us, "gpx_s": 375 * 1242 / (us * 1e-6) / 1e9,
+        "t_absdiff_s": absdiff / (us * 1e-6) / 1e12, "valu_roof_t_absdiff_s": VALU_LANE_OPS * 4 / 1e12,
+        "frac": absdiff / (us * 1e-6) / (VALU_LANE_OPS * 4), "equals_volume_argmin": ok}
     # --- weighted 3x3 filter
     img = g.random((1024, 1024), dtype=np.float32)
     wt = g.normal(size=9)

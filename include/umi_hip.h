@@ -146,6 +146,21 @@ umi_status umi_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, 
                               int s0, int step, int KY, int KX, int32_t *out, umi_stream s);
 
 /* ---------------------------------------------------------------------------
+ * Stereo winner-take-all over the same cost volume, fused so that the volume
+ * never reaches device memory.  No reference counterpart: it consumes the
+ * volume that the MotionEst launch of main/cost_volume.cu:35-65 writes (the
+ * stereo matching step downstream of it).  Per pixel, with cost[y][x][i] as
+ * in umi_cost_volume_u8:
+ *
+ *   out[y][x][0] = the smallest i in [0, D) with cost[y][x][i] minimal,
+ *   out[y][x][1] = that cost.
+ *
+ * out: int32 [H][W][2].  0 <= KX, KY <= 3; step = +1 or -1; 1 <= D <= 1024.
+ * ------------------------------------------------------------------------- */
+umi_status umi_stereo_wta_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D,
+                             int s0, int step, int KY, int KX, int32_t *out, umi_stream s);
+
+/* ---------------------------------------------------------------------------
  * 3x3 filters.  Pitches are in elements.  mode: 0 = PAD (out-of-image taps
  * read `pad`, the reference's PAD boundary), 1 = clamp to edge (its WRAP).
  * Taps are accumulated dy outer, dx inner, starting from `init`; the weighted

@@ -18,6 +18,7 @@ _SIGS = {
     "oracle_me_full_search_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P],
     "oracle_sad_volume_i32": [_P, _P, _I, _I, _I, _I, _I, _P],
     "oracle_cost_volume_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "oracle_stereo_wta_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_filter3x3_i32": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _I],
     "oracle_filter3x3_f32": [_P, _I, _I, _I, _F, _F, _F, _I, _P, _I],
     "oracle_wfilter3x3_i32": [_P, _I, _I, _I, _P, _I, _I, _I, _P, _I],
@@ -79,6 +80,17 @@ def cost_volume(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     y0, y1 = y_range if y_range is not None else (0, H)
     out = np.zeros((H, W, D), np.int32)
     lib().oracle_cost_volume_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
+    return out
+
+
+def stereo_wta(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
+    """Per pixel (first minimising shift index, its cost) over the cost volume."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    y0, y1 = y_range if y_range is not None else (0, H)
+    out = np.zeros((H, W, 2), np.int32)
+    lib().oracle_stereo_wta_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
     return out
 
 

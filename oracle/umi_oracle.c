@@ -115,6 +115,40 @@ void oracle_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int
             }
 }
 
+/*
+ * Winner-take-all over the cost volume above, per pixel: the first i (in
+ * increasing order) whose cost is minimal, and that cost.  No loop of the
+ * reference computes it; it is the consumer of the volume that
+ * main/cost_volume.cu:35-65 writes.  Rows [y_begin, y_end); out[y][x] = (i,
+ * cost).
+ */
+void oracle_stereo_wta_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step,
+                          int KY, int KX, int y_begin, int y_end, int32_t *out) {
+    for (int y = y_begin; y < y_end; ++y)
+        for (int x = 0; x < W; ++x) {
+            int32_t best = 0;
+            int bi = -1;
+            for (int i = 0; i < D; ++i) {
+                const int s = s0 + step * i;
+                int32_t sad = 0;
+                for (int dy = -KY; dy <= KY; ++dy) {
+                    const int cy = clampi(y + dy, 0, H - 1);
+                    for (int dx = -KX; dx <= KX; ++dx) {
+                        const int lx = clampi(x + dx, 0, W - 1);
+                        const int rx = clampi(x + dx + s, 0, W - 1);
+                        sad += abs((int)L[(size_t)cy * pitch + lx] - (int)R[(size_t)cy * pitch + rx]);
+                    }
+                }
+                if (bi < 0 || sad < best) {
+                    best = sad;
+                    bi = i;
+                }
+            }
+            out[((size_t)y * W + x) * 2] = bi;
+            out[((size_t)y * W + x) * 2 + 1] = best;
+        }
+}
+
 /* Reads in[y][x] or applies the boundary: mode 0 -> pad value, 1 -> clamp
  * (the PAD / "WRAP"=clamp modes of src/UMI.h:205-218). */
 #define BOUNDED(T, in, H, W, ip, y, x, mode, pad) \

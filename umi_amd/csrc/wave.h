@@ -10,6 +10,7 @@ constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, 
 constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 constexpr int DPP_QUAD_XOR1 = 0xb1, DPP_QUAD_XOR2 = 0x4e;  // quad_perm [1,0,3,2], [2,3,0,1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;                 // lane i <- lane 7 - i within 8
+constexpr int DPP_ROW_ROR8 = 0x128;                        // lane i <- lane (i + 8) mod 16 within a row
 
 template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
 __device__ inline uint32_t dpp_u32(uint32_t v) {

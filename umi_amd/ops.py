@@ -70,6 +70,28 @@ def cost_volume_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int 
     return out.download((H, W, D), np.int32)
 
 
+def stereo_wta(L: DeviceBuffer, R: DeviceBuffer, H: int, W: int, pitch: int, D: int, s0: int, step: int,
+               KY: int, KX: int, out: DeviceBuffer, stream=None) -> None:
+    """Winner-take-all shift per pixel over the cost volume of `cost_volume`
+    (first minimum over i), without storing the volume: out int32 [H][W][2]
+    = (i, cost)."""
+    if out.nbytes < H * W * 8:
+        raise ValueError("winner-take-all buffer too small")
+    if L.nbytes < (H - 1) * pitch + W or R.nbytes < (H - 1) * pitch + W:
+        raise ValueError("image buffer too small")
+    call("umi_stereo_wta_u8", L.ptr, R.ptr, H, W, pitch, D, s0, step, KY, KX, out.ptr, _stream(stream))
+
+
+def stereo_wta_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int = -1, KY: int = 2,
+                  KX: int = 2) -> np.ndarray:
+    assert L.dtype == np.uint8 and R.dtype == np.uint8 and L.shape == R.shape
+    H, W = L.shape
+    dl, dr = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R)
+    out = DeviceBuffer(H * W * 8)
+    stereo_wta(dl, dr, H, W, W, D, s0, step, KY, KX, out)
+    return out.download((H, W, 2), np.int32)
+
+
 # ---------------------------------------------------------------------------
 # 3x3 filters (main/filter3x3.cu, main/wfilter3x3.cu) and bilateral
 # (main/bilateral.cu), johnjohnlin/UMI
