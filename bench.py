@@ -175,6 +175,17 @@ def extra_configs(st):
                                "bilateral_ms": t_bl, "bilateral_gpx_s": px / (t_bl * 1e-3) / 1e9,
                                "iimg_u8_ms": t_iu, "iimg_u8_gbs": 5 * px / (t_iu * 1e-3) / 1e9,
                                "iimg_f32_ms": t_if, "iimg_f32_gbs": 8 * px / (t_if * 1e-3) / 1e9}
+    # --- box sums read from the integral images (9x9 windows): one 1080p image and the batch
+    bu.iimg(wsu, st)
+    bbox = DeviceBuffer(px * 4)
+    t_bx = batch.time_graph(lambda s: bu.box_sum(4, 4, bbox, s), st)
+    one = g.integers(0, 256, size=(1080, 1920), dtype=np.int64).astype(np.uint8)
+    d1, s1, o1 = DeviceBuffer.from_array(one), DeviceBuffer(1080 * 1920 * 4), DeviceBuffer(1080 * 1920 * 4)
+    ws1 = DeviceBuffer(max(ops.iimg_workspace_size(1080, 1920, np.uint8), 4))
+    ops.iimg(d1, 1080, 1920, 1920, s1, 1920, np.uint8, ws1, stream=st)
+    us1 = timed(lambda: ops.box_sum(s1, 1080, 1920, 1920, 4, 4, o1, 1920, stream=st), st, 100) * 1e3
+    out["box_sum_from_iimg_r4"] = {"one_1080p_us": us1, "one_1080p_gbs": 8 * 1080 * 1920 / (us1 * 1e-6) / 1e9,
+                                   "batch_ms": t_bx, "batch_gbs": 8 * px / (t_bx * 1e-3) / 1e9}
     lp = learned_proposer(st)
     if lp is not None:
         out["learned_proposal_me_1080p_r32"] = lp

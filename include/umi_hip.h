@@ -217,6 +217,18 @@ umi_status umi_iimg_f32(const float *in, int H, int W, int in_pitch, float *out,
                         size_t workspace_bytes, umi_stream s);
 
 /* ---------------------------------------------------------------------------
+ * Box sums from an inclusive uint32 integral image (as umi_iimg_u8_u32
+ * writes it): out[y][x] = sum of the source image over rows
+ * [max(y-ry,0), min(y+ry,H-1)] and columns [max(x-rx,0), min(x+rx,W-1)],
+ * from the four corners of the integral image, unsigned arithmetic mod 2^32
+ * (exact whenever the window sum is below 2^32).  No reference counterpart:
+ * a consumer of the integral image that main/iimg.cu:37-82 computes.
+ * Pitches in elements; ry, rx >= 0.
+ * ------------------------------------------------------------------------- */
+umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch, int ry, int rx, uint32_t *out,
+                                int out_pitch, umi_stream s);
+
+/* ---------------------------------------------------------------------------
  * Dense contractions on the matrix cores.  The reference runs GEMM and
  * direct convolution as scalar UMI launches of its Gemm strategy
  * (main/gemm.cu:8-15,37-67; main/fprop.cu:10-17,53-82); these entry points
@@ -284,6 +296,10 @@ umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, 
                                  size_t workspace_bytes, umi_stream s);
 umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
                               umi_stream s);
+/* Box sums (as umi_box_sum_iimg_u32) of every image of the batch from its
+ * integral image; `iimg` and `out` both use the batch's output offsets and
+ * pitches.  One persistent launch; no counterpart in the reference. */
+umi_status umi_batch_box_sum_u32(umi_batch b, const uint32_t *iimg, int ry, int rx, uint32_t *out, umi_stream s);
 
 /* ---------------------------------------------------------------------------
  * Learned motion proposals (no reference counterpart: johnjohnlin/UMI has

@@ -19,6 +19,7 @@ _SIGS = {
     "oracle_sad_volume_i32": [_P, _P, _I, _I, _I, _I, _I, _P],
     "oracle_cost_volume_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_stereo_wta_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "oracle_box_sum_u8": [_P, _I, _I, _I, _I, _I, _P],
     "oracle_filter3x3_i32": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _I],
     "oracle_filter3x3_f32": [_P, _I, _I, _I, _F, _F, _F, _I, _P, _I],
     "oracle_wfilter3x3_i32": [_P, _I, _I, _I, _P, _I, _I, _I, _P, _I],
@@ -93,6 +94,14 @@ def stereo_wta(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     lib().oracle_stereo_wta_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
     return out
 
+
+def box_sum(img, ry, rx):
+    """Brute-force box sums (window clipped to the image) of a uint8 image, uint32."""
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    out = np.zeros((H, W), np.uint32)
+    lib().oracle_box_sum_u8(_ptr(img), H, W, W, ry, rx, _ptr(out))
+    return out
 
 def filter3x3(img, init, mul, pad, mode=0):
     dt = np.int32 if img.dtype == np.int32 else np.float32

@@ -149,6 +149,24 @@ void oracle_stereo_wta_u8(const uint8_t *L, const uint8_t *R, int H, int W, int 
         }
 }
 
+/*
+ * Box sums of a uint8 image by brute force: the sum over rows
+ * [max(y-ry,0), min(y+ry,H-1)] and columns [max(x-rx,0), min(x+rx,W-1)]
+ * (the window clipped to the image), uint32.  No loop of the reference
+ * computes it; it checks the box sums read from the integral image of
+ * main/iimg.cu:37-82.
+ */
+void oracle_box_sum_u8(const uint8_t *in, int H, int W, int ip, int ry, int rx, uint32_t *out) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            uint32_t s = 0;
+            for (int yy = y - ry < 0 ? 0 : y - ry; yy <= (y + ry > H - 1 ? H - 1 : y + ry); ++yy)
+                for (int xx = x - rx < 0 ? 0 : x - rx; xx <= (x + rx > W - 1 ? W - 1 : x + rx); ++xx)
+                    s += in[(size_t)yy * ip + xx];
+            out[(size_t)y * W + x] = s;
+        }
+}
+
 /* Reads in[y][x] or applies the boundary: mode 0 -> pad value, 1 -> clamp
  * (the PAD / "WRAP"=clamp modes of src/UMI.h:205-218). */
 #define BOUNDED(T, in, H, W, ip, y, x, mode, pad) \

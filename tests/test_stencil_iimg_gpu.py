@@ -130,3 +130,55 @@ def test_iimg_f32_fp64_column_pass(gpu, H, W):
     rel = np.abs(out.astype(np.float64) - ref) / ref
     print(f"[iimg] f32 {H}x{W} max rel {rel.max():.3e} mean {rel.mean():.3e}", flush=True)
     assert rel.max() <= 4e-6
+
+
+# ---------------------------------------------------------------------------
+# Box sums read from the integral image (umi_box_sum_iimg_u32, batch form)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("H,W", [(1, 1), (3, 5), (17, 70), (64, 64), (200, 333)])
+@pytest.mark.parametrize("ry,rx", [(0, 0), (1, 2), (4, 4), (100, 3)])
+def test_box_sum_from_gpu_iimg(gpu, H, W, ry, rx):
+    img = workloads.rng(H * 7 + W + ry).integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    print(f"[box] {H}x{W} r=({ry},{rx})", flush=True)
+    S = ops.iimg_np(img)
+    np.testing.assert_array_equal(ops.box_sum_np(S, ry, rx), oracle.box_sum(img, ry, rx))
+
+
+def test_box_sum_wraps_mod_2_32(gpu):
+    # an arbitrary uint32 table: every output is the corner formula mod 2^32
+    g = workloads.rng(5)
+    H, W, ry, rx = 37, 90, 3, 6
+    S = g.integers(0, 2**32, size=(H, W), dtype=np.uint64).astype(np.uint32)
+    print("[box] wrap", flush=True)
+    got = ops.box_sum_np(S, ry, rx)
+    Sp = np.zeros((H + 1, W + 1), np.uint32)
+    Sp[1:, 1:] = S
+    y = np.arange(H)[:, None]
+    x = np.arange(W)[None, :]
+    y1, y2 = np.maximum(y - ry, 0), np.minimum(y + ry, H - 1) + 1
+    x1, x2 = np.maximum(x - rx, 0), np.minimum(x + rx, W - 1) + 1
+    with np.errstate(over="ignore"):
+        ref = Sp[y2, x2] - Sp[y1, x2] - Sp[y2, x1] + Sp[y1, x1]
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_box_sum_1080p(gpu):
+    img = workloads.rng(9).integers(0, 256, size=(1080, 1920), dtype=np.int64).astype(np.uint8)
+    print("[box] 1080p r=4", flush=True)
+    np.testing.assert_array_equal(ops.box_sum_np(ops.iimg_np(img), 4, 4), oracle.box_sum(img, 4, 4))
+
+
+def test_batch_box_sum(gpu):
+    from umi_amd import batch
+
+    _, u8 = workloads.uneven_batch(n=20, lo=30, hi=400, seed=8)
+    b = batch.ImageBatch(u8, out_dtype=np.uint32)
+    ws = b.iimg_workspace()
+    b.iimg(ws)
+    box = DeviceBuffer(b.total * 4)
+    print("[box] batch", flush=True)
+    b.box_sum(3, 5, box)
+    flat = box.download((b.total,), np.uint32)
+    for im, o in zip(u8, b.offsets):
+        h, w = im.shape
+        np.testing.assert_array_equal(flat[o:o + h * w].reshape(h, w), oracle.box_sum(im, 3, 5))

@@ -88,6 +88,11 @@ class ImageBatch:
         call(name, self.handle, self.inp.ptr, (out or self.out).ptr, workspace.ptr, workspace.nbytes,
              _stream(stream))
 
+    def box_sum(self, ry: int, rx: int, out: DeviceBuffer, stream=None, iimg: DeviceBuffer | None = None):
+        """Box sums of every image from its uint32 integral image (`iimg`,
+        default self.out, as `iimg` writes it) into `out` (same layout)."""
+        call("umi_batch_box_sum_u32", self.handle, (iimg or self.out).ptr, ry, rx, out.ptr, _stream(stream))
+
 
 def _cache_path() -> Path:
     p = os.environ.get("UMI_TUNE_CACHE")

@@ -77,6 +77,25 @@ __global__ __launch_bounds__(256) void batch_bilateral_kernel(const BatchImage *
     }
 }
 
+// Box sums from the batch's integral images: input and output both use the
+// output offsets and pitches (the layout the integral images were written
+// in).  Work items: BX_TX x BX_TY tiles.
+__global__ __launch_bounds__(256) void batch_box_sum_kernel(const BatchImage *__restrict__ imgs, int n,
+                                                            const int *__restrict__ prefix,
+                                                            const int *__restrict__ owner,
+                                                            const uint32_t *__restrict__ S, int ry, int rx,
+                                                            uint32_t *__restrict__ out) {
+    const int total = prefix[n];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int i = owner[t];
+        const BatchImage im = imgs[i];
+        const int local = t - prefix[i], tiles_x = ceil_div(im.W, BX_TX);
+        const int ty = local / tiles_x, tx = local - ty * tiles_x;
+        box_sum_tile(S + im.out_off, im.H, im.W, im.out_pitch, ry, rx, out + im.out_off, im.out_pitch, tx * BX_TX,
+                     ty * BX_TY);
+    }
+}
+
 // Integral image (the passes of iimg_dev.h).  Passes A and C: one workgroup
 // per tile of every image; pass B: one thread per carry item (rows, columns,
 // tile columns) of every image.
@@ -147,6 +166,8 @@ struct umi_batch_s {
     int ftotal[NF] = {}, btotal[NB] = {};
     int *itiles = nullptr, *icarry = nullptr;  // integral image tiles and carry items
     int *iown = nullptr, *cown = nullptr;      // image of every tile, of every 64 carry items
+    int *xtiles = nullptr, *xown = nullptr;    // box-sum tiles and their images
+    int nxtiles = 0;
     int nitiles = 0, nicarry = 0;
     long long *ws_off = nullptr;
     long long ws_elems = 0;
@@ -225,7 +246,7 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         return set_error(UMI_ERR_INVALID, "umi_batch_create: n=%d or null arrays", n);
     auto *b = new umi_batch_s;
     b->n = n;
-    std::vector<int> fp[NF], bp[NB], itiles(n + 1, 0), icarry(n + 1, 0);
+    std::vector<int> fp[NF], bp[NB], itiles(n + 1, 0), icarry(n + 1, 0), xtiles(n + 1, 0);
     for (int c = 0; c < NF; ++c) fp[c].assign(n + 1, 0);
     for (int c = 0; c < NB; ++c) bp[c].assign(n + 1, 0);
     std::vector<long long> wso(n, 0);
@@ -241,6 +262,7 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         for (int c = 0; c < NF; ++c) fp[c][i + 1] = fp[c][i] + ceil_div(W[i], FTX[c]) * ceil_div(H[i], FTY[c]);
         for (int c = 0; c < NB; ++c) bp[c][i + 1] = bp[c][i] + ceil_div(W[i], BTX[c]) * ceil_div(H[i], BTY[c]);
         itiles[i + 1] = itiles[i] + g.ntx * g.nty;
+        xtiles[i + 1] = xtiles[i] + ceil_div(W[i], BX_TX) * ceil_div(H[i], BX_TY);
         icarry[i + 1] = icarry[i] + iimg_carry_items(H[i], W[i]);
         wso[i] = ws;
         ws += iimg_ws_elems(H[i], W[i]);
@@ -267,6 +289,9 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     for (int c = 0; c < NB && st == UMI_OK; ++c) st = upload(b, owners(bp[c], 1), &b->bown[c]);
     if (st == UMI_OK) st = upload(b, owners(itiles, 1), &b->iown);
     if (st == UMI_OK) st = upload(b, owners(icarry, 64), &b->cown);
+    if (st == UMI_OK) st = upload(b, xtiles, &b->xtiles);
+    if (st == UMI_OK) st = upload(b, owners(xtiles, 1), &b->xown);
+    b->nxtiles = xtiles[n];
     if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
     b->nitiles = itiles[n];
     b->nicarry = icarry[n];
@@ -341,6 +366,14 @@ umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, 
 umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
                               umi_stream s) {
     return launch_iimg<IF32>(b, in, out, workspace, workspace_bytes, as_stream(s), "umi_batch_iimg_f32");
+}
+
+umi_status umi_batch_box_sum_u32(umi_batch b, const uint32_t *iimg, int ry, int rx, uint32_t *out, umi_stream s) {
+    UMI_TRY(batch_check(b, iimg, out, "umi_batch_box_sum_u32"));
+    if (ry < 0 || rx < 0) return set_error(UMI_ERR_INVALID, "umi_batch_box_sum_u32: radii %d %d", ry, rx);
+    hipLaunchKernelGGL(batch_box_sum_kernel, dim3(grid_for(b->nxtiles)), dim3(256), 0, as_stream(s), b->imgs, b->n,
+                       b->xtiles, b->xown, iimg, ry, rx, out);
+    return launch_check("batch_box_sum_kernel");
 }
 
 }  // extern "C"

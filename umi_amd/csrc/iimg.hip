@@ -85,6 +85,11 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     return rc;
 }
 
+__global__ __launch_bounds__(256) void box_sum_kernel(const uint32_t *__restrict__ S, int H, int W, int ip, int ry,
+                                                      int rx, uint32_t *__restrict__ out, int op) {
+    box_sum_tile(S, H, W, ip, ry, rx, out, op, blockIdx.x * BX_TX, blockIdx.y * BX_TY);
+}
+
 }  // namespace
 
 extern "C" {
@@ -114,6 +119,17 @@ umi_status umi_iimg_i32(const int32_t *in, int H, int W, int in_pitch, int32_t *
 umi_status umi_iimg_f32(const float *in, int H, int W, int in_pitch, float *out, int out_pitch, void *workspace,
                         size_t workspace_bytes, umi_stream s) {
     return iimg_launch<IF32>(in, H, W, in_pitch, out, out_pitch, workspace, workspace_bytes, s, "umi_iimg_f32");
+}
+
+umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch, int ry, int rx, uint32_t *out,
+                                int out_pitch, umi_stream s) {
+    if (!iimg || !out) return set_error(UMI_ERR_INVALID, "umi_box_sum_iimg_u32: null pointer");
+    if (H < 1 || W < 1 || in_pitch < W || out_pitch < W || ry < 0 || rx < 0)
+        return set_error(UMI_ERR_INVALID, "umi_box_sum_iimg_u32: H=%d W=%d pitches %d %d radii %d %d", H, W, in_pitch,
+                         out_pitch, ry, rx);
+    hipLaunchKernelGGL(box_sum_kernel, dim3(ceil_div(W, BX_TX), ceil_div(H, BX_TY)), dim3(256), 0, as_stream(s), iimg,
+                       H, W, in_pitch, ry, rx, out, out_pitch);
+    return launch_check("box_sum_kernel");
 }
 
 }  // extern "C"

@@ -277,4 +277,33 @@ __device__ inline void iimg_tile_out(typename T::Row *lds, typename T::Col *rl, 
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- box sums
+// A consumer of the integral image: the sum over the (2 ry + 1) x (2 rx + 1)
+// window around (y, x), clipped to the image, from the four corners of the
+// inclusive integral image S:
+//   out = S(y2, x2) - S(y1 - 1, x2) - S(y2, x1 - 1) + S(y1 - 1, x1 - 1),
+// y1 = max(y - ry, 0), y2 = min(y + ry, H - 1) (x likewise), S(-1, .) =
+// S(., -1) = 0; unsigned arithmetic, exact mod 2^32 (as the integral image).
+// A tile of BX_TX x BX_TY outputs per 256 threads: thread (t % 64) takes a
+// column, rows t / 64, + 4, ...; the four corner loads of a row are issued
+// together and a wave's loads cover 64 consecutive columns.
+constexpr int BX_TX = 64, BX_TY = 16;
+
+__device__ inline void box_sum_tile(const uint32_t *__restrict__ S, int H, int W, int ip, int ry, int rx,
+                                    uint32_t *__restrict__ out, int op, int x0, int y0) {
+    const int x = x0 + (int)(threadIdx.x & 63);
+    if (x >= W) return;
+    const int xa = max(x - rx, 0) - 1, xb = min(x + rx, W - 1);
+    const int xs = max(xa, 0);  // a valid column for the unconditional load
+    for (int r = (int)(threadIdx.x >> 6); r < BX_TY; r += 4) {
+        const int y = y0 + r;
+        if (y >= H) return;
+        const int ya = max(y - ry, 0) - 1, yb = min(y + ry, H - 1);
+        const uint32_t *rb = S + (size_t)yb * ip, *ra = S + (size_t)max(ya, 0) * ip;
+        const uint32_t a = rb[xb], b = rb[xs], c = ra[xb], d = ra[xs];
+        const uint32_t v = a - (xa >= 0 ? b : 0u) - (ya >= 0 ? c : 0u) + (xa >= 0 && ya >= 0 ? d : 0u);
+        out[(size_t)y * op + x] = v;
+    }
+}
+
 }  // namespace umi

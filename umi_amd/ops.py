@@ -186,6 +186,23 @@ def iimg_np(img: np.ndarray, use_workspace: bool = True) -> np.ndarray:
     return o.download((H, W), odt)
 
 
+def box_sum(S: DeviceBuffer, H: int, W: int, in_pitch: int, ry: int, rx: int, out: DeviceBuffer, out_pitch: int,
+            stream=None) -> None:
+    """Box sums over (2 ry + 1) x (2 rx + 1) windows clipped to the image, from
+    a uint32 integral image (mod 2^32)."""
+    if S.nbytes < ((H - 1) * in_pitch + W) * 4 or out.nbytes < ((H - 1) * out_pitch + W) * 4:
+        raise ValueError("box sum buffers too small")
+    call("umi_box_sum_iimg_u32", S.ptr, H, W, in_pitch, ry, rx, out.ptr, out_pitch, _stream(stream))
+
+
+def box_sum_np(S: np.ndarray, ry: int, rx: int) -> np.ndarray:
+    S = np.ascontiguousarray(S, np.uint32)
+    H, W = S.shape
+    d, o = DeviceBuffer.from_array(S), DeviceBuffer(H * W * 4)
+    box_sum(d, H, W, W, ry, rx, o, W)
+    return o.download((H, W), np.uint32)
+
+
 # ---------------------------------------------------------------------------
 # MFMA GEMM / convolution (main/gemm.cu, main/fprop.cu), johnjohnlin/UMI
 # ---------------------------------------------------------------------------
