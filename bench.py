@@ -104,6 +104,18 @@ def extra_configs(st):
 
     out = {}
     g = workloads.rng(3)
+    # --- quarter-pel refinement of the +-16 full search's vectors, 1080p
+    cur, ref_f, _ = workloads.me_pair(1080, 1920, 16, seed=2)
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref_f)
+    mvi, mvq = DeviceBuffer(32400 * 12), DeviceBuffer(32400 * 12)
+    t_fs = timed(lambda: ops.me_full_search(dc, dr, 1080, 1920, 1920, 8, 16, mvi, stream=st), st, 50)
+    t_sp = timed(lambda: ops.me_subpel_refine(dc, dr, 1080, 1920, 1920, mvi, mvq, stream=st), st, 50)
+    q = mvq.download((135, 240, 3), np.int32)
+    i = mvi.download((135, 240, 3), np.int32)
+    out["subpel_refine_1080p_after_r16"] = {
+        "us": t_sp * 1e3, "full_search_us": t_fs * 1e3, "blocks_per_s": 32400 / (t_sp * 1e-3),
+        "mean_sad_integer": float(i[..., 2].mean()), "mean_sad_quarter_pel": float(q[..., 2].mean()),
+        "blocks_moved": float(np.mean((q[..., 0] != 4 * i[..., 0]) | (q[..., 1] != 4 * i[..., 1])))}
     # --- GEMM 4096^3 (error on 64 sampled rows against FP64)
     M = N = K = 4096
     A64, B64 = g.standard_normal((M, K)), g.standard_normal((K, N))

@@ -307,6 +307,24 @@ umi_status umi_batch_box_sum_u32(umi_batch b, const uint32_t *iimg, int ry, int 
  * proposal pipeline and of its on-GPU training; umi_amd/proposer.py strings
  * them together.
  * ------------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------------
+ * Quarter-pel refinement of integer 8x8 block motion vectors (as the full
+ * search returns them), the step a video encoder runs before coding them.
+ * No reference counterpart (main/full_me.cu is integer-pel).  Reference
+ * samples at quarter-pel positions (qy, qx):
+ *   Y = floor(qy/4), fy = qy - 4Y (X, fx likewise),
+ *   v = ((4-fx)(4-fy) A + fx(4-fy) B + (4-fx)fy C + fx fy D + 8) >> 4,
+ *   A, B, C, D = ref at (Y,X), (Y,X+1), (Y+1,X), (Y+1,X+1), clamped to the frame.
+ * From c = 4 x (integer MV): the 8 half-pel neighbours c + 2 o, then the 8
+ * quarter-pel neighbours of the best (o in {-1,0,1}^2 \ {0}, raster order),
+ * each replacing the centre only when its SAD is smaller (ties keep the
+ * centre).  mv_in: int32 [H/8][W/8][3] (dy, dx, SAD); mv_out: the same in
+ * quarter pels (dy4, dx4, SAD at that position).  An NV12 frame's luma plane
+ * is a uint8 frame with its pitch.
+ * ------------------------------------------------------------------------- */
+umi_status umi_me_subpel_refine_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
+                                   const int32_t *mv_in, int32_t *mv_out, umi_stream s);
+
 /* Full search of the listed 8x8/16x16 blocks only (block index by*nbx+bx in
  * list[0 .. *count)), writing their (dy, dx, SAD) into mv_sad; semantics of
  * umi_me_full_search_u8.  max_count bounds *count. */

@@ -20,6 +20,7 @@ _SIGS = {
     "oracle_cost_volume_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_stereo_wta_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_box_sum_u8": [_P, _I, _I, _I, _I, _I, _P],
+    "oracle_me_subpel_u8": [_P, _P, _I, _I, _I, _P, _P],
     "oracle_filter3x3_i32": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _I],
     "oracle_filter3x3_f32": [_P, _I, _I, _I, _F, _F, _F, _I, _P, _I],
     "oracle_wfilter3x3_i32": [_P, _I, _I, _I, _P, _I, _I, _I, _P, _I],
@@ -94,6 +95,17 @@ def stereo_wta(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     lib().oracle_stereo_wta_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
     return out
 
+
+def me_subpel(cur, ref, mv):
+    """Quarter-pel refinement of integer 8x8 block vectors mv [H/8][W/8][3];
+    returns (dy, dx, SAD) in quarter pels."""
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    mv = np.ascontiguousarray(mv, np.int32)
+    H, W = cur.shape
+    out = np.zeros((H // 8, W // 8, 3), np.int32)
+    lib().oracle_me_subpel_u8(_ptr(cur), _ptr(ref), H, W, W, _ptr(mv), _ptr(out))
+    return out
 
 def box_sum(img, ry, rx):
     """Brute-force box sums (window clipped to the image) of a uint8 image, uint32."""

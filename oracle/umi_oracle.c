@@ -167,6 +167,66 @@ void oracle_box_sum_u8(const uint8_t *in, int H, int W, int ip, int ry, int rx, 
         }
 }
 
+/*
+ * Quarter-pel refinement of 8x8 block motion vectors (no loop of the
+ * reference computes it; it refines the integer vectors of the full search,
+ * main/full_me.cu:66-87 generalised).  Reference samples at quarter-pel
+ * positions by bilinear interpolation with rounding, coordinates clamped to
+ * the frame; from c = 4 * (integer MV): the 8 half-pel neighbours c + 2 o,
+ * then the 8 quarter-pel neighbours of the best, o in {-1,0,1}^2 \ {0} in
+ * raster order, each replacing the centre only when its SAD is smaller.
+ * mv_in / mv_out: int32 [H/8][W/8][3] = (dy, dx, SAD); mv_out in quarter pels.
+ */
+static int floor4(int q) { return q >= 0 ? q / 4 : -((-q + 3) / 4); }
+
+static int refq(const uint8_t *ref, int H, int W, int pitch, int qy, int qx) {
+    const int Y = floor4(qy), X = floor4(qx), fy = qy - 4 * Y, fx = qx - 4 * X;
+    const int y0 = clampi(Y, 0, H - 1), y1 = clampi(Y + 1, 0, H - 1);
+    const int x0 = clampi(X, 0, W - 1), x1 = clampi(X + 1, 0, W - 1);
+    const int A = ref[(size_t)y0 * pitch + x0], B = ref[(size_t)y0 * pitch + x1];
+    const int C = ref[(size_t)y1 * pitch + x0], D = ref[(size_t)y1 * pitch + x1];
+    return ((4 - fx) * (4 - fy) * A + fx * (4 - fy) * B + (4 - fx) * fy * C + fx * fy * D + 8) >> 4;
+}
+
+static int32_t subpel_sad(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int by, int bx, int vy,
+                          int vx) {
+    int32_t s = 0;
+    for (int py = 0; py < 8; ++py)
+        for (int px = 0; px < 8; ++px) {
+            const int y = 8 * by + py, x = 8 * bx + px;
+            s += abs((int)cur[(size_t)y * pitch + x] - refq(ref, H, W, pitch, 4 * y + vy, 4 * x + vx));
+        }
+    return s;
+}
+
+void oracle_me_subpel_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, const int32_t *mv_in,
+                         int32_t *mv_out) {
+    const int nby = H / 8, nbx = W / 8;
+    for (int by = 0; by < nby; ++by)
+        for (int bx = 0; bx < nbx; ++bx) {
+            const int32_t *m = mv_in + ((size_t)by * nbx + bx) * 3;
+            int cy = 4 * m[0], cx = 4 * m[1];
+            int32_t best = subpel_sad(cur, ref, H, W, pitch, by, bx, cy, cx);
+            for (int step = 2; step >= 1; step /= 2) {
+                const int oy = cy, ox = cx;
+                for (int k = 0; k < 9; ++k) {
+                    if (k == 4) continue;
+                    const int vy = oy + step * (k / 3 - 1), vx = ox + step * (k % 3 - 1);
+                    const int32_t s = subpel_sad(cur, ref, H, W, pitch, by, bx, vy, vx);
+                    if (s < best) {
+                        best = s;
+                        cy = vy;
+                        cx = vx;
+                    }
+                }
+            }
+            int32_t *o = mv_out + ((size_t)by * nbx + bx) * 3;
+            o[0] = cy;
+            o[1] = cx;
+            o[2] = best;
+        }
+}
+
 /* Reads in[y][x] or applies the boundary: mode 0 -> pad value, 1 -> clamp
  * (the PAD / "WRAP"=clamp modes of src/UMI.h:205-218). */
 #define BOUNDED(T, in, H, W, ip, y, x, mode, pad) \

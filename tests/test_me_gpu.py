@@ -145,3 +145,69 @@ def test_me_listed_vs_oracle(gpu, H, W, B, R, nlist):
     listed[lst] = True
     np.testing.assert_array_equal(got[listed], want[listed])
     np.testing.assert_array_equal(got[~listed], init[~listed])
+
+
+# ---------------------------------------------------------------------------
+# Quarter-pel refinement of the integer vectors (umi_me_subpel_refine_u8)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("H,W,R", [(64, 96, 8), (120, 200, 16)])
+def test_subpel_after_full_search(gpu, H, W, R):
+    g = workloads.rng(H + W)
+    ref = workloads.smooth_noise_u8(H, W, g)
+    cur = np.roll(ref, (3, -5), axis=(0, 1))
+    cur = np.clip(cur.astype(int) + g.normal(0, 2, cur.shape).round().astype(int), 0, 255).astype(np.uint8)
+    print(f"[subpel] {H}x{W} R={R}", flush=True)
+    mv, _ = ops.me_full_search_np(cur, ref, 8, R)
+    got = ops.me_subpel_refine_np(cur, ref, mv)
+    np.testing.assert_array_equal(got, oracle.me_subpel(cur, ref, mv))
+    assert (got[..., 2] <= mv[..., 2]).all()
+
+
+def test_subpel_arbitrary_vectors_and_edges(gpu):
+    # vectors pointing past every frame edge: the clamp inside the interpolation
+    g = workloads.rng(31)
+    H, W = 56, 72
+    ref = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    cur = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    mv = np.zeros((H // 8, W // 8, 3), np.int32)
+    mv[..., :2] = g.integers(-20, 21, size=(H // 8, W // 8, 2))
+    print("[subpel] arbitrary vectors", flush=True)
+    np.testing.assert_array_equal(ops.me_subpel_refine_np(cur, ref, mv), oracle.me_subpel(cur, ref, mv))
+
+
+def test_subpel_quarter_shift_known_answer(gpu):
+    from test_oracle_subpel import shifted  # the test directory is on sys.path
+
+    ref = workloads.smooth_noise_u8(64, 96, workloads.rng(6))
+    cur = shifted(ref, 1, -3)
+    mv = np.zeros((8, 12, 3), np.int32)
+    mv[..., 1] = -1
+    print("[subpel] known answer", flush=True)
+    out = ops.me_subpel_refine_np(cur, ref, mv)
+    assert (out[..., 0] == 1).all() and (out[..., 1] == -3).all() and (out[..., 2] == 0).all()
+
+
+def test_subpel_1080p_and_nv12_luma(gpu):
+    # full 1080p pair; the same search on the luma plane of NV12 buffers (Y
+    # plane, then the interleaved UV plane) through their pitch gives the same
+    from umi_amd import DeviceBuffer
+
+    H, W = 1080, 1920
+    cur, ref, _ = workloads.me_pair(H, W, seed=3)
+    print("[subpel] 1080p", flush=True)
+    mv, _ = ops.me_full_search_np(cur, ref, 8, 16)
+    got = ops.me_subpel_refine_np(cur, ref, mv)
+    np.testing.assert_array_equal(got, oracle.me_subpel(cur, ref, mv))
+    P = W + 64
+    nv = []
+    for f in (cur, ref):
+        buf = np.zeros((H + H // 2, P), np.uint8)
+        buf[:H, :W] = f
+        buf[H:, :W] = 128
+        nv.append(DeviceBuffer.from_array(buf))
+    mv_d, out_d = DeviceBuffer(mv.size * 4), DeviceBuffer(mv.size * 4)
+    print("[subpel] NV12 luma planes", flush=True)
+    ops.me_full_search(nv[0], nv[1], H, W, P, 8, 16, mv_d)
+    ops.me_subpel_refine(nv[0], nv[1], H, W, P, mv_d, out_d)
+    np.testing.assert_array_equal(mv_d.download(mv.shape, np.int32), mv)
+    np.testing.assert_array_equal(out_d.download(mv.shape, np.int32), got)

@@ -48,6 +48,24 @@ def me_full_search_np(cur: np.ndarray, ref: np.ndarray, B: int = 8, R: int = 16,
     return out_mv, out_vol
 
 
+def me_subpel_refine(cur: DeviceBuffer, ref: DeviceBuffer, H: int, W: int, pitch: int, mv_in: DeviceBuffer,
+                     mv_out: DeviceBuffer, stream=None) -> None:
+    """Quarter-pel refinement of integer 8x8 block vectors (dy, dx, SAD) ->
+    (dy4, dx4, SAD) in quarter pels."""
+    nb = (H // 8) * (W // 8)
+    if mv_in.nbytes < nb * 12 or mv_out.nbytes < nb * 12:
+        raise ValueError("motion vector buffers too small")
+    call("umi_me_subpel_refine_u8", cur.ptr, ref.ptr, H, W, pitch, mv_in.ptr, mv_out.ptr, _stream(stream))
+
+
+def me_subpel_refine_np(cur: np.ndarray, ref: np.ndarray, mv: np.ndarray) -> np.ndarray:
+    H, W = cur.shape
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    di, do = DeviceBuffer.from_array(np.ascontiguousarray(mv, np.int32)), DeviceBuffer(max(mv.size * 4, 4))
+    me_subpel_refine(dc, dr, H, W, W, di, do)
+    return do.download(mv.shape, np.int32)
+
+
 # ---------------------------------------------------------------------------
 # SAD cost volume (main/cost_volume.cu:35-80, johnjohnlin/UMI)
 # ---------------------------------------------------------------------------
