@@ -27,6 +27,14 @@ struct BatchImage {
     int H, W, in_pitch, out_pitch;
 };
 
+// One integral-image tile of the batch: everything its passes need, so that
+// a tile costs one (wave-uniform) descriptor load instead of a chain of
+// owner -> image -> prefix -> workspace-offset loads.
+struct IimgTile {
+    long long in_off, out_off, ws_off;
+    int H, W, in_pitch, out_pitch, tx, ty;
+};
+
 // 3x3 filter tile shapes (TX, TY) and bilateral tile shapes (TX x TY = 256).
 constexpr int NF = 4, NB = 4;
 constexpr int FTX[NF] = {64, 64, 128, 32}, FTY[NF] = {32, 16, 16, 32};
@@ -96,27 +104,38 @@ __global__ __launch_bounds__(256) void batch_box_sum_kernel(const BatchImage *__
     }
 }
 
-// Integral image (the passes of iimg_dev.h).  Passes A and C: one workgroup
-// per tile of every image; pass B: one thread per carry item (rows, columns,
-// tile columns) of every image.
+// Integral image (the passes of iimg_dev.h).  Passes A and C: a workgroup
+// per tile of every image at a time; pass B: one thread per carry item (rows,
+// columns, tile columns) of every image.
+
+// Persistent loop over the tile descriptors: the next tile's descriptor is
+// loaded before the current tile is processed.
+template <class F>
+__device__ inline void for_each_tile(const IimgTile *__restrict__ tiles, int total, F &&body) {
+    int t = blockIdx.x;
+    if (t >= total) return;
+    IimgTile d = tiles[t];
+    for (;;) {
+        const int tn = t + (int)gridDim.x;
+        const IimgTile dn = tiles[tn < total ? tn : t];
+        body(d);
+        if (tn >= total) return;
+        t = tn;
+        d = dn;
+    }
+}
+
 template <class T>
-__global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                                       const int *__restrict__ tile_prefix,
-                                                                       const int *__restrict__ owner,
-                                                                       const long long *__restrict__ ws_off,
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const IimgTile *__restrict__ tiles, int total,
                                                                        const typename T::In *__restrict__ in,
                                                                        typename T::Col *__restrict__ ws) {
     __shared__ typename T::Row lds[IM_TH * IM_TW];
     __shared__ typename T::Col tmp[4];
-    const int total = tile_prefix[n];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = owner[t];
-        const BatchImage im = imgs[i];
-        const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
-        const typename T::In *src = in + im.in_off;
-        iimg_tile_totals<T>(lds, tmp, src, im.H, im.W, im.in_pitch, iimg_rows16(src, im.in_pitch), local % ntx,
-                            local / ntx, iimg_ws<T>(ws + ws_off[i], im.H, im.W));
-    }
+    for_each_tile(tiles, total, [&](const IimgTile &d) {
+        const typename T::In *src = in + d.in_off;
+        iimg_tile_totals<T>(lds, tmp, src, d.H, d.W, d.in_pitch, iimg_rows16(src, d.in_pitch), d.tx, d.ty,
+                            iimg_ws<T>(ws + d.ws_off, d.H, d.W));
+    });
 }
 
 template <class T>
@@ -134,24 +153,17 @@ __global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const BatchImage 
 }
 
 template <class T>
-__global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                                    const int *__restrict__ tile_prefix,
-                                                                    const int *__restrict__ owner,
-                                                                    const long long *__restrict__ ws_off,
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const IimgTile *__restrict__ tiles, int total,
                                                                     const typename T::In *__restrict__ in,
                                                                     typename T::Col *__restrict__ ws,
                                                                     typename T::Out *__restrict__ out) {
     __shared__ typename T::Row lds[IM_TH * IM_TW];
     __shared__ typename T::Col rl[IM_TH];
-    const int total = tile_prefix[n];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int i = owner[t];
-        const BatchImage im = imgs[i];
-        const int ntx = iimg_geom(im.H, im.W).ntx, local = t - tile_prefix[i];
-        const typename T::In *src = in + im.in_off;
-        iimg_tile_out<T>(lds, rl, src, im.H, im.W, im.in_pitch, iimg_rows16(src, im.in_pitch), local % ntx,
-                         local / ntx, iimg_ws<T>(ws + ws_off[i], im.H, im.W), out + im.out_off, im.out_pitch);
-    }
+    for_each_tile(tiles, total, [&](const IimgTile &d) {
+        const typename T::In *src = in + d.in_off;
+        iimg_tile_out<T>(lds, rl, src, d.H, d.W, d.in_pitch, iimg_rows16(src, d.in_pitch), d.tx, d.ty,
+                         iimg_ws<T>(ws + d.ws_off, d.H, d.W), out + d.out_off, d.out_pitch);
+    });
 }
 
 }  // namespace
@@ -164,9 +176,10 @@ struct umi_batch_s {
     int *ftiles[NF] = {}, *btiles[NB] = {};
     int *fown[NF] = {}, *bown[NB] = {};  // owner tables: the image of every tile
     int ftotal[NF] = {}, btotal[NB] = {};
-    int *itiles = nullptr, *icarry = nullptr;  // integral image tiles and carry items
-    int *iown = nullptr, *cown = nullptr;      // image of every tile, of every 64 carry items
+    int *icarry = nullptr;                     // integral image carry items (prefix over images)
+    int *cown = nullptr;                       // image of every 64 carry items
     int *xtiles = nullptr, *xown = nullptr;    // box-sum tiles and their images
+    IimgTile *idesc = nullptr;                 // integral-image tile descriptors
     int nxtiles = 0;
     int nitiles = 0, nicarry = 0;
     long long *ws_off = nullptr;
@@ -222,14 +235,14 @@ umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *o
     const size_t need = (size_t)b->ws_elems * sizeof(typename T::Col);
     if (need && (!ws || ws_bytes < need)) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
     auto *w = static_cast<typename T::Col *>(ws);
-    hipLaunchKernelGGL(batch_iimg_totals_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs,
-                       b->n, b->itiles, b->iown, b->ws_off, in, w);
+    hipLaunchKernelGGL(batch_iimg_totals_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->idesc,
+                       b->nitiles, in, w);
     UMI_TRY(launch_check("batch_iimg_totals_kernel"));
     hipLaunchKernelGGL(batch_iimg_carry_kernel<T>, dim3(grid_for(ceil_div(b->nicarry, 256))), dim3(256), 0, st,
                        b->imgs, b->n, b->icarry, b->cown, b->ws_off, w);
     UMI_TRY(launch_check("batch_iimg_carry_kernel"));
-    hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->imgs, b->n,
-                       b->itiles, b->iown, b->ws_off, in, w, out);
+    hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->idesc,
+                       b->nitiles, in, w, out);
     UMI_TRY(launch_check("batch_iimg_out_kernel"));
     return UMI_OK;
 }
@@ -250,6 +263,7 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     for (int c = 0; c < NF; ++c) fp[c].assign(n + 1, 0);
     for (int c = 0; c < NB; ++c) bp[c].assign(n + 1, 0);
     std::vector<long long> wso(n, 0);
+    std::vector<IimgTile> idesc;
     long long ws = 0;
     for (int i = 0; i < n; ++i) {
         if (H[i] < 1 || W[i] < 1 || in_pitch[i] < W[i] || out_pitch[i] < W[i] || in_off[i] < 0 || out_off[i] < 0) {
@@ -265,6 +279,9 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         xtiles[i + 1] = xtiles[i] + ceil_div(W[i], BX_TX) * ceil_div(H[i], BX_TY);
         icarry[i + 1] = icarry[i] + iimg_carry_items(H[i], W[i]);
         wso[i] = ws;
+        for (int ty = 0; ty < g.nty; ++ty)
+            for (int tx = 0; tx < g.ntx; ++tx)
+                idesc.push_back({in_off[i], out_off[i], ws, H[i], W[i], in_pitch[i], out_pitch[i], tx, ty});
         ws += iimg_ws_elems(H[i], W[i]);
     }
     umi_status st = upload(b, b->host, &b->imgs);
@@ -276,7 +293,6 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         st = upload(b, bp[c], &b->btiles[c]);
         b->btotal[c] = bp[c][n];
     }
-    if (st == UMI_OK) st = upload(b, itiles, &b->itiles);
     if (st == UMI_OK) st = upload(b, icarry, &b->icarry);
     // owner tables: item t of a list belongs to image owner[t]
     auto owners = [&](const std::vector<int> &pre, int per) {
@@ -287,12 +303,12 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     };
     for (int c = 0; c < NF && st == UMI_OK; ++c) st = upload(b, owners(fp[c], 1), &b->fown[c]);
     for (int c = 0; c < NB && st == UMI_OK; ++c) st = upload(b, owners(bp[c], 1), &b->bown[c]);
-    if (st == UMI_OK) st = upload(b, owners(itiles, 1), &b->iown);
     if (st == UMI_OK) st = upload(b, owners(icarry, 64), &b->cown);
     if (st == UMI_OK) st = upload(b, xtiles, &b->xtiles);
     if (st == UMI_OK) st = upload(b, owners(xtiles, 1), &b->xown);
     b->nxtiles = xtiles[n];
     if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
+    if (st == UMI_OK) st = upload(b, idesc, &b->idesc);
     b->nitiles = itiles[n];
     b->nicarry = icarry[n];
     b->ws_elems = ws;

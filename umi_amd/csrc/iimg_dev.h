@@ -87,6 +87,58 @@ __device__ inline void iimg_load16(typename T::Row (&v)[16], const typename T::I
         const In *e = reinterpret_cast<const In *>(q);
 #pragma unroll
         for (int j = 0; j < 16; ++j) v[j] = (Row)e[j];
+    } else if constexpr (sizeof(In) == 1) {
+        // bytes of a row that is not 16-byte aligned: the five aligned dwords
+        // around them, each clamped to the dword holding the row's last byte
+        // (so every load holds at least one byte of the row), then v_alignbyte;
+        // bytes past W and rows past H are zero
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (y < H && x < W) {
+            const uintptr_t p = reinterpret_cast<uintptr_t>(in + (size_t)y * ip + x);
+            const uintptr_t last = reinterpret_cast<uintptr_t>(in + (size_t)y * ip + W - 1) & ~(uintptr_t)3;
+            const uintptr_t a0 = p & ~(uintptr_t)3;
+            const uint32_t sh = (uint32_t)(p & 3);
+            uint32_t d[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) d[k] = *reinterpret_cast<const uint32_t *>(min(a0 + 4 * k, last));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const Row e = (Row)__builtin_amdgcn_ubfe(w[j >> 2], 8 * (j & 3), 8);
+            v[j] = x + j < W ? e : (Row)0;
+        }
+    } else if constexpr (sizeof(In) == 4) {
+        // 4-byte elements of a row that is not 16-byte aligned: the five
+        // aligned 16-byte chunks around them, each clamped to the chunk
+        // holding the row's last element, then a per-lane select of the
+        // starting element; elements past W and rows past H are zero
+        uint32_t w[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) w[j] = 0;
+        if (y < H && x < W) {
+            const uintptr_t p = reinterpret_cast<uintptr_t>(in + (size_t)y * ip + x);
+            const uintptr_t last = reinterpret_cast<uintptr_t>(in + (size_t)y * ip + W - 1) & ~(uintptr_t)15;
+            const uintptr_t a0 = p & ~(uintptr_t)15;
+            const int e = (int)((p & 15) >> 2);
+            uint32_t dw[20];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(min(a0 + 16 * k, last));
+                dw[4 * k] = q.x;
+                dw[4 * k + 1] = q.y;
+                dw[4 * k + 2] = q.z;
+                dw[4 * k + 3] = q.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) w[j] = e == 0 ? dw[j] : e == 1 ? dw[j + 1] : e == 2 ? dw[j + 2] : dw[j + 3];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const Row el = (Row)__builtin_bit_cast(In, w[j]);
+            v[j] = x + j < W ? el : (Row)0;
+        }
     } else {
         // clamped indices keep every load inside the image; the select zeroes the rest
         const In *row = in + (size_t)min(y, H - 1) * ip;
