@@ -269,6 +269,13 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
                           const void *in_hi, const void *in_lo, const void *w_hi, const void *w_lo, float *out,
                           const float *q_in, const float *q_w, umi_stream s);
 
+/* Selects the form of umi_conv_fprop for bf16, 3x3, stride 1, padding 1,
+ * C = K = 64, 8 <= W <= 64 (the batch-16 56x56 layer): 1 (default) = a direct
+ * convolution with the input tile staged once in LDS, 0 = the implicit GEMM.
+ * Both use the same operands and MFMA order and give identical results; the
+ * switch exists for measurement and tests.  No reference counterpart. */
+umi_status umi_conv_set_direct(int enable);
+
 /* ---------------------------------------------------------------------------
  * Batches of images of uneven sizes.  The reference runs one Execute per
  * image (src/UMI_gpu.h:409-623, e.g. main/wfilter3x3.cu:39,

@@ -50,6 +50,7 @@ SIGNATURES = {
     "umi_stereo_wta_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
     "umi_box_sum_iimg_u32": [P, c_int, c_int, c_int, c_int, c_int, P, c_int, P],
     "umi_me_subpel_refine_u8": [P, P, c_int, c_int, c_int, P, P, P],
+    "umi_conv_set_direct": [c_int],
     "umi_batch_box_sum_u32": [P, P, c_int, c_int, P, P],
     "umi_filter3x3_f32": [P, c_int, c_int, c_int, c_float, c_float, c_float, c_int, P, c_int, P],
     "umi_filter3x3_i32": [P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_int, P],
