@@ -150,24 +150,25 @@ def test_conv_config_precision(mode, gpu):
 
 @pytest.mark.parametrize("shape", [(16, 56, 56), (2, 56, 56), (3, 7, 9), (1, 64, 64), (2, 13, 8), (1, 3, 63), (2, 40, 33),
                                    (2, 13, 1), (1, 1, 5)])
-def test_conv_direct_equals_implicit_gemm(shape, gpu):
-    # the direct 3x3 form (C = K = 64, stride 1, pad 1, bf16) uses the implicit
-    # GEMM's operands and MFMA order: identical bits, and both within the bf16
-    # bound of the FP64 oracle
+@pytest.mark.parametrize("mode", [ops.BF16, ops.FP8], ids=["bf16", "fp8"])
+def test_conv_direct_equals_implicit_gemm(shape, mode, gpu):
+    # the direct 3x3 form (C = K = 64, stride 1, pad 1, bf16 / fp8) uses the
+    # implicit GEMM's operands, MFMA order and scaling: identical bits, and both
+    # within the mode's bound of the FP64 oracle
     from umi_amd._lib import call
 
     N, H, W = shape
     g = workloads.rng(N * 131 + H * 7 + W)
     x = g.standard_normal((N, 64, H, W)).astype(np.float32)
     w = g.standard_normal((64, 64, 3, 3)).astype(np.float32)
-    print(f"[conv] direct vs implicit GEMM {shape}", flush=True)
+    print(f"[conv] direct vs implicit GEMM {ops.MODE_NAMES[mode]} {shape}", flush=True)
     try:
         call("umi_conv_set_direct", 0)
-        ref = ops.conv_fprop_np(x, w, 1, 1, ops.BF16)
+        ref = ops.conv_fprop_np(x, w, 1, 1, mode)
     finally:
         call("umi_conv_set_direct", 1)
-    out = ops.conv_fprop_np(x, w, 1, 1, ops.BF16)
+    out = ops.conv_fprop_np(x, w, 1, 1, mode)
     np.testing.assert_array_equal(out, ref)
     f64 = oracle.conv_f64(x, w, 1, 1)
     scale = np.sqrt(64 * 9) * np.abs(x).max() * np.abs(w).max()
-    assert np.abs(out - f64).max() <= PRECISION[ops.BF16] * 40 * scale
+    assert np.abs(out - f64).max() <= PRECISION[mode] * 40 * scale

@@ -538,89 +538,121 @@ umi_status pack_common(int mode, const float *amax, void *hi, void *lo, const ch
     return UMI_OK;
 }
 
-// Direct 3x3 convolution, stride 1, padding 1, C = K = 64, bf16 (the
-// implicit GEMM's operands and MFMA order, so the same bits as
-// gemm_kernel<.., GM_BF16, DenseSrc, ConvSrc, ConvDst>).  A workgroup (14
-// waves) owns DC_PX = 224 consecutive output pixels of one image (the batch-16
-// 56x56 layer is 224 workgroups: one per CU, a single round).  It stages the
-// filter (64 rows of 576 k, KRSC order, at DC_WROW bytes per row) and the
-// input rows the pixels touch plus the padding (at most dc_rows(W) rows of
-// W + 2 pixels, 64 channels, at DC_PXB bytes per pixel) in LDS once; the
-// spare 16 bytes per row or pixel spread the 16-byte operand reads of
-// consecutive lanes over the banks.  Wave w computes pixels 32 (w % 7) .. + 31
-// for output channels 32 (w / 7) .. + 31: 36 MFMAs (k = 16 q .. 16 q + 15) on
-// LDS operands, no barrier in the K walk.
-constexpr int DC_GROUPS = 7, DC_PX = 32 * DC_GROUPS, DC_PXB = 144, DC_WROW = 1168, DC_THREADS = 128 * DC_GROUPS;
-constexpr int DC_WBYTES = 64 * DC_WROW;
+// Direct 3x3 convolution, stride 1, padding 1, C = K = 64, bf16 or fp8 (the
+// implicit GEMM's operands, MFMA order and scaling, so the same bits as
+// gemm_kernel<.., MODE, DenseSrc, ConvSrc, ConvDst>).  A workgroup (14 waves)
+// owns DC_PX = 224 consecutive output pixels of one image (the batch-16 56x56
+// layer is 224 workgroups: one per CU, a single round).  It stages the filter
+// (64 rows of 576 k, KRSC order) and the input rows the pixels touch plus the
+// padding (at most dc_rows(W) rows of W + 2 pixels of 64 channels) in LDS
+// once, each row or pixel followed by 16 spare bytes so that the operand
+// reads of consecutive lanes spread over the banks.  Wave w computes pixels
+// 32 (w % 7) .. + 31 for output channels 32 (w / 7) .. + 31: 36 bf16 MFMAs
+// (k = 16 q .. + 15) or 9 fp8 ones (k = 64 q .. + 63) on LDS operands, no
+// barrier in the K walk.
+constexpr int DC_GROUPS = 7, DC_PX = 32 * DC_GROUPS, DC_THREADS = 128 * DC_GROUPS;
 bool direct_conv_enabled = true;  // umi_conv_set_direct
 __host__ __device__ constexpr int dc_rows(int W) { return (DC_PX + W - 1) / W + 3; }
+template <bool FP8>
+struct DcGeom {
+    static constexpr int ESZ = FP8 ? 1 : 2, PIXB = 64 * ESZ, PXB = PIXB + 16, WROW = 576 * ESZ + 16;
+    static constexpr int WBYTES = 64 * WROW, WCHUNKS = 64 * 576 * ESZ / 16, LANEB = FP8 ? 32 : 16;
+    static constexpr int NQ = FP8 ? 9 : 36;
+};
+template <bool FP8>
+__host__ inline size_t dc_lds(int W) {
+    using G = DcGeom<FP8>;
+    return (size_t)G::WBYTES + (size_t)dc_rows(W) * (W + 2) * G::PXB;
+}
 
-__global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_bf16_kernel(const uint16_t *__restrict__ in,
-                                                                         const uint16_t *__restrict__ wt, int N,
-                                                                         int H, int W, float *__restrict__ out) {
+template <bool FP8>
+__global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_t *__restrict__ in,
+                                                                    const uint8_t *__restrict__ wt, int N, int H,
+                                                                    int W, const float *qw, const float *qi,
+                                                                    float *__restrict__ out) {
+    using G = DcGeom<FP8>;
     extern __shared__ __attribute__((aligned(16))) uint8_t dsm[];
-    uint8_t *wsm = dsm, *xsm = dsm + DC_WBYTES;
+    uint8_t *wsm = dsm, *xsm = dsm + G::WBYTES;
     const int HW = H * W, tiles = (HW + DC_PX - 1) / DC_PX;
     const int n = blockIdx.x / tiles, p0 = (blockIdx.x - n * tiles) * DC_PX;
     const int ra = p0 / W, rb = min(p0 + DC_PX - 1, HW - 1) / W;
     const int y0 = ra - 1, nrows = rb - ra + 3, W2 = W + 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 31, h = lane >> 5;
     const int g = wv % DC_GROUPS, hc = wv / DC_GROUPS;
-    // filter: 64 rows x 72 chunks of 16 bytes, all of a thread's loads issued first
+    // filter: 64 rows of 576 ESZ bytes in 16-byte chunks, all of a thread's loads issued first
     {
-        constexpr int U = (4608 + DC_THREADS - 1) / DC_THREADS;
+        constexpr int U = (G::WCHUNKS + DC_THREADS - 1) / DC_THREADS, RC = 576 * G::ESZ / 16;
         i32x4 v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int e = min(u * DC_THREADS + tid, 4607);  // past the end: chunk 4607 again
-            v[u] = *(const i32x4 *)(wt + (size_t)e * 8);
+            const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1);  // past the end: the last chunk again
+            v[u] = *(const i32x4 *)(wt + (size_t)e * 16);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int e = min(u * DC_THREADS + tid, 4607), row = e / 72;
-            *(i32x4 *)(wsm + row * DC_WROW + 16 * (e - row * 72)) = v[u];
+            const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1), row = e / RC;
+            *(i32x4 *)(wsm + row * G::WROW + 16 * (e - row * RC)) = v[u];
         }
     }
     // input: chunk e = (slot, 16-byte part c) of the nrows x W2 pixel slots
-    const uint16_t *img = in + (size_t)n * HW * 64;
-    const int nchunks = nrows * W2 * 8;
+    constexpr int PC = G::PIXB / 16;
+    const uint8_t *img = in + (size_t)n * HW * G::PIXB;
+    const int nchunks = nrows * W2 * PC;
     for (int e0 = 0; e0 < nchunks; e0 += 4 * DC_THREADS) {
         i32x4 v[4];
         int slot[4], cpart[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int e = e0 + u * DC_THREADS + tid;
-            slot[u] = e >> 3;
-            cpart[u] = e & 7;
+            slot[u] = e / PC;
+            cpart[u] = e - slot[u] * PC;
             const int iy = y0 + slot[u] / W2, ix = slot[u] % W2 - 1;
             const bool in_img = e < nchunks && iy >= 0 && iy < H && ix >= 0 && ix < W;
-            v[u] = in_img ? *(const i32x4 *)(img + ((size_t)iy * W + ix) * 64 + 8 * cpart[u]) : i32x4{0, 0, 0, 0};
+            v[u] = in_img ? *(const i32x4 *)(img + ((size_t)iy * W + ix) * G::PIXB + 16 * cpart[u])
+                          : i32x4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsm + slot[u] * DC_PXB + 16 * cpart[u]) = v[u];
+            if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsm + slot[u] * G::PXB + 16 * cpart[u]) = v[u];
     }
     __syncthreads();
     const int p = p0 + 32 * g + lr;
     const bool valid = p < HW;
     const int pp = valid ? p : p0;
     const int oy = pp / W, ox = pp - oy * W;
-    const uint8_t *xb = xsm + ((oy - y0) * W2 + ox + 1) * DC_PXB + 16 * h;
-    const uint8_t *wb = wsm + (32 * hc + lr) * DC_WROW + 16 * h;
+    const uint8_t *xb = xsm + ((oy - y0) * W2 + ox + 1) * G::PXB + G::LANEB * h;
+    const uint8_t *wb = wsm + (32 * hc + lr) * G::WROW + G::LANEB * h;
     f32x16 acc = {};
 #pragma unroll
-    for (int q = 0; q < 36; ++q) {
-        const int t = q >> 2, dy = t / 3 - 1, dx = t % 3 - 1;
-        const bf16x8 a = *(const bf16x8 *)(wb + 32 * q);
-        const bf16x8 b = *(const bf16x8 *)(xb + (dy * W2 + dx) * DC_PXB + 32 * (q & 3));
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    for (int q = 0; q < G::NQ; ++q) {
+        if constexpr (FP8) {
+            const int dy = q / 3 - 1, dx = q % 3 - 1;
+            const uint8_t *aq = wb + 64 * q, *bq = xb + (dy * W2 + dx) * G::PXB;
+            const i32x4 a0 = *(const i32x4 *)aq, a1 = *(const i32x4 *)(aq + 16);
+            const i32x4 b0 = *(const i32x4 *)bq, b1 = *(const i32x4 *)(bq + 16);
+            const i32x8 av = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+            const i32x8 bv = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+            acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+        } else {
+            const int t = q >> 2, dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 a = *(const bf16x8 *)(wb + 32 * q);
+            const bf16x8 b = *(const bf16x8 *)(xb + (dy * W2 + dx) * G::PXB + 32 * (q & 3));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
     }
+    // The MFMAs (and the per-lane scale operands of the fp8 form) must run
+    // with every lane active: an MFMA reads all 64 lanes' operands whatever
+    // EXEC says.  The empty asm consumes acc here, so the compiler cannot
+    // sink the chain into the branch below.
+    asm volatile("" : "+v"(acc));
     if (!valid) return;
+    // the implicit GEMM's scaling: alpha = 1 / (qa qb) with A = filter, B = input
+    const float alpha = FP8 ? (1.0f / *qw) * (1.0f / *qi) : 1.0f;
     float *o = out + (size_t)n * 64 * HW + p;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = 32 * hc + 8 * (r >> 2) + 4 * h + (r & 3);
-        o[(size_t)m * HW] = acc[r];
+        o[(size_t)m * HW] = acc[r] * alpha;
     }
 }
 
@@ -733,23 +765,25 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
     const ConvDst d{out, K, OH * OW, N * OH * OW};
     const int NP = N * OH * OW;
     hipStream_t st = as_stream(s);
-    if (mode == GM_BF16 && C == 64 && K == 64 && R == 3 && S == 3 && stride == 1 && pad == 1 && W >= 8 && W <= 64 &&
-        ((reinterpret_cast<uintptr_t>(in_hi) | reinterpret_cast<uintptr_t>(w_hi)) & 15) == 0 && direct_conv_enabled) {
-        // the direct form: same operands and MFMA order as the implicit GEMM below
-        const size_t lds = (size_t)DC_WBYTES + (size_t)dc_rows(W) * (W + 2) * DC_PXB;
-        const int tiles = ceil_div(OH * OW, DC_PX);
-        static bool attr_set = false;
-        if (!attr_set) {
-            int most = 0;  // the largest input tile over the widths this form takes
-            for (int w = 8; w <= 64; ++w) most = std::max(most, dc_rows(w) * (w + 2) * DC_PXB);
-            UMI_TRY(hip_check(hipFuncSetAttribute((const void *)conv3x3_direct_bf16_kernel,
-                                                  hipFuncAttributeMaxDynamicSharedMemorySize, DC_WBYTES + most),
+    if ((mode == GM_BF16 || mode == GM_FP8) && C == 64 && K == 64 && R == 3 && S == 3 && stride == 1 && pad == 1 &&
+        W >= 8 && W <= 64 && ((reinterpret_cast<uintptr_t>(in_hi) | reinterpret_cast<uintptr_t>(w_hi)) & 15) == 0 &&
+        direct_conv_enabled) {
+        // the direct form: same operands, MFMA order and scaling as the implicit GEMM below
+        const bool fp8 = mode == GM_FP8;
+        const auto kern = fp8 ? conv3x3_direct_kernel<true> : conv3x3_direct_kernel<false>;
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[fp8]) {
+            size_t most = 0;  // the largest LDS over the widths this form takes
+            for (int w = 8; w <= 64; ++w) most = std::max(most, fp8 ? dc_lds<true>(w) : dc_lds<false>(w));
+            UMI_TRY(hip_check(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)most),
                               "hipFuncSetAttribute(direct conv LDS)"));
-            attr_set = true;
+            attr_set[fp8] = true;
         }
-        hipLaunchKernelGGL(conv3x3_direct_bf16_kernel, dim3(N * tiles), dim3(DC_THREADS), lds, st,
-                           (const uint16_t *)in_hi, (const uint16_t *)w_hi, N, H, W, out);
-        return launch_check("conv3x3_direct_bf16_kernel");
+        const size_t lds = fp8 ? dc_lds<true>(W) : dc_lds<false>(W);
+        hipLaunchKernelGGL(kern, dim3(N * ceil_div(OH * OW, DC_PX)), dim3(DC_THREADS), lds, st,
+                           (const uint8_t *)in_hi, (const uint8_t *)w_hi, N, H, W, qw, qi, out);
+        return launch_check("conv3x3_direct_kernel");
     }
     switch (mode) {
         case GM_F32: return gemm_launch<GM_F32>(ah, al, bh, bl, d, K, NP, KG, nullptr, nullptr, st);
