@@ -90,7 +90,9 @@ def learned_proposer(st):
             "blocks_per_s_pipeline": prop.nblk / (with_net["ms_pipeline"] * 1e-3),
             "blocks_per_s_full_search": prop.nblk / (with_net["ms_full_search"] * 1e-3),
             "with_network": with_net, "without_network": without, "four_streams": concurrent,
-            "six_streams": concurrent6}
+            "six_streams": concurrent6,
+            # a second held-out set, one pair at a time (the ratio moves with the pairs' motion)
+            "second_set_6_pairs_from_3000000": proposer.evaluate(prop, 6, 3_000_000, proposals=True, reps=20)}
 
 
 def extra_configs(st):
