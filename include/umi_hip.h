@@ -269,7 +269,7 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
                           const void *in_hi, const void *in_lo, const void *w_hi, const void *w_lo, float *out,
                           const float *q_in, const float *q_w, umi_stream s);
 
-/* Selects the form of umi_conv_fprop for bf16 and fp8, 3x3, stride 1,
+/* Selects the form of umi_conv_fprop for bf16, fp8 and fp8x3, 3x3, stride 1,
  * padding 1, C = K = 64, 8 <= W <= 64 (the batch-16 56x56 layer): 1 (default)
  * = a direct convolution with the filter and the input tile staged once in
  * LDS, 0 = the implicit GEMM.  Both use the same operands, MFMA order and

@@ -553,32 +553,41 @@ umi_status pack_common(int mode, const float *amax, void *hi, void *lo, const ch
 constexpr int DC_GROUPS = 7, DC_PX = 32 * DC_GROUPS, DC_THREADS = 128 * DC_GROUPS;
 bool direct_conv_enabled = true;  // umi_conv_set_direct
 __host__ __device__ constexpr int dc_rows(int W) { return (DC_PX + W - 1) / W + 3; }
-template <bool FP8>
+template <bool FP8, bool SPLIT = false>
 struct DcGeom {
     static constexpr int ESZ = FP8 ? 1 : 2, PIXB = 64 * ESZ, PXB = PIXB + 16, WROW = 576 * ESZ + 16;
     static constexpr int WBYTES = 64 * WROW, WCHUNKS = 64 * 576 * ESZ / 16, LANEB = FP8 ? 32 : 16;
-    static constexpr int NQ = FP8 ? 9 : 36;
+    static constexpr int NQ = FP8 ? 9 : 36, PARTS = SPLIT ? 2 : 1;  // SPLIT: hi and lo operands
 };
-template <bool FP8>
+template <bool FP8, bool SPLIT>
 __host__ inline size_t dc_lds(int W) {
-    using G = DcGeom<FP8>;
-    return (size_t)G::WBYTES + (size_t)dc_rows(W) * (W + 2) * G::PXB;
+    using G = DcGeom<FP8, SPLIT>;
+    return G::PARTS * ((size_t)G::WBYTES + (size_t)dc_rows(W) * (W + 2) * G::PXB);
 }
 
-template <bool FP8>
-__global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_t *__restrict__ in,
-                                                                    const uint8_t *__restrict__ wt, int N, int H,
+template <bool FP8, bool SPLIT>
+__global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_t *__restrict__ in_hi,
+                                                                    const uint8_t *__restrict__ in_lo,
+                                                                    const uint8_t *__restrict__ wt_hi,
+                                                                    const uint8_t *__restrict__ wt_lo, int N, int H,
                                                                     int W, const float *qw, const float *qi,
                                                                     float *__restrict__ out) {
-    using G = DcGeom<FP8>;
+    static_assert(FP8 || !SPLIT, "the split form is staged for fp8 only");
+    using G = DcGeom<FP8, SPLIT>;
     extern __shared__ __attribute__((aligned(16))) uint8_t dsm[];
-    uint8_t *wsm = dsm, *xsm = dsm + G::WBYTES;
+    // LDS: filter hi [, lo], then the input tile hi [, lo]
+    uint8_t *wsm = dsm, *xsm = dsm + G::PARTS * G::WBYTES;
     const int HW = H * W, tiles = (HW + DC_PX - 1) / DC_PX;
     const int n = blockIdx.x / tiles, p0 = (blockIdx.x - n * tiles) * DC_PX;
     const int ra = p0 / W, rb = min(p0 + DC_PX - 1, HW - 1) / W;
     const int y0 = ra - 1, nrows = rb - ra + 3, W2 = W + 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 31, h = lane >> 5;
     const int g = wv % DC_GROUPS, hc = wv / DC_GROUPS;
+    const int xbytes = nrows * W2 * G::PXB;  // one input part in LDS
+#pragma unroll
+    for (int part = 0; part < G::PARTS; ++part) {
+    const uint8_t *wt = part ? wt_lo : wt_hi, *in = part ? in_lo : in_hi;
+    uint8_t *wsp = wsm + part * G::WBYTES, *xsp = xsm + part * xbytes;
     // filter: 64 rows of 576 ESZ bytes in 16-byte chunks, all of a thread's loads issued first
     {
         constexpr int U = (G::WCHUNKS + DC_THREADS - 1) / DC_THREADS, RC = 576 * G::ESZ / 16;
@@ -591,7 +600,7 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1), row = e / RC;
-            *(i32x4 *)(wsm + row * G::WROW + 16 * (e - row * RC)) = v[u];
+            *(i32x4 *)(wsp + row * G::WROW + 16 * (e - row * RC)) = v[u];
         }
     }
     // input: chunk e = (slot, 16-byte part c) of the nrows x W2 pixel slots
@@ -613,7 +622,8 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsm + slot[u] * G::PXB + 16 * cpart[u]) = v[u];
+            if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsp + slot[u] * G::PXB + 16 * cpart[u]) = v[u];
+    }
     }
     __syncthreads();
     const int p = p0 + 32 * g + lr;
@@ -628,11 +638,19 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
         if constexpr (FP8) {
             const int dy = q / 3 - 1, dx = q % 3 - 1;
             const uint8_t *aq = wb + 64 * q, *bq = xb + (dy * W2 + dx) * G::PXB;
-            const i32x4 a0 = *(const i32x4 *)aq, a1 = *(const i32x4 *)(aq + 16);
-            const i32x4 b0 = *(const i32x4 *)bq, b1 = *(const i32x4 *)(bq + 16);
-            const i32x8 av = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-            const i32x8 bv = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+            const auto ld32 = [](const uint8_t *p) {
+                const i32x4 u0 = *(const i32x4 *)p, u1 = *(const i32x4 *)(p + 16);
+                return i32x8{u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
+            };
+            const i32x8 av = ld32(aq), bv = ld32(bq);
             acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+            if constexpr (SPLIT) {
+                // the implicit GEMM's residual terms: A lo * B hi, then A hi * B lo, each
+                // with the 2^-4 weight as the lo operand's E8M0 scale (127 - 4)
+                const i32x8 al = ld32(aq + G::WBYTES), bl = ld32(bq + xbytes);
+                acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(al, bv, acc, 0, 0, 0, 123, 0, 127);
+                acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bl, acc, 0, 0, 0, 127, 0, 123);
+            }
         } else {
             const int t = q >> 2, dy = t / 3 - 1, dx = t % 3 - 1;
             const bf16x8 a = *(const bf16x8 *)(wb + 32 * q);
@@ -765,24 +783,30 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
     const ConvDst d{out, K, OH * OW, N * OH * OW};
     const int NP = N * OH * OW;
     hipStream_t st = as_stream(s);
-    if ((mode == GM_BF16 || mode == GM_FP8) && C == 64 && K == 64 && R == 3 && S == 3 && stride == 1 && pad == 1 &&
-        W >= 8 && W <= 64 && ((reinterpret_cast<uintptr_t>(in_hi) | reinterpret_cast<uintptr_t>(w_hi)) & 15) == 0 &&
+    if ((mode == GM_BF16 || mode == GM_FP8 || mode == GM_FP8X3) && C == 64 && K == 64 && R == 3 && S == 3 &&
+        stride == 1 && pad == 1 && W >= 8 && W <= 64 &&
+        ((reinterpret_cast<uintptr_t>(in_hi) | reinterpret_cast<uintptr_t>(w_hi) |
+          (mode == GM_FP8X3 ? reinterpret_cast<uintptr_t>(in_lo) | reinterpret_cast<uintptr_t>(w_lo) : 0)) & 15) == 0 &&
         direct_conv_enabled) {
         // the direct form: same operands, MFMA order and scaling as the implicit GEMM below
-        const bool fp8 = mode == GM_FP8;
-        const auto kern = fp8 ? conv3x3_direct_kernel<true> : conv3x3_direct_kernel<false>;
-        static bool attr_set[2] = {false, false};
-        if (!attr_set[fp8]) {
+        const int form = mode == GM_BF16 ? 0 : mode == GM_FP8 ? 1 : 2;
+        const auto kern = form == 0 ? conv3x3_direct_kernel<false, false>
+                                    : form == 1 ? conv3x3_direct_kernel<true, false> : conv3x3_direct_kernel<true, true>;
+        const auto lds_for = [form](int w) {
+            return form == 0 ? dc_lds<false, false>(w) : form == 1 ? dc_lds<true, false>(w) : dc_lds<true, true>(w);
+        };
+        static bool attr_set[3] = {false, false, false};
+        if (!attr_set[form]) {
             size_t most = 0;  // the largest LDS over the widths this form takes
-            for (int w = 8; w <= 64; ++w) most = std::max(most, fp8 ? dc_lds<true>(w) : dc_lds<false>(w));
+            for (int w = 8; w <= 64; ++w) most = std::max(most, lds_for(w));
             UMI_TRY(hip_check(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                   (int)most),
                               "hipFuncSetAttribute(direct conv LDS)"));
-            attr_set[fp8] = true;
+            attr_set[form] = true;
         }
-        const size_t lds = fp8 ? dc_lds<true>(W) : dc_lds<false>(W);
-        hipLaunchKernelGGL(kern, dim3(N * ceil_div(OH * OW, DC_PX)), dim3(DC_THREADS), lds, st,
-                           (const uint8_t *)in_hi, (const uint8_t *)w_hi, N, H, W, qw, qi, out);
+        hipLaunchKernelGGL(kern, dim3(N * ceil_div(OH * OW, DC_PX)), dim3(DC_THREADS), lds_for(W), st,
+                           (const uint8_t *)in_hi, (const uint8_t *)in_lo, (const uint8_t *)w_hi,
+                           (const uint8_t *)w_lo, N, H, W, qw, qi, out);
         return launch_check("conv3x3_direct_kernel");
     }
     switch (mode) {
