@@ -182,3 +182,15 @@ def test_batch_box_sum(gpu):
     for im, o in zip(u8, b.offsets):
         h, w = im.shape
         np.testing.assert_array_equal(flat[o:o + h * w].reshape(h, w), oracle.box_sum(im, 3, 5))
+
+
+def test_box_sum_pitched(gpu):
+    # input and output pitches differ from the width (and from each other)
+    H, W, ip, op = 45, 77, 84, 80
+    img = workloads.rng(12).integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    S = np.zeros((H, ip), np.uint32)
+    S[:, :W] = oracle.iimg(img)
+    out = DeviceBuffer(H * op * 4)
+    print("[box] pitched", flush=True)
+    ops.box_sum(DeviceBuffer.from_array(S), H, W, ip, 3, 2, out, op)
+    np.testing.assert_array_equal(out.download((H, op), np.uint32)[:, :W], oracle.box_sum(img, 3, 2))
