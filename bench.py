@@ -200,6 +200,19 @@ def extra_configs(st):
     us1 = timed(lambda: ops.box_sum(s1, 1080, 1920, 1920, 4, 4, o1, 1920, stream=st), st, 100) * 1e3
     out["box_sum_from_iimg_r4"] = {"one_1080p_us": us1, "one_1080p_gbs": 8 * 1080 * 1920 / (us1 * 1e-6) / 1e9,
                                    "batch_ms": t_bx, "batch_gbs": 8 * px / (t_bx * 1e-3) / 1e9}
+    # --- 16 rectangle (Haar-like) features of 2-3 rectangles, 24x24 window, on the same 1080p integral image
+    gf = workloads.rng(24)
+    feats = np.zeros((16, 3, 5), np.int32)
+    for i in range(16):
+        for k in range(2 + i % 2):
+            w_, h_ = int(gf.integers(2, 25)), int(gf.integers(2, 25))
+            feats[i, k] = (int(gf.integers(0, 25 - w_)), int(gf.integers(0, 25 - h_)), w_, h_, (-1, 2, -1)[k])
+    fd = DeviceBuffer.from_array(feats)
+    npos = (1080 - 23) * (1920 - 23)
+    hf = DeviceBuffer(16 * npos * 4)
+    us_h = timed(lambda: ops.haar_features(s1, 1080, 1920, 1920, feats, 24, 24, hf, fd, stream=st), st, 50) * 1e3
+    out["haar_16_features_1080p_24x24"] = {"us": us_h, "g_feature_values_s": 16 * npos / (us_h * 1e-6) / 1e9,
+                                           "output_gbs": 16 * npos * 4 / (us_h * 1e-6) / 1e9}
     lp = learned_proposer(st)
     if lp is not None:
         out["learned_proposal_me_1080p_r32"] = lp

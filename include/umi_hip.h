@@ -229,6 +229,22 @@ umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch
                                 int out_pitch, umi_stream s);
 
 /* ---------------------------------------------------------------------------
+ * Rectangle (Haar-like) features from an inclusive uint32 integral image.
+ * A feature is up to 3 rectangles (x, y, w, h, weight) relative to a
+ * win_h x win_w detection window; feats: int32 [nfeat][3][5], weight 0 =
+ * unused rectangle, every used one inside the window (else UMI_ERR_INVALID).
+ * For every window position (y, x), 0 <= y <= H - win_h, 0 <= x <= W - win_w:
+ *   out[f][y][x] = sum_k weight_k * (sum of the source image over rectangle
+ *                  k moved to (y, x)),
+ * int32 with wrap-around (unsigned arithmetic mod 2^32).  feats_host is the
+ * same table in host memory (checked on the host), feats its device copy.
+ * out: int32 [nfeat][H - win_h + 1][W - win_w + 1].  No reference
+ * counterpart: a consumer of the integral image of main/iimg.cu:37-82.
+ * ------------------------------------------------------------------------- */
+umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats_host,
+                                 const int32_t *feats, int nfeat, int win_h, int win_w, int32_t *out, umi_stream s);
+
+/* ---------------------------------------------------------------------------
  * Dense contractions on the matrix cores.  The reference runs GEMM and
  * direct convolution as scalar UMI launches of its Gemm strategy
  * (main/gemm.cu:8-15,37-67; main/fprop.cu:10-17,53-82); these entry points

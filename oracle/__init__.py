@@ -21,6 +21,7 @@ _SIGS = {
     "oracle_stereo_wta_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_box_sum_u8": [_P, _I, _I, _I, _I, _I, _P],
     "oracle_me_subpel_u8": [_P, _P, _I, _I, _I, _P, _P],
+    "oracle_haar_u8": [_P, _I, _I, _I, _P, _I, _I, _I, _P],
     "oracle_filter3x3_i32": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _I],
     "oracle_filter3x3_f32": [_P, _I, _I, _I, _F, _F, _F, _I, _P, _I],
     "oracle_wfilter3x3_i32": [_P, _I, _I, _I, _P, _I, _I, _I, _P, _I],
@@ -105,6 +106,15 @@ def me_subpel(cur, ref, mv):
     H, W = cur.shape
     out = np.zeros((H // 8, W // 8, 3), np.int32)
     lib().oracle_me_subpel_u8(_ptr(cur), _ptr(ref), H, W, W, _ptr(mv), _ptr(out))
+    return out
+
+def haar(img, feats, win_h, win_w):
+    """Brute-force rectangle features (feats int32 [nfeat][3][5]) at every window position, int32."""
+    img = np.ascontiguousarray(img, np.uint8)
+    feats = np.ascontiguousarray(feats, np.int32).reshape(-1, 3, 5)
+    H, W = img.shape
+    out = np.zeros((feats.shape[0], H - win_h + 1, W - win_w + 1), np.int32)
+    lib().oracle_haar_u8(_ptr(img), H, W, W, _ptr(feats), feats.shape[0], win_h, win_w, _ptr(out))
     return out
 
 def box_sum(img, ry, rx):

@@ -168,6 +168,33 @@ void oracle_box_sum_u8(const uint8_t *in, int H, int W, int ip, int ry, int rx, 
 }
 
 /*
+ * Rectangle (Haar-like) features by brute force over the uint8 image: at
+ * every window position, sum over the feature's rectangles (x, y, w, h,
+ * weight; weight 0 = unused) of weight x the pixel sum of the rectangle
+ * moved to the position; unsigned arithmetic (int32 with wrap-around).  No
+ * loop of the reference computes it; it checks features read from the
+ * integral image of main/iimg.cu:37-82.
+ */
+void oracle_haar_u8(const uint8_t *in, int H, int W, int ip, const int32_t *feats, int nfeat, int win_h, int win_w,
+                    int32_t *out) {
+    const int OH = H - win_h + 1, OW = W - win_w + 1;
+    for (int f = 0; f < nfeat; ++f)
+        for (int y = 0; y < OH; ++y)
+            for (int x = 0; x < OW; ++x) {
+                uint32_t v = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const int32_t *r = feats + (f * 3 + k) * 5;
+                    if (r[4] == 0) continue;
+                    uint32_t s = 0;
+                    for (int yy = y + r[1]; yy < y + r[1] + r[3]; ++yy)
+                        for (int xx = x + r[0]; xx < x + r[0] + r[2]; ++xx) s += in[(size_t)yy * ip + xx];
+                    v += (uint32_t)r[4] * s;
+                }
+                out[((size_t)f * OH + y) * OW + x] = (int32_t)v;
+            }
+}
+
+/*
  * Quarter-pel refinement of 8x8 block motion vectors (no loop of the
  * reference computes it; it refines the integer vectors of the full search,
  * main/full_me.cu:66-87 generalised).  Reference samples at quarter-pel

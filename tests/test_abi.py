@@ -47,6 +47,13 @@ def test_errors_are_status_codes_not_aborts():
     assert b"null" in L.umi_last_error()
     assert L.umi_me_full_search_u8(1, 1, 64, 64, 64, 12, 4, 1, None, None) == 3
     assert L.umi_cost_volume_u8(1, 1, 8, 8, 8, 4, 0, 2, 1, 1, 1, None) == 1
+    # a feature rectangle outside its window is refused on the host
+    import numpy as np
+
+    feats = np.zeros((1, 3, 5), np.int32)
+    feats[0, 0] = (20, 0, 8, 4, 1)  # x + w = 28 > 24
+    assert L.umi_haar_features_u32(1, 64, 64, 64, feats.ctypes.data, 1, 1, 24, 24, 1, None) == 1
+    assert b"outside" in L.umi_last_error()
 
 
 def _pack(pd, ad):

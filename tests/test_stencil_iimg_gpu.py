@@ -194,3 +194,24 @@ def test_box_sum_pitched(gpu):
     print("[box] pitched", flush=True)
     ops.box_sum(DeviceBuffer.from_array(S), H, W, ip, 3, 2, out, op)
     np.testing.assert_array_equal(out.download((H, op), np.uint32)[:, :W], oracle.box_sum(img, 3, 2))
+
+
+def _random_features(g, n, win_h, win_w):
+    f = np.zeros((n, 3, 5), np.int32)
+    for i in range(n):
+        for k in range(int(g.integers(1, 4))):
+            w, h = int(g.integers(1, win_w + 1)), int(g.integers(1, win_h + 1))
+            x, y = int(g.integers(0, win_w - w + 1)), int(g.integers(0, win_h - h + 1))
+            f[i, k] = (x, y, w, h, int(g.integers(-4, 5)) or 1)
+    return f
+
+
+@pytest.mark.parametrize("H,W,wh,ww", [(100, 130, 24, 24), (24, 24, 24, 24), (37, 300, 5, 19), (270, 480, 24, 24)])
+def test_haar_features_from_gpu_iimg(gpu, H, W, wh, ww):
+    g = workloads.rng(H + W + wh)
+    img = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
+    feats = _random_features(g, 3 if H > 500 else 12, wh, ww)
+    feats[0, 0] = (0, 0, ww, wh, 2**29)  # a large weight: the sums wrap as int32
+    print(f"[haar] {H}x{W} window {wh}x{ww} features {len(feats)}", flush=True)
+    got = ops.haar_features_np(ops.iimg_np(img), feats, wh, ww)
+    np.testing.assert_array_equal(got, oracle.haar(img, feats, wh, ww))

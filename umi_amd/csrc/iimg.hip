@@ -90,6 +90,37 @@ __global__ __launch_bounds__(256) void box_sum_kernel(const uint32_t *__restrict
     box_sum_tile(S, H, W, ip, ry, rx, out, op, blockIdx.x * BX_TX, blockIdx.y * BX_TY);
 }
 
+
+// Rectangle (Haar-like) features from an inclusive integral image: thread =
+// window position (x, y) (64 consecutive x per wave); per feature, up to
+// three rectangles (x, y, w, h, weight) relative to the window, each sum
+// from four corners; unsigned arithmetic, exact mod 2^32.  Feature
+// parameters are the same for every lane (scalar loads); the output planes
+// are written coalesced.
+constexpr int HF_RECTS = 3, HF_FIELDS = 5;
+__global__ __launch_bounds__(256) void haar_kernel(const uint32_t *__restrict__ S, int ip, int OH, int OW,
+                                                   const int32_t *__restrict__ feats, int nfeat,
+                                                   int32_t *__restrict__ out) {
+    const int x = blockIdx.x * 64 + (int)(threadIdx.x & 63), y = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (x >= OW || y >= OH) return;
+    const size_t plane = (size_t)OH * OW;
+    for (int f = 0; f < nfeat; ++f) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < HF_RECTS; ++k) {
+            const int32_t *r = feats + (f * HF_RECTS + k) * HF_FIELDS;
+            const int rx = r[0], ry = r[1], rw = r[2], rh = r[3], wt = r[4];
+            if (wt == 0) continue;  // uniform
+            const int xa = x + rx - 1, xb = x + rx + rw - 1, ya = y + ry - 1, yb = y + ry + rh - 1;
+            const uint32_t *rowb = S + (size_t)yb * ip, *rowa = S + (size_t)max(ya, 0) * ip;
+            const uint32_t a = rowb[xb], b = rowb[max(xa, 0)], c = rowa[xb], d = rowa[max(xa, 0)];
+            const uint32_t sum = a - (xa >= 0 ? b : 0u) - (ya >= 0 ? c : 0u) + (xa >= 0 && ya >= 0 ? d : 0u);
+            v += (uint32_t)wt * sum;
+        }
+        out[f * plane + (size_t)y * OW + x] = (int32_t)v;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -130,6 +161,28 @@ umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch
     hipLaunchKernelGGL(box_sum_kernel, dim3(ceil_div(W, BX_TX), ceil_div(H, BX_TY)), dim3(256), 0, as_stream(s), iimg,
                        H, W, in_pitch, ry, rx, out, out_pitch);
     return launch_check("box_sum_kernel");
+}
+
+umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats_host,
+                                 const int32_t *feats, int nfeat, int win_h, int win_w, int32_t *out, umi_stream s) {
+    if (!iimg || !feats || !feats_host || !out) return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: null pointer");
+    if (H < 1 || W < 1 || pitch < W || nfeat < 0 || win_h < 1 || win_w < 1 || win_h > H || win_w > W)
+        return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: H=%d W=%d pitch=%d window %dx%d", H, W, pitch,
+                         win_h, win_w);
+    // every rectangle must lie inside the window (so every corner read stays inside the image)
+    for (int f = 0; f < nfeat; ++f)
+        for (int k = 0; k < HF_RECTS; ++k) {
+            const int32_t *r = feats_host + (f * HF_RECTS + k) * HF_FIELDS;
+            if (r[4] == 0) continue;
+            if (r[0] < 0 || r[1] < 0 || r[2] < 1 || r[3] < 1 || r[0] + r[2] > win_w || r[1] + r[3] > win_h)
+                return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: feature %d rectangle %d outside the %dx%d window",
+                                 f, k, win_h, win_w);
+        }
+    if (nfeat == 0) return UMI_OK;
+    const int OH = H - win_h + 1, OW = W - win_w + 1;
+    hipLaunchKernelGGL(haar_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), 0, as_stream(s), iimg, pitch,
+                       OH, OW, feats, nfeat, out);
+    return launch_check("haar_kernel");
 }
 
 }  // extern "C"

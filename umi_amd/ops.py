@@ -221,6 +221,29 @@ def box_sum_np(S: np.ndarray, ry: int, rx: int) -> np.ndarray:
     return o.download((H, W), np.uint32)
 
 
+def haar_features(S: DeviceBuffer, H: int, W: int, pitch: int, feats: np.ndarray, win_h: int, win_w: int,
+                  out: DeviceBuffer, feats_dev: DeviceBuffer | None = None, stream=None) -> None:
+    """Rectangle features (feats int32 [nfeat][3][5] = x, y, w, h, weight per
+    rectangle) at every position of a win_h x win_w window, from a uint32
+    integral image; out int32 [nfeat][H - win_h + 1][W - win_w + 1]."""
+    feats = np.ascontiguousarray(feats, np.int32).reshape(-1, 3, 5)
+    n = feats.shape[0]
+    if out.nbytes < max(n * (H - win_h + 1) * (W - win_w + 1) * 4, 0):
+        raise ValueError("feature buffer too small")
+    fd = feats_dev if feats_dev is not None else DeviceBuffer.from_array(feats)
+    call("umi_haar_features_u32", S.ptr, H, W, pitch, feats.ctypes.data, fd.ptr, n, win_h, win_w, out.ptr,
+         _stream(stream))
+
+
+def haar_features_np(S: np.ndarray, feats: np.ndarray, win_h: int, win_w: int) -> np.ndarray:
+    S = np.ascontiguousarray(S, np.uint32)
+    H, W = S.shape
+    feats = np.ascontiguousarray(feats, np.int32).reshape(-1, 3, 5)
+    o = DeviceBuffer(max(feats.shape[0] * (H - win_h + 1) * (W - win_w + 1) * 4, 4))
+    haar_features(DeviceBuffer.from_array(S), H, W, W, feats, win_h, win_w, o)
+    return o.download((feats.shape[0], H - win_h + 1, W - win_w + 1), np.int32)
+
+
 # ---------------------------------------------------------------------------
 # MFMA GEMM / convolution (main/gemm.cu, main/fprop.cu), johnjohnlin/UMI
 # ---------------------------------------------------------------------------
