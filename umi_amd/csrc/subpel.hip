@@ -84,25 +84,44 @@ __global__ __launch_bounds__(SP_THREADS) void subpel_kernel(const uint8_t *__res
     const int by = b / nbx, bx = b - by * nbx;
     const int dy = mv_in[3 * b], dx = mv_in[3 * b + 1];
     const int y = 8 * by + py, x0 = 8 * bx;
+    // Bytes by aligned dword loads when a row span lies inside its row (the
+    // dwords around it hold at least one byte of the span, so they stay in
+    // the row's mapped memory), then v_alignbyte; a window reaching past a
+    // frame edge takes clamped byte loads.
+    const auto load_span = [](const uint8_t *row, int x, int nd, uint32_t *dst) {
+        const uintptr_t p = reinterpret_cast<uintptr_t>(row + x), a0 = p & ~(uintptr_t)3;
+        const uint32_t sh = (uint32_t)(p & 3);
+        const uintptr_t last = (p + 4 * nd - 1) & ~(uintptr_t)3;  // dword holding the span's last byte
+        uint32_t d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = *reinterpret_cast<const uint32_t *>(min(a0 + 4 * k, last));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (k < nd) dst[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh);
+    };
     uint32_t cw[2];
     {
-        const uint8_t *r = cur + (size_t)y * pitch + x0;
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            cw[j] = (uint32_t)r[4 * j] | (uint32_t)r[4 * j + 1] << 8 | (uint32_t)r[4 * j + 2] << 16 |
-                    (uint32_t)r[4 * j + 3] << 24;
+        uint32_t t[3];
+        load_span(cur + (size_t)y * pitch, x0, 2, t);
+        cw[0] = t[0];
+        cw[1] = t[1];
     }
     uint32_t win[3][3];
+    const int xs = x0 + dx - 1;
+    const bool inside = xs >= 0 && xs + 12 <= W;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         const uint8_t *row = ref + (size_t)clampi(y + dy - 1 + r, 0, H - 1) * pitch;
-        const int xs = x0 + dx - 1;
+        if (inside) {
+            load_span(row, xs, 3, win[r]);
+        } else {
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            uint32_t v = 0;
+            for (int j = 0; j < 3; ++j) {
+                uint32_t v = 0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v |= (uint32_t)row[clampi(xs + 4 * j + k, 0, W - 1)] << (8 * k);
-            win[r][j] = v;
+                for (int k = 0; k < 4; ++k) v |= (uint32_t)row[clampi(xs + 4 * j + k, 0, W - 1)] << (8 * k);
+                win[r][j] = v;
+            }
         }
     }
     uint32_t best = sum8(row_sad(win, cw, 0, 0));
