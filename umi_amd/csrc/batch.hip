@@ -313,7 +313,7 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     b->nicarry = icarry[n];
     b->ws_elems = ws;
     if (st != UMI_OK) {
-        for (void *p : b->allocs) hipFree(p);
+        for (void *p : b->allocs) (void)hipFree(p);  // the first failure is the status reported
         delete b;
         return st;
     }
@@ -323,9 +323,12 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
 
 umi_status umi_batch_destroy(umi_batch b) {
     if (!b) return UMI_OK;
-    for (void *p : b->allocs) hipFree(p);
+    umi_status rc = UMI_OK;
+    for (void *p : b->allocs)
+        if (rc == UMI_OK) rc = hip_check(hipFree(p), "hipFree (batch tables)");
+        else (void)hipFree(p);
     delete b;
-    return UMI_OK;
+    return rc;
 }
 
 umi_status umi_batch_configs(int op, int *count) {

@@ -79,7 +79,7 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     }
     if (own) {
         hipError_t e = hipStreamSynchronize(st);
-        hipFree(tmp);
+        (void)hipFree(tmp);  // the synchronisation's status is the one reported
         if (rc == UMI_OK) rc = hip_check(e, "hipStreamSynchronize (integral image)");
     }
     return rc;
