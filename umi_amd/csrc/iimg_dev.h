@@ -261,12 +261,21 @@ __device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) 
         Col *__restrict__ p = w.C + min(x, W - 1);
         const bool own = x < W;
         Col tot = 0;
+        // segments of at most U tile rows (every height up to 16 * 8 * U) keep
+        // their values in registers between the two passes: one load round trip
+        Col keep[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) keep[u] = 0;
+        const bool one = seg <= U;  // uniform: then every segment fits one batch of loads
         for (int j = j0; j < j1; j += U) {
             Col v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = own && j + u < j1 ? p[(size_t)(j + u) * W] : (Col)0;
 #pragma unroll
-            for (int u = 0; u < U; ++u) tot += v[u];
+            for (int u = 0; u < U; ++u) {
+                tot += v[u];
+                keep[u] = v[u];
+            }
         }
         Col incl = tot;
 #pragma unroll
@@ -276,6 +285,15 @@ __device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) 
         }
         Col run = __shfl_up(incl, 8);
         if (q == 0) run = 0;
+        if (one) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (own && j0 + u < j1) {
+                    p[(size_t)(j0 + u) * W] = run;
+                    run += keep[u];
+                }
+            return;
+        }
         for (int j = j0; j < j1; j += U) {
             Col v[U];
 #pragma unroll
