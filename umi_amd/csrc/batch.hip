@@ -108,6 +108,13 @@ __global__ __launch_bounds__(256) void batch_box_sum_kernel(const BatchImage *__
 // per tile of every image at a time; pass B: one thread per carry item (rows,
 // columns, tile columns) of every image.
 
+// One wave of carry items (an image's items are whole waves): the image's
+// workspace offset and size, and the wave's first item within the image.
+struct CarryWave {
+    long long ws_off;
+    int H, W, k0, pad;
+};
+
 // Persistent loop over the tile descriptors: the next tile's descriptor is
 // loaded before the current tile is processed.
 template <class F>
@@ -139,16 +146,11 @@ __global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const Iim
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const BatchImage *__restrict__ imgs, int n,
-                                                               const int *__restrict__ item_prefix,
-                                                               const int *__restrict__ owner,
-                                                               const long long *__restrict__ ws_off,
+__global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const CarryWave *__restrict__ waves, int total,
                                                                typename T::Col *__restrict__ ws) {
-    const int total = item_prefix[n];
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        const int i = owner[t >> 6];  // every image's carry items are whole waves of 64
-        const BatchImage im = imgs[i];
-        iimg_carry_item<T>(im.H, im.W, t - item_prefix[i], iimg_ws<T>(ws + ws_off[i], im.H, im.W));
+        const CarryWave cw = waves[t >> 6];  // one (wave-uniform) load instead of owner -> image -> prefix
+        iimg_carry_item<T>(cw.H, cw.W, cw.k0 + (t & 63), iimg_ws<T>(ws + cw.ws_off, cw.H, cw.W));
     }
 }
 
@@ -176,13 +178,11 @@ struct umi_batch_s {
     int *ftiles[NF] = {}, *btiles[NB] = {};
     int *fown[NF] = {}, *bown[NB] = {};  // owner tables: the image of every tile
     int ftotal[NF] = {}, btotal[NB] = {};
-    int *icarry = nullptr;                     // integral image carry items (prefix over images)
-    int *cown = nullptr;                       // image of every 64 carry items
+    CarryWave *cwaves = nullptr;               // integral image carry items, per wave of 64
     int *xtiles = nullptr, *xown = nullptr;    // box-sum tiles and their images
     IimgTile *idesc = nullptr;                 // integral-image tile descriptors
     int nxtiles = 0;
     int nitiles = 0, nicarry = 0;
-    long long *ws_off = nullptr;
     long long ws_elems = 0;
     std::vector<void *> allocs;
 };
@@ -239,7 +239,7 @@ umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *o
                        b->nitiles, in, w);
     UMI_TRY(launch_check("batch_iimg_totals_kernel"));
     hipLaunchKernelGGL(batch_iimg_carry_kernel<T>, dim3(grid_for(ceil_div(b->nicarry, 256))), dim3(256), 0, st,
-                       b->imgs, b->n, b->icarry, b->cown, b->ws_off, w);
+                       b->cwaves, b->nicarry, w);
     UMI_TRY(launch_check("batch_iimg_carry_kernel"));
     hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->idesc,
                        b->nitiles, in, w, out);
@@ -293,7 +293,6 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         st = upload(b, bp[c], &b->btiles[c]);
         b->btotal[c] = bp[c][n];
     }
-    if (st == UMI_OK) st = upload(b, icarry, &b->icarry);
     // owner tables: item t of a list belongs to image owner[t]
     auto owners = [&](const std::vector<int> &pre, int per) {
         std::vector<int> o((size_t)pre[n] / per + 1, 0);
@@ -303,11 +302,15 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     };
     for (int c = 0; c < NF && st == UMI_OK; ++c) st = upload(b, owners(fp[c], 1), &b->fown[c]);
     for (int c = 0; c < NB && st == UMI_OK; ++c) st = upload(b, owners(bp[c], 1), &b->bown[c]);
-    if (st == UMI_OK) st = upload(b, owners(icarry, 64), &b->cown);
+    {
+        std::vector<CarryWave> cw;
+        for (int i = 0; i < n; ++i)
+            for (int k = icarry[i]; k < icarry[i + 1]; k += 64) cw.push_back({wso[i], H[i], W[i], k - icarry[i], 0});
+        if (st == UMI_OK) st = upload(b, cw, &b->cwaves);
+    }
     if (st == UMI_OK) st = upload(b, xtiles, &b->xtiles);
     if (st == UMI_OK) st = upload(b, owners(xtiles, 1), &b->xown);
     b->nxtiles = xtiles[n];
-    if (st == UMI_OK) st = upload(b, wso, &b->ws_off);
     if (st == UMI_OK) st = upload(b, idesc, &b->idesc);
     b->nitiles = itiles[n];
     b->nicarry = icarry[n];
