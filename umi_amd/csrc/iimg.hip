@@ -98,6 +98,41 @@ __global__ __launch_bounds__(256) void box_sum_kernel(const uint32_t *__restrict
 // parameters are the same for every lane (scalar loads); the output planes
 // are written coalesced.
 constexpr int HF_RECTS = 3, HF_FIELDS = 5;
+
+// The same from LDS: a workgroup covers 64 x 4 positions and stages the
+// integral image's (4 + win_h) x (64 + win_w) values they read, from row y0 - 1
+// and column x0 - 1, with zeros standing in for row or column -1 (S(-1, .) =
+// S(., -1) = 0), so every corner read is one branch-free LDS read.
+__global__ __launch_bounds__(256) void haar_lds_kernel(const uint32_t *__restrict__ S, int ip, int OH, int OW,
+                                                       int win_h, int win_w, const int32_t *__restrict__ feats,
+                                                       int nfeat, int32_t *__restrict__ out) {
+    extern __shared__ uint32_t sm[];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+    const int tw = 64 + win_w, th = 4 + win_h;  // staged columns x0 - 1 .., rows y0 - 1 ..
+    const int H = OH + win_h - 1, W = OW + win_w - 1;
+    for (int e = threadIdx.x; e < th * tw; e += 256) {
+        const int r = e / tw, c = e - r * tw, gy = y0 - 1 + r, gx = x0 - 1 + c;
+        sm[e] = (gy >= 0 && gx >= 0 && gy < H && gx < W) ? S[(size_t)gy * ip + gx] : 0u;
+    }
+    __syncthreads();
+    const int lx = (int)(threadIdx.x & 63), ly = (int)(threadIdx.x >> 6);
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= OW || y >= OH) return;
+    const size_t plane = (size_t)OH * OW;
+    const uint32_t *base = sm + ly * tw + lx;  // staged (y - 1, x - 1)
+    for (int f = 0; f < nfeat; ++f) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < HF_RECTS; ++k) {
+            const int32_t *r = feats + (f * HF_RECTS + k) * HF_FIELDS;
+            const int rx = r[0], ry = r[1], rw = r[2], rh = r[3], wt = r[4];
+            if (wt == 0) continue;  // uniform
+            const uint32_t *ta = base + ry * tw + rx, *tb = base + (ry + rh) * tw + rx;
+            v += (uint32_t)wt * (tb[rw] - tb[0] - ta[rw] + ta[0]);
+        }
+        out[f * plane + (size_t)y * OW + x] = (int32_t)v;
+    }
+}
 __global__ __launch_bounds__(256) void haar_kernel(const uint32_t *__restrict__ S, int ip, int OH, int OW,
                                                    const int32_t *__restrict__ feats, int nfeat,
                                                    int32_t *__restrict__ out) {
@@ -180,6 +215,12 @@ umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, 
         }
     if (nfeat == 0) return UMI_OK;
     const int OH = H - win_h + 1, OW = W - win_w + 1;
+    const size_t lds = (size_t)(4 + win_h) * (64 + win_w) * 4;
+    if (lds <= 48 * 1024) {  // windows up to about 100 x 100: the tile from LDS
+        hipLaunchKernelGGL(haar_lds_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), lds, as_stream(s),
+                           iimg, pitch, OH, OW, win_h, win_w, feats, nfeat, out);
+        return launch_check("haar_lds_kernel");
+    }
     hipLaunchKernelGGL(haar_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), 0, as_stream(s), iimg, pitch,
                        OH, OW, feats, nfeat, out);
     return launch_check("haar_kernel");
