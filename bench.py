@@ -207,10 +207,9 @@ def extra_configs(st):
         for k in range(2 + i % 2):
             w_, h_ = int(gf.integers(2, 25)), int(gf.integers(2, 25))
             feats[i, k] = (int(gf.integers(0, 25 - w_)), int(gf.integers(0, 25 - h_)), w_, h_, (-1, 2, -1)[k])
-    fd = DeviceBuffer.from_array(feats)
     npos = (1080 - 23) * (1920 - 23)
     hf = DeviceBuffer(16 * npos * 4)
-    us_h = timed(lambda: ops.haar_features(s1, 1080, 1920, 1920, feats, 24, 24, hf, fd, stream=st), st, 50) * 1e3
+    us_h = timed(lambda: ops.haar_features(s1, 1080, 1920, 1920, feats, 24, 24, hf, stream=st), st, 50) * 1e3
     out["haar_16_features_1080p_24x24"] = {"us": us_h, "g_feature_values_s": 16 * npos / (us_h * 1e-6) / 1e9,
                                            "output_gbs": 16 * npos * 4 / (us_h * 1e-6) / 1e9}
     lp = learned_proposer(st)

@@ -50,6 +50,11 @@ umi_status umi_host_free(void *ptr);
 umi_status umi_memcpy(void *dst, const void *src, size_t bytes, int kind);
 umi_status umi_memcpy_async(void *dst, const void *src, size_t bytes, int kind, umi_stream s);
 umi_status umi_memset(void *ptr, int value, size_t bytes);
+/* Fill count 32-bit words at ptr (4-byte aligned) with value, ordered on
+ * stream s by a small kernel (graph-capturable; hipMemset runs on the null
+ * stream, which a non-blocking stream does not wait for).  No reference
+ * counterpart: the reference zeroes nothing on the device. */
+umi_status umi_fill_u32(void *ptr, uint32_t value, size_t count, umi_stream s);
 umi_status umi_synchronize(void);
 /* Device-to-device copy by a streaming kernel (16-byte aligned, size a
  * multiple of 16): graph-capturable, and the byte-count reference for HBM
@@ -236,13 +241,14 @@ umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch
  * For every window position (y, x), 0 <= y <= H - win_h, 0 <= x <= W - win_w:
  *   out[f][y][x] = sum_k weight_k * (sum of the source image over rectangle
  *                  k moved to (y, x)),
- * int32 with wrap-around (unsigned arithmetic mod 2^32).  feats_host is the
- * same table in host memory (checked on the host), feats its device copy.
+ * int32 with wrap-around (unsigned arithmetic mod 2^32).  feats is a host
+ * array; the library checks it and passes it to the kernels by value (32
+ * features per launch), so no separate device copy can disagree with it.
  * out: int32 [nfeat][H - win_h + 1][W - win_w + 1].  No reference
  * counterpart: a consumer of the integral image of main/iimg.cu:37-82.
  * ------------------------------------------------------------------------- */
-umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats_host,
-                                 const int32_t *feats, int nfeat, int win_h, int win_w, int32_t *out, umi_stream s);
+umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats, int nfeat,
+                                 int win_h, int win_w, int32_t *out, umi_stream s);
 
 /* ---------------------------------------------------------------------------
  * Dense contractions on the matrix cores.  The reference runs GEMM and

@@ -1,6 +1,10 @@
 """The C ABI: libumi_hip.so loads and exports every function include/*.h declares.
 
-No compute call is made here (no GPU needed).
+No compute call is made here (no GPU needed): every call below is refused by
+the library's host-side checks.  Pointer arguments that are not the subject of
+a null check are real device buffers of the requested size when a device is
+visible, so a check that regressed would launch on valid memory and show up as
+a wrong status code, never as a fault.
 """
 import ctypes
 import re
@@ -38,6 +42,18 @@ def test_library_exports_every_declared_symbol():
     assert not unbound, f"declared but not bound in umi_amd._lib: {unbound}"
 
 
+def _buffers(*sizes):
+    """Device buffers of the given byte sizes when a device is visible, else
+    distinct non-null placeholders (with no device nothing can be launched)."""
+    import umi_amd
+    from umi_amd import DeviceBuffer
+
+    if umi_amd.device_count() > 0:
+        bufs = [DeviceBuffer(max(n, 4)) for n in sizes]
+        return bufs, [b.ptr for b in bufs]
+    return [], [0x1000 * (i + 1) for i in range(len(sizes))]
+
+
 def test_errors_are_status_codes_not_aborts():
     # argument validation happens before any device call
     from umi_amd._lib import lib
@@ -45,14 +61,17 @@ def test_errors_are_status_codes_not_aborts():
     L = lib()
     assert L.umi_me_full_search_u8(None, None, 8, 8, 8, 8, 4, None, None, None) == 1
     assert b"null" in L.umi_last_error()
-    assert L.umi_me_full_search_u8(1, 1, 64, 64, 64, 12, 4, 1, None, None) == 3
-    assert L.umi_cost_volume_u8(1, 1, 8, 8, 8, 4, 0, 2, 1, 1, 1, None) == 1
+    keep, (cur, ref, mv) = _buffers(64 * 64, 64 * 64, 64 * 3 * 4)
+    assert L.umi_me_full_search_u8(cur, ref, 64, 64, 64, 12, 4, mv, None, None) == 3  # B = 12 unsupported
+    keep2, (l, r, cv) = _buffers(8 * 8, 8 * 8, 8 * 8 * 4 * 4)
+    assert L.umi_cost_volume_u8(l, r, 8, 8, 8, 4, 0, 2, 1, 1, cv, None) == 1  # step 2 unsupported
     # a feature rectangle outside its window is refused on the host
     import numpy as np
 
     feats = np.zeros((1, 3, 5), np.int32)
     feats[0, 0] = (20, 0, 8, 4, 1)  # x + w = 28 > 24
-    assert L.umi_haar_features_u32(1, 64, 64, 64, feats.ctypes.data, 1, 1, 24, 24, 1, None) == 1
+    keep3, (S, out) = _buffers(64 * 64 * 4, 41 * 41 * 4)
+    assert L.umi_haar_features_u32(S, 64, 64, 64, feats.ctypes.data, 1, 24, 24, out, None) == 1
     assert b"outside" in L.umi_last_error()
 
 

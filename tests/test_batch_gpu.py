@@ -95,13 +95,27 @@ def test_full_uneven_batch_all_ops_in_graphs(gpu):
         for k, v in outs.items():
             np.testing.assert_array_equal(v.download((bf.total,), np.float32), eager[k], err_msg=f"{k} replay {rep}")
         np.testing.assert_array_equal(bu.out.download((bu.total,), np.uint32), eager["iu"])
-    # spot checks against the oracle / one-image kernels
+    # every image against the oracle: wfilter bit-exact (same fmaf order), bilateral within 2e-6
+    # of the fp32 oracle, u8 integral image bit-exact, f32 integral image within 4e-6 of FP64
     o = bf.offsets
-    for i in (0, 17, 255):
+    wt = weights()
+    ws_h = ops.spatial_weights(2, 2.0)
+    worst = {"bl": 0.0, "if": 0.0}
+    for i in range(256):
         h, wd = bf.shapes[i]
         sl = slice(o[i], o[i] + h * wd)
-        np.testing.assert_array_equal(eager["iu"][sl].reshape(h, wd), oracle.iimg(u8[i]))
-        np.testing.assert_array_equal(eager["bl"][sl].reshape(h, wd), ops.bilateral_np(imgs[i], 2, 2.0, 0.1))
+        np.testing.assert_array_equal(eager["wf"][sl].reshape(h, wd), oracle.wfilter3x3(imgs[i], wt, 0.0, 0.0),
+                                      err_msg=f"wfilter image {i}")
+        np.testing.assert_array_equal(eager["iu"][sl].reshape(h, wd), oracle.iimg(u8[i]), err_msg=f"iimg u8 {i}")
+        ref = oracle.bilateral(imgs[i], 2, ws_h, norm, 0.0, 0)
+        worst["bl"] = max(worst["bl"], float(np.max(np.abs(eager["bl"][sl].reshape(h, wd) - ref) /
+                                                    np.maximum(np.abs(ref), 1e-30))))
+        ref = oracle.iimg(imgs[i])
+        worst["if"] = max(worst["if"], float(np.max(np.abs(eager["if"][sl].reshape(h, wd) - ref) / ref)))
+    print(f"[batch] all 256 images: bilateral max rel {worst['bl']:.3e}, iimg f32 max rel {worst['if']:.3e}",
+          flush=True)
+    assert worst["bl"] <= 2e-6
+    assert worst["if"] <= 4e-6
 
 
 def test_autotune_picks_a_config(gpu, tmp_path, monkeypatch):

@@ -81,6 +81,22 @@ def test_gemm_precision_vs_fp64(mode, gpu):
     assert e["max_rel"] <= PRECISION[mode]
 
 
+@pytest.mark.parametrize("mode", MODES, ids=[ops.MODE_NAMES[m] for m in MODES])
+def test_gemm_4096_precision_vs_fp64(mode, gpu):
+    # the precision config: M = N = K = 4096, N(0,1) in FP64 cast to fp32; every 64th row of C
+    # (64 rows, all 4096 columns) against the FP64 oracle, with the same bounds as 1024^3
+    g = workloads.rng(4096)
+    A64 = g.standard_normal((4096, 4096))
+    B64 = g.standard_normal((4096, 4096))
+    print(f"[gemm] precision {ops.MODE_NAMES[mode]} 4096^3", flush=True)
+    C = ops.gemm_np(A64.astype(np.float32), B64.astype(np.float32), mode)
+    rows = np.arange(0, 4096, 64)
+    e = normed_err(C[rows], oracle.gemm_f64(A64[rows], B64), A64[rows], B64.T)
+    print(f"[gemm] precision {ops.MODE_NAMES[mode]} 4096^3 (64 rows): {e}", flush=True)
+    assert e["max_rel"] <= PRECISION[mode]
+    assert np.isfinite(C).all()
+
+
 def test_gemm_identity_and_int_exact(gpu):
     # small integers are exact in every mode's operand format (fp8: {-2..2} with q = 224)
     g = workloads.rng(9)

@@ -206,11 +206,15 @@ def _random_features(g, n, win_h, win_w):
     return f
 
 
-@pytest.mark.parametrize("H,W,wh,ww", [(100, 130, 24, 24), (24, 24, 24, 24), (37, 300, 5, 19), (270, 480, 24, 24)])
-def test_haar_features_from_gpu_iimg(gpu, H, W, wh, ww):
+# (200, 300, 96, 96) and (120, 400, 40, 160) stage more than 48 KB and take the global-memory kernel;
+# (100, 130, 24, 24) with 70 features spans three launches of at most 32 features
+@pytest.mark.parametrize("H,W,wh,ww,nf", [(100, 130, 24, 24, 70), (24, 24, 24, 24, 12), (37, 300, 5, 19, 12),
+                                          (270, 480, 24, 24, 12), (200, 300, 96, 96, 5), (120, 400, 40, 160, 5),
+                                          (90, 100, 80, 80, 4)])
+def test_haar_features_from_gpu_iimg(gpu, H, W, wh, ww, nf):
     g = workloads.rng(H + W + wh)
     img = g.integers(0, 256, size=(H, W), dtype=np.int64).astype(np.uint8)
-    feats = _random_features(g, 3 if H > 500 else 12, wh, ww)
+    feats = _random_features(g, nf, wh, ww)
     feats[0, 0] = (0, 0, ww, wh, 2**29)  # a large weight: the sums wrap as int32
     print(f"[haar] {H}x{W} window {wh}x{ww} features {len(feats)}", flush=True)
     got = ops.haar_features_np(ops.iimg_np(img), feats, wh, ww)

@@ -26,6 +26,7 @@ SIGNATURES = {
     "umi_memcpy": [P, P, c_size_t, c_int],
     "umi_memcpy_async": [P, P, c_size_t, c_int, P],
     "umi_memset": [P, c_int, c_size_t],
+    "umi_fill_u32": [P, C.c_uint32, c_size_t, P],
     "umi_synchronize": [],
     "umi_copy_d2d": [P, P, c_size_t, P],
     "umi_stream_create": [C.POINTER(c_void_p)],
@@ -49,7 +50,7 @@ SIGNATURES = {
     "umi_cost_volume_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
     "umi_stereo_wta_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
     "umi_box_sum_iimg_u32": [P, c_int, c_int, c_int, c_int, c_int, P, c_int, P],
-    "umi_haar_features_u32": [P, c_int, c_int, c_int, P, P, c_int, c_int, c_int, P, P],
+    "umi_haar_features_u32": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, P, P],
     "umi_me_subpel_refine_u8": [P, P, c_int, c_int, c_int, P, P, P],
     "umi_conv_set_direct": [c_int],
     "umi_batch_box_sum_u32": [P, P, c_int, c_int, P, P],

@@ -12,6 +12,7 @@
 // Design: three launches of the tiled reduce-then-scan in iimg_dev.h (tile
 // totals, carries, tile output).  The input is read twice and the output
 // written once; the carries are small (row, column and tile prefix sums).
+#include <cstring>
 #include "common.h"
 #include "iimg_dev.h"
 
@@ -95,17 +96,22 @@ __global__ __launch_bounds__(256) void box_sum_kernel(const uint32_t *__restrict
 // window position (x, y) (64 consecutive x per wave); per feature, up to
 // three rectangles (x, y, w, h, weight) relative to the window, each sum
 // from four corners; unsigned arithmetic, exact mod 2^32.  Feature
-// parameters are the same for every lane (scalar loads); the output planes
-// are written coalesced.
-constexpr int HF_RECTS = 3, HF_FIELDS = 5;
+// parameters are the same for every lane and travel by value in the kernel
+// arguments (scalar loads from the argument segment), at most HF_CHUNK
+// features per launch, so the kernel reads exactly the table the host
+// checked; the output planes are written coalesced.
+constexpr int HF_RECTS = 3, HF_FIELDS = 5, HF_CHUNK = 32;
+struct HaarChunk {
+    int32_t r[HF_CHUNK * HF_RECTS * HF_FIELDS];
+};
 
 // The same from LDS: a workgroup covers 64 x 4 positions and stages the
 // integral image's (4 + win_h) x (64 + win_w) values they read, from row y0 - 1
 // and column x0 - 1, with zeros standing in for row or column -1 (S(-1, .) =
 // S(., -1) = 0), so every corner read is one branch-free LDS read.
 __global__ __launch_bounds__(256) void haar_lds_kernel(const uint32_t *__restrict__ S, int ip, int OH, int OW,
-                                                       int win_h, int win_w, const int32_t *__restrict__ feats,
-                                                       int nfeat, int32_t *__restrict__ out) {
+                                                       int win_h, int win_w, const HaarChunk feats, int nfeat,
+                                                       int32_t *__restrict__ out) {
     extern __shared__ uint32_t sm[];
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
     const int tw = 64 + win_w, th = 4 + win_h;  // staged columns x0 - 1 .., rows y0 - 1 ..
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(256) void haar_lds_kernel(const uint32_t *__restric
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < HF_RECTS; ++k) {
-            const int32_t *r = feats + (f * HF_RECTS + k) * HF_FIELDS;
+            const int32_t *r = feats.r + (f * HF_RECTS + k) * HF_FIELDS;
             const int rx = r[0], ry = r[1], rw = r[2], rh = r[3], wt = r[4];
             if (wt == 0) continue;  // uniform
             const uint32_t *ta = base + ry * tw + rx, *tb = base + (ry + rh) * tw + rx;
@@ -134,8 +140,7 @@ __global__ __launch_bounds__(256) void haar_lds_kernel(const uint32_t *__restric
     }
 }
 __global__ __launch_bounds__(256) void haar_kernel(const uint32_t *__restrict__ S, int ip, int OH, int OW,
-                                                   const int32_t *__restrict__ feats, int nfeat,
-                                                   int32_t *__restrict__ out) {
+                                                   const HaarChunk feats, int nfeat, int32_t *__restrict__ out) {
     const int x = blockIdx.x * 64 + (int)(threadIdx.x & 63), y = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
     if (x >= OW || y >= OH) return;
     const size_t plane = (size_t)OH * OW;
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(256) void haar_kernel(const uint32_t *__restrict__ 
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < HF_RECTS; ++k) {
-            const int32_t *r = feats + (f * HF_RECTS + k) * HF_FIELDS;
+            const int32_t *r = feats.r + (f * HF_RECTS + k) * HF_FIELDS;
             const int rx = r[0], ry = r[1], rw = r[2], rh = r[3], wt = r[4];
             if (wt == 0) continue;  // uniform
             const int xa = x + rx - 1, xb = x + rx + rw - 1, ya = y + ry - 1, yb = y + ry + rh - 1;
@@ -198,32 +203,41 @@ umi_status umi_box_sum_iimg_u32(const uint32_t *iimg, int H, int W, int in_pitch
     return launch_check("box_sum_kernel");
 }
 
-umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats_host,
-                                 const int32_t *feats, int nfeat, int win_h, int win_w, int32_t *out, umi_stream s) {
-    if (!iimg || !feats || !feats_host || !out) return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: null pointer");
+umi_status umi_haar_features_u32(const uint32_t *iimg, int H, int W, int pitch, const int32_t *feats, int nfeat,
+                                 int win_h, int win_w, int32_t *out, umi_stream s) {
+    if (!iimg || (nfeat > 0 && !feats) || !out) return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: null pointer");
     if (H < 1 || W < 1 || pitch < W || nfeat < 0 || win_h < 1 || win_w < 1 || win_h > H || win_w > W)
         return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: H=%d W=%d pitch=%d window %dx%d", H, W, pitch,
                          win_h, win_w);
     // every rectangle must lie inside the window (so every corner read stays inside the image)
     for (int f = 0; f < nfeat; ++f)
         for (int k = 0; k < HF_RECTS; ++k) {
-            const int32_t *r = feats_host + (f * HF_RECTS + k) * HF_FIELDS;
+            const int32_t *r = feats + (f * HF_RECTS + k) * HF_FIELDS;
             if (r[4] == 0) continue;
             if (r[0] < 0 || r[1] < 0 || r[2] < 1 || r[3] < 1 || r[0] + r[2] > win_w || r[1] + r[3] > win_h)
                 return set_error(UMI_ERR_INVALID, "umi_haar_features_u32: feature %d rectangle %d outside the %dx%d window",
                                  f, k, win_h, win_w);
         }
-    if (nfeat == 0) return UMI_OK;
     const int OH = H - win_h + 1, OW = W - win_w + 1;
+    const size_t plane = (size_t)OH * OW;
+    // the staged tile of (4 + win_h) x (64 + win_w) values fits 48 KB for square windows up to 80 x 80
     const size_t lds = (size_t)(4 + win_h) * (64 + win_w) * 4;
-    if (lds <= 48 * 1024) {  // windows up to about 100 x 100: the tile from LDS
-        hipLaunchKernelGGL(haar_lds_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), lds, as_stream(s),
-                           iimg, pitch, OH, OW, win_h, win_w, feats, nfeat, out);
-        return launch_check("haar_lds_kernel");
+    for (int f0 = 0; f0 < nfeat; f0 += HF_CHUNK) {
+        const int nf = min(HF_CHUNK, nfeat - f0);
+        HaarChunk c{};
+        std::memcpy(c.r, feats + (size_t)f0 * HF_RECTS * HF_FIELDS, sizeof(int32_t) * nf * HF_RECTS * HF_FIELDS);
+        int32_t *o = out + (size_t)f0 * plane;
+        if (lds <= 48 * 1024) {
+            hipLaunchKernelGGL(haar_lds_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), lds, as_stream(s),
+                               iimg, pitch, OH, OW, win_h, win_w, c, nf, o);
+            UMI_TRY(launch_check("haar_lds_kernel"));
+        } else {
+            hipLaunchKernelGGL(haar_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), 0, as_stream(s), iimg,
+                               pitch, OH, OW, c, nf, o);
+            UMI_TRY(launch_check("haar_kernel"));
+        }
     }
-    hipLaunchKernelGGL(haar_kernel, dim3(ceil_div(OW, 64), ceil_div(OH, 4)), dim3(256), 0, as_stream(s), iimg, pitch,
-                       OH, OW, feats, nfeat, out);
-    return launch_check("haar_kernel");
+    return UMI_OK;
 }
 
 }  // extern "C"

@@ -30,6 +30,12 @@ __global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
 }
+// Stream-ordered fill of 32-bit words (a kernel, not hipMemsetAsync, so it is
+// ordered with the caller's other work and safe inside captured graphs).
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
 }  // namespace
 
 struct umi_timer_s {
@@ -62,7 +68,9 @@ umi_status umi_device_name(char *buf, size_t len) {
     UMI_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
     hipDeviceProp_t p;
     UMI_TRY(hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties"));
-    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    // some drivers report an empty marketing name: fall back to the architecture and the board's size
+    snprintf(buf, len, "%s (%s, %d CUs, %.0f GiB, %d MHz)", p.name[0] ? p.name : "AMD GPU", p.gcnArchName,
+             p.multiProcessorCount, (double)p.totalGlobalMem / (1u << 30), p.clockRate / 1000);
     return UMI_OK;
 }
 
@@ -109,6 +117,14 @@ umi_status umi_memset(void *ptr, int value, size_t bytes) {
     if (bytes == 0) return UMI_OK;
     if (!ptr) return set_error(UMI_ERR_INVALID, "umi_memset: null pointer");
     return hip_check(hipMemset(ptr, value, bytes), "hipMemset");
+}
+
+umi_status umi_fill_u32(void *ptr, uint32_t value, size_t count, umi_stream s) {
+    if (count == 0) return UMI_OK;
+    if (!ptr || ((uintptr_t)ptr & 3)) return set_error(UMI_ERR_INVALID, "umi_fill_u32: null or unaligned pointer");
+    const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, as_stream(s), (uint32_t *)ptr, value, count);
+    return launch_check("fill_u32_kernel");
 }
 
 umi_status umi_copy_d2d(void *dst, const void *src, size_t bytes, umi_stream s) {

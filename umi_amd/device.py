@@ -54,8 +54,15 @@ class DeviceBuffer:
         call("umi_memcpy", out.ctypes.data, self.ptr, out.nbytes, D2H)
         return out
 
-    def fill(self, byte: int) -> None:
-        call("umi_memset", self.ptr, int(byte), self.nbytes)
+    def fill(self, byte: int, stream=None) -> None:
+        """Every byte = `byte`.  With a stream, a fill kernel ordered on that
+        stream (nbytes a multiple of 4); without, a synchronous memset."""
+        if stream is None:
+            call("umi_memset", self.ptr, int(byte), self.nbytes)
+        else:
+            if self.nbytes % 4:
+                raise ValueError("stream-ordered fill needs a whole number of 32-bit words")
+            call("umi_fill_u32", self.ptr, (int(byte) & 0xFF) * 0x01010101, self.nbytes // 4, _handle(stream))
 
     def free(self) -> None:
         if getattr(self, "ptr", 0):

@@ -222,7 +222,7 @@ def box_sum_np(S: np.ndarray, ry: int, rx: int) -> np.ndarray:
 
 
 def haar_features(S: DeviceBuffer, H: int, W: int, pitch: int, feats: np.ndarray, win_h: int, win_w: int,
-                  out: DeviceBuffer, feats_dev: DeviceBuffer | None = None, stream=None) -> None:
+                  out: DeviceBuffer, stream=None) -> None:
     """Rectangle features (feats int32 [nfeat][3][5] = x, y, w, h, weight per
     rectangle) at every position of a win_h x win_w window, from a uint32
     integral image; out int32 [nfeat][H - win_h + 1][W - win_w + 1]."""
@@ -230,8 +230,7 @@ def haar_features(S: DeviceBuffer, H: int, W: int, pitch: int, feats: np.ndarray
     n = feats.shape[0]
     if out.nbytes < max(n * (H - win_h + 1) * (W - win_w + 1) * 4, 0):
         raise ValueError("feature buffer too small")
-    fd = feats_dev if feats_dev is not None else DeviceBuffer.from_array(feats)
-    call("umi_haar_features_u32", S.ptr, H, W, pitch, feats.ctypes.data, fd.ptr, n, win_h, win_w, out.ptr,
+    call("umi_haar_features_u32", S.ptr, H, W, pitch, feats.ctypes.data, n, win_h, win_w, out.ptr,
          _stream(stream))
 
 

@@ -306,7 +306,7 @@ class Proposer:
         if proposals:
             self.propose(cur, ref, stream)
         else:
-            self.prop.fill(0)
+            self.prop.fill(0, stream)
         self.refine(cur, ref, mv_sad, stream)
 
     def fallback_count(self) -> int:
@@ -402,7 +402,7 @@ def throughput(net: "Network", pairs: int, seed0: int, reps: int = 20, R: int = 
         g.begin()
         props[i].run(curs[i], refs[i], mvs[i], streams[i])
         g.end()
-        mvs[i].fill(0)
+        mvs[i].fill(0, streams[i])
         g.launch()
         streams[i].synchronize()
         if not np.array_equal(mvs[i].download((nblk, 3), np.int32), eager):
