@@ -195,7 +195,7 @@ def extra_configs(st):
     t_bx = batch.time_graph(lambda s: bu.box_sum(4, 4, bbox, s), st)
     one = g.integers(0, 256, size=(1080, 1920), dtype=np.int64).astype(np.uint8)
     d1, s1, o1 = DeviceBuffer.from_array(one), DeviceBuffer(1080 * 1920 * 4), DeviceBuffer(1080 * 1920 * 4)
-    ws1 = DeviceBuffer(max(ops.iimg_workspace_size(1080, 1920, np.uint8), 4))
+    ws1 = ops.iimg_workspace(1080, 1920, np.uint8)
     ops.iimg(d1, 1080, 1920, 1920, s1, 1920, np.uint8, ws1, stream=st)
     us1 = timed(lambda: ops.box_sum(s1, 1080, 1920, 1920, 4, 4, o1, 1920, stream=st), st, 100) * 1e3
     out["box_sum_from_iimg_r4"] = {"one_1080p_us": us1, "one_1080p_gbs": 8 * 1080 * 1920 / (us1 * 1e-6) / 1e9,

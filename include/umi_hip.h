@@ -205,14 +205,35 @@ umi_status umi_bilateral_f32(const float *in, int H, int W, int in_pitch, int ra
  * Integral image out[y][x] = sum_{y'<=y, x'<=x} in[y'][x'].  Replaces the two
  * scan launches of main/iimg.cu:37-82 (row scan into a transposed
  * temporary, then a scan of it).  Forms: uint8 -> uint32 and int32 -> int32
- * (exact; int32 wraps), float -> float with row sums inside 16 x 256 tiles
- * in fp32 and every vertical sum and carry in fp64.  Three launches (tile
- * totals, carries, tile output); the input is read twice, the output
- * written once.
+ * (exact; int32 wraps), float -> float with row sums inside a tile in fp32
+ * and every vertical sum and carry in fp64.  One launch: a persistent grid
+ * walks tiles of TH x 256 in order; each tile publishes its row, column and
+ * tile sums and adds those of the tiles above and to its left.  The input is
+ * read once and the output written once.
  * workspace: device memory of umi_iimg_workspace_size(H, W, kind) bytes
- * (kind 0 = u8, 1 = i32, 2 = f32); NULL makes the call allocate and free it
- * itself, which synchronises the stream.
+ * (kind 0 = u8, 1 = i32, 2 = f32), ZERO-FILLED ONCE before its first use
+ * (umi_memset or umi_fill_u32); each call leaves it ready for the next, so
+ * later calls (graph replays included) clear nothing.  Two calls that
+ * overlap in time must not share a workspace.  NULL makes the call allocate,
+ * zero and free one itself, which synchronises the stream.
+ * umi_iimg_set_config selects the tile height (rows per lane 1, 2, 4 -> TH
+ * 16, 32, 64; 0 = chosen by size); results do not depend on it.
  * ------------------------------------------------------------------------- */
+umi_status umi_iimg_set_config(int rows_per_lane);
+/* Launch shape of the integral-image kernels (one image and batch):
+ * 0 (default) = one workgroup per tile; tiles wait only for tiles of smaller
+ * workgroup id, which relies on the dispatcher starting workgroups in id
+ * order (per XCD), as it does; per_cu > 0 = a persistent grid of per_cu
+ * workgroups per CU walking the tiles in order, which needs only that all of
+ * them fit on the GPU at once (per_cu <= 2 always does).  Results do not
+ * depend on it; either way every wait is bounded in time and reported by
+ * umi_iimg_workspace_status. */
+umi_status umi_iimg_set_grid(int per_cu);
+/* Whether a wait inside an integral-image call on this workspace (one image
+ * or batch) ran past its time bound since the last query (which clears the
+ * flag); every wait normally ends within microseconds, so 1 means the
+ * outputs of that call are not to be trusted.  Synchronous. */
+umi_status umi_iimg_workspace_status(void *workspace, int *timed_out);
 umi_status umi_iimg_workspace_size(int H, int W, int kind, size_t *bytes);
 umi_status umi_iimg_u8_u32(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
                            void *workspace, size_t workspace_bytes, umi_stream s);
@@ -304,12 +325,11 @@ umi_status umi_conv_set_direct(int enable);
  * image (src/UMI_gpu.h:409-623, e.g. main/wfilter3x3.cu:39,
  * main/bilateral.cu:54, main/iimg.cu:37,60); here a batch object describes
  * n images inside one input and one output buffer (element offsets and
- * pitches), and each batched operator is one persistent launch over the
- * tiles of all images (the integral image: three launches, tile totals /
- * carries / tile output).  Creation uploads the work tables once; the operator
- * calls allocate nothing and can be captured in a graph.  `config` selects
- * a tile shape (0 .. umi_batch_configs(op)-1, -1 = default); op 0 = 3x3
- * filter, 1 = bilateral.
+ * pitches), and each batched operator is one launch over the tiles of all
+ * images.  Creation uploads the work tables once; the operator calls
+ * allocate nothing and can be captured in a graph.  `config` selects a tile
+ * shape (0 .. umi_batch_configs(op)-1, -1 = default); op 0 = 3x3 filter,
+ * 1 = bilateral, 2 = integral image (rows per lane 1, 2, 4).
  * ------------------------------------------------------------------------- */
 typedef struct umi_batch_s *umi_batch;
 umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in_off, const int *in_pitch,
@@ -320,12 +340,15 @@ umi_status umi_batch_wfilter3x3_f32(umi_batch b, const float *in, const float *w
                                     int mode, float *out, int config, umi_stream s);
 umi_status umi_batch_bilateral_f32(umi_batch b, const float *in, int radius, const float *spatial, float norm,
                                    float pad, int mode, float *out, int config, umi_stream s);
-/* kind 0: uint8 -> uint32, kind 2: float -> float (fp64 vertical sums and carries) */
+/* kind 0: uint8 -> uint32, kind 2: float -> float (fp64 vertical sums and
+ * carries).  The workspace (any configuration) follows the rules of
+ * umi_iimg_u8_u32's: zero-filled once before its first use, then reusable
+ * without clearing; not shared by overlapping calls. */
 umi_status umi_batch_iimg_workspace_size(umi_batch b, int kind, size_t *bytes);
 umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, void *workspace,
-                                 size_t workspace_bytes, umi_stream s);
+                                 size_t workspace_bytes, int config, umi_stream s);
 umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
-                              umi_stream s);
+                              int config, umi_stream s);
 /* Box sums (as umi_box_sum_iimg_u32) of every image of the batch from its
  * integral image; `iimg` and `out` both use the batch's output offsets and
  * pitches.  One persistent launch; no counterpart in the reference. */

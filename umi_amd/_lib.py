@@ -60,6 +60,9 @@ SIGNATURES = {
     "umi_wfilter3x3_i32": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, P, c_int, P],
     "umi_bilateral_f32": [P, c_int, c_int, c_int, c_int, P, c_float, c_float, c_int, P, c_int, P],
     "umi_iimg_workspace_size": [c_int, c_int, c_int, C.POINTER(c_size_t)],
+    "umi_iimg_set_config": [c_int],
+    "umi_iimg_set_grid": [c_int],
+    "umi_iimg_workspace_status": [P, C.POINTER(c_int)],
     "umi_iimg_u8_u32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_iimg_i32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_iimg_f32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
@@ -75,8 +78,8 @@ SIGNATURES = {
     "umi_batch_wfilter3x3_f32": [P, P, P, c_float, c_float, c_int, P, c_int, P],
     "umi_batch_bilateral_f32": [P, P, c_int, P, c_float, c_float, c_int, P, c_int, P],
     "umi_batch_iimg_workspace_size": [P, c_int, C.POINTER(c_size_t)],
-    "umi_batch_iimg_u8_u32": [P, P, P, P, c_size_t, P],
-    "umi_batch_iimg_f32": [P, P, P, P, c_size_t, P],
+    "umi_batch_iimg_u8_u32": [P, P, P, P, c_size_t, c_int, P],
+    "umi_batch_iimg_f32": [P, P, P, P, c_size_t, c_int, P],
     "umi_me_full_search_listed_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
     "umi_synth_frame0": [P, c_int, c_int, c_int, C.c_uint64, P],
     "umi_synth_warp": [P, P, c_int, c_int, c_int, P, c_float, C.c_uint64, P],

@@ -20,7 +20,7 @@ import numpy as np
 from ._lib import call, lib
 from .device import DeviceBuffer, Graph, Stream, Timer, device_name
 
-OP_FILTER, OP_BILATERAL = 0, 1
+OP_FILTER, OP_BILATERAL, OP_IIMG = 0, 1, 2
 
 
 def _stream(s):
@@ -78,14 +78,15 @@ class ImageBatch:
              (out or self.out).ptr, config, _stream(stream))
 
     def iimg_workspace(self) -> DeviceBuffer:
+        """The integral image's workspace, zero-filled (as it must be before its first use)."""
         n = C.c_size_t(0)
         kind = 0 if self.dtype == np.uint8 else 2
         call("umi_batch_iimg_workspace_size", self.handle, kind, C.byref(n))
-        return DeviceBuffer(max(n.value, 8))
+        return DeviceBuffer.zeros(max(n.value, 16))
 
-    def iimg(self, workspace: DeviceBuffer, stream=None, out=None):
+    def iimg(self, workspace: DeviceBuffer, stream=None, out=None, config=-1):
         name = "umi_batch_iimg_u8_u32" if self.dtype == np.uint8 else "umi_batch_iimg_f32"
-        call(name, self.handle, self.inp.ptr, (out or self.out).ptr, workspace.ptr, workspace.nbytes,
+        call(name, self.handle, self.inp.ptr, (out or self.out).ptr, workspace.ptr, workspace.nbytes, config,
              _stream(stream))
 
     def box_sum(self, ry: int, rx: int, out: DeviceBuffer, stream=None, iimg: DeviceBuffer | None = None):

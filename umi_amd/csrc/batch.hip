@@ -17,6 +17,7 @@
 #include "iimg_dev.h"
 #include "stencil_dev.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -27,13 +28,16 @@ struct BatchImage {
     int H, W, in_pitch, out_pitch;
 };
 
-// One integral-image tile of the batch: everything its passes need, so that
-// a tile costs one (wave-uniform) descriptor load instead of a chain of
-// owner -> image -> prefix -> workspace-offset loads.
-struct IimgTile {
-    long long in_off, out_off, ws_off;
-    int H, W, in_pitch, out_pitch, tx, ty;
+// One integral-image tile of the batch, indexed by its ticket: everything the
+// tile needs in one (wave-uniform) descriptor load.  tbase is the ticket of
+// the image's tile (0, 0), so the published words of tile (a, b) of the same
+// image sit at tbase + a * ntx + b.
+struct IimgBatchTile {
+    long long in_off, out_off, tbase;
+    int H, W, in_pitch, out_pitch, tx, ty, ntx, pad;
 };
+constexpr int NI = 3;                  // integral-image configurations: rows per lane 1, 2, 4
+constexpr int IRPL[NI] = {1, 2, 4};
 
 // 3x3 filter tile shapes (TX, TY) and bilateral tile shapes (TX x TY = 256).
 constexpr int NF = 4, NB = 4;
@@ -104,68 +108,26 @@ __global__ __launch_bounds__(256) void batch_box_sum_kernel(const BatchImage *__
     }
 }
 
-// Integral image (the passes of iimg_dev.h).  Passes A and C: a workgroup
-// per tile of every image at a time; pass B: one thread per carry item (rows,
-// columns, tile columns) of every image.
-
-// One wave of carry items (an image's items are whole waves): the image's
-// workspace offset and size, and the wave's first item within the image.
-struct CarryWave {
-    long long ws_off;
-    int H, W, k0, pad;
-};
-
-// Persistent loop over the tile descriptors: the next tile's descriptor is
-// loaded before the current tile is processed.
-template <class F>
-__device__ inline void for_each_tile(const IimgTile *__restrict__ tiles, int total, F &&body) {
-    int t = blockIdx.x;
-    if (t >= total) return;
-    IimgTile d = tiles[t];
-    for (;;) {
-        const int tn = t + (int)gridDim.x;
-        const IimgTile dn = tiles[tn < total ? tn : t];
-        body(d);
-        if (tn >= total) return;
-        t = tn;
-        d = dn;
+// Integral image: the single pass of iimg_dev.h over the tiles of every
+// image, a workgroup per tile in ticket order.
+template <class T, int RPL>
+__global__ __launch_bounds__(IM_THREADS) void batch_iimg_kernel(const IimgBatchTile *__restrict__ tiles, int total,
+                                                                const typename T::In *__restrict__ in,
+                                                                typename T::Out *__restrict__ out,
+                                                                uint8_t *__restrict__ ws) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t biimg_lds[];
+    auto &sm = *reinterpret_cast<IimgSmem<T, RPL> *>(biimg_lds);
+    auto *ctl = reinterpret_cast<IimgCtl *>(ws);
+    auto *words = reinterpret_cast<uint64_t *>(ws + IM_CTL_BYTES);
+    iimg_begin(sm, ctl);
+    for (int k = blockIdx.x; k < total; k += gridDim.x) {
+        const IimgBatchTile t = tiles[k];
+        const IimgTileRef d{t.tbase, t.H, t.W, t.in_pitch, t.out_pitch, t.tx, t.ty, t.ntx};
+        const typename T::In *src = in + t.in_off;
+        typename T::Out *dst = out + t.out_off;
+        iimg_tile<T, RPL>(sm, d, src, iimg_rows16(src, t.in_pitch), dst, ctl, words);
     }
-}
-
-template <class T>
-__global__ __launch_bounds__(IM_THREADS) void batch_iimg_totals_kernel(const IimgTile *__restrict__ tiles, int total,
-                                                                       const typename T::In *__restrict__ in,
-                                                                       typename T::Col *__restrict__ ws) {
-    __shared__ typename T::Row lds[IM_TH * IM_TW];
-    __shared__ typename T::Col tmp[4];
-    for_each_tile(tiles, total, [&](const IimgTile &d) {
-        const typename T::In *src = in + d.in_off;
-        iimg_tile_totals<T>(lds, tmp, src, d.H, d.W, d.in_pitch, iimg_rows16(src, d.in_pitch), d.tx, d.ty,
-                            iimg_ws<T>(ws + d.ws_off, d.H, d.W));
-    });
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void batch_iimg_carry_kernel(const CarryWave *__restrict__ waves, int total,
-                                                               typename T::Col *__restrict__ ws) {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        const CarryWave cw = waves[t >> 6];  // one (wave-uniform) load instead of owner -> image -> prefix
-        iimg_carry_item<T>(cw.H, cw.W, cw.k0 + (t & 63), iimg_ws<T>(ws + cw.ws_off, cw.H, cw.W));
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(IM_THREADS) void batch_iimg_out_kernel(const IimgTile *__restrict__ tiles, int total,
-                                                                    const typename T::In *__restrict__ in,
-                                                                    typename T::Col *__restrict__ ws,
-                                                                    typename T::Out *__restrict__ out) {
-    __shared__ typename T::Row lds[IM_TH * IM_TW];
-    __shared__ typename T::Col rl[IM_TH];
-    for_each_tile(tiles, total, [&](const IimgTile &d) {
-        const typename T::In *src = in + d.in_off;
-        iimg_tile_out<T>(lds, rl, src, d.H, d.W, d.in_pitch, iimg_rows16(src, d.in_pitch), d.tx, d.ty,
-                         iimg_ws<T>(ws + d.ws_off, d.H, d.W), out + d.out_off, d.out_pitch);
-    });
+    iimg_finish(ctl, sm.tag);
 }
 
 }  // namespace
@@ -178,12 +140,10 @@ struct umi_batch_s {
     int *ftiles[NF] = {}, *btiles[NB] = {};
     int *fown[NF] = {}, *bown[NB] = {};  // owner tables: the image of every tile
     int ftotal[NF] = {}, btotal[NB] = {};
-    CarryWave *cwaves = nullptr;               // integral image carry items, per wave of 64
     int *xtiles = nullptr, *xown = nullptr;    // box-sum tiles and their images
-    IimgTile *idesc = nullptr;                 // integral-image tile descriptors
+    IimgBatchTile *idesc[NI] = {};            // integral-image tile descriptors, per configuration
     int nxtiles = 0;
-    int nitiles = 0, nicarry = 0;
-    long long ws_elems = 0;
+    int nitiles[NI] = {};
     std::vector<void *> allocs;
 };
 
@@ -228,23 +188,37 @@ umi_status launch_bilateral(umi_batch b, const float *in, int r, const float *ws
     return launch_check("batch_bilateral_kernel");
 }
 
+template <class T, int C>
+umi_status launch_iimg_c(umi_batch b, const typename T::In *in, typename T::Out *out, uint8_t *ws, hipStream_t st) {
+    constexpr int RPL = IRPL[C];
+    const size_t lds = sizeof(IimgSmem<T, RPL>);
+    const int grid = iimg_grid((const void *)batch_iimg_kernel<T, RPL>, lds, b->nitiles[C], g_iimg_per_cu);
+    if (grid < 1) return set_error(UMI_ERR_HIP, "batch integral image: launch setup failed");
+    hipLaunchKernelGGL((batch_iimg_kernel<T, RPL>), dim3(grid), dim3(IM_THREADS), lds, st,
+                       b->idesc[C], b->nitiles[C], in, out, ws);
+    return launch_check("batch_iimg_kernel");
+}
+
+template <class T>
+size_t batch_iimg_ws(umi_batch b) {
+    size_t m = 0;
+    for (int c = 0; c < NI; ++c) m = std::max(m, iimg_ws_bytes_tiles<typename T::Col>(b->nitiles[c], 16 * IRPL[c]));
+    return m;
+}
+
 template <class T>
 umi_status launch_iimg(umi_batch b, const typename T::In *in, typename T::Out *out, void *ws, size_t ws_bytes,
-                       hipStream_t st, const char *who) {
+                       int config, hipStream_t st, const char *who) {
     UMI_TRY(batch_check(b, in, out, who));
-    const size_t need = (size_t)b->ws_elems * sizeof(typename T::Col);
-    if (need && (!ws || ws_bytes < need)) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
-    auto *w = static_cast<typename T::Col *>(ws);
-    hipLaunchKernelGGL(batch_iimg_totals_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->idesc,
-                       b->nitiles, in, w);
-    UMI_TRY(launch_check("batch_iimg_totals_kernel"));
-    hipLaunchKernelGGL(batch_iimg_carry_kernel<T>, dim3(grid_for(ceil_div(b->nicarry, 256))), dim3(256), 0, st,
-                       b->cwaves, b->nicarry, w);
-    UMI_TRY(launch_check("batch_iimg_carry_kernel"));
-    hipLaunchKernelGGL(batch_iimg_out_kernel<T>, dim3(grid_for(b->nitiles)), dim3(IM_THREADS), 0, st, b->idesc,
-                       b->nitiles, in, w, out);
-    UMI_TRY(launch_check("batch_iimg_out_kernel"));
-    return UMI_OK;
+    const size_t need = batch_iimg_ws<T>(b);
+    if (!ws || ws_bytes < need) return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", who, ws_bytes, need);
+    auto *w = static_cast<uint8_t *>(ws);
+    switch (config < 0 ? 2 : config) {  // default: 64-row tiles, measured fastest on 256 images of 64..1024
+        case 0: return launch_iimg_c<T, 0>(b, in, out, w, st);
+        case 1: return launch_iimg_c<T, 1>(b, in, out, w, st);
+        case 2: return launch_iimg_c<T, 2>(b, in, out, w, st);
+        default: return set_error(UMI_ERR_INVALID, "%s: config %d", who, config);
+    }
 }
 
 }  // namespace
@@ -259,30 +233,28 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
         return set_error(UMI_ERR_INVALID, "umi_batch_create: n=%d or null arrays", n);
     auto *b = new umi_batch_s;
     b->n = n;
-    std::vector<int> fp[NF], bp[NB], itiles(n + 1, 0), icarry(n + 1, 0), xtiles(n + 1, 0);
+    std::vector<int> fp[NF], bp[NB], xtiles(n + 1, 0);
     for (int c = 0; c < NF; ++c) fp[c].assign(n + 1, 0);
     for (int c = 0; c < NB; ++c) bp[c].assign(n + 1, 0);
-    std::vector<long long> wso(n, 0);
-    std::vector<IimgTile> idesc;
-    long long ws = 0;
+    std::vector<IimgBatchTile> idesc[NI];
     for (int i = 0; i < n; ++i) {
         if (H[i] < 1 || W[i] < 1 || in_pitch[i] < W[i] || out_pitch[i] < W[i] || in_off[i] < 0 || out_off[i] < 0) {
             delete b;
             return set_error(UMI_ERR_INVALID, "umi_batch_create: image %d: H=%d W=%d pitches %d %d", i, H[i], W[i],
                              in_pitch[i], out_pitch[i]);
         }
-        const IimgGeom g = iimg_geom(H[i], W[i]);
         b->host.push_back({in_off[i], out_off[i], H[i], W[i], in_pitch[i], out_pitch[i]});
         for (int c = 0; c < NF; ++c) fp[c][i + 1] = fp[c][i] + ceil_div(W[i], FTX[c]) * ceil_div(H[i], FTY[c]);
         for (int c = 0; c < NB; ++c) bp[c][i + 1] = bp[c][i] + ceil_div(W[i], BTX[c]) * ceil_div(H[i], BTY[c]);
-        itiles[i + 1] = itiles[i] + g.ntx * g.nty;
         xtiles[i + 1] = xtiles[i] + ceil_div(W[i], BX_TX) * ceil_div(H[i], BX_TY);
-        icarry[i + 1] = icarry[i] + iimg_carry_items(H[i], W[i]);
-        wso[i] = ws;
-        for (int ty = 0; ty < g.nty; ++ty)
-            for (int tx = 0; tx < g.ntx; ++tx)
-                idesc.push_back({in_off[i], out_off[i], ws, H[i], W[i], in_pitch[i], out_pitch[i], tx, ty});
-        ws += iimg_ws_elems(H[i], W[i]);
+        for (int c = 0; c < NI; ++c) {
+            const IimgGeom g = iimg_geom(H[i], W[i], 16 * IRPL[c]);
+            const long long base = (long long)idesc[c].size();
+            for (int ty = 0; ty < g.nty; ++ty)
+                for (int tx = 0; tx < g.ntx; ++tx)
+                    idesc[c].push_back({in_off[i], out_off[i], base, H[i], W[i], in_pitch[i], out_pitch[i], tx, ty,
+                                        g.ntx, 0});
+        }
     }
     umi_status st = upload(b, b->host, &b->imgs);
     for (int c = 0; c < NF && st == UMI_OK; ++c) {
@@ -302,19 +274,13 @@ umi_status umi_batch_create(int n, const int *H, const int *W, const int64_t *in
     };
     for (int c = 0; c < NF && st == UMI_OK; ++c) st = upload(b, owners(fp[c], 1), &b->fown[c]);
     for (int c = 0; c < NB && st == UMI_OK; ++c) st = upload(b, owners(bp[c], 1), &b->bown[c]);
-    {
-        std::vector<CarryWave> cw;
-        for (int i = 0; i < n; ++i)
-            for (int k = icarry[i]; k < icarry[i + 1]; k += 64) cw.push_back({wso[i], H[i], W[i], k - icarry[i], 0});
-        if (st == UMI_OK) st = upload(b, cw, &b->cwaves);
-    }
     if (st == UMI_OK) st = upload(b, xtiles, &b->xtiles);
     if (st == UMI_OK) st = upload(b, owners(xtiles, 1), &b->xown);
     b->nxtiles = xtiles[n];
-    if (st == UMI_OK) st = upload(b, idesc, &b->idesc);
-    b->nitiles = itiles[n];
-    b->nicarry = icarry[n];
-    b->ws_elems = ws;
+    for (int c = 0; c < NI && st == UMI_OK; ++c) {
+        st = upload(b, idesc[c], &b->idesc[c]);
+        b->nitiles[c] = (int)idesc[c].size();
+    }
     if (st != UMI_OK) {
         for (void *p : b->allocs) (void)hipFree(p);  // the first failure is the status reported
         delete b;
@@ -338,6 +304,7 @@ umi_status umi_batch_configs(int op, int *count) {
     if (!count) return set_error(UMI_ERR_INVALID, "umi_batch_configs: null pointer");
     if (op == 0) *count = NF;
     else if (op == 1) *count = NB;
+    else if (op == 2) *count = NI;
     else return set_error(UMI_ERR_INVALID, "umi_batch_configs: op %d", op);
     return UMI_OK;
 }
@@ -376,18 +343,18 @@ umi_status umi_batch_bilateral_f32(umi_batch b, const float *in, int radius, con
 umi_status umi_batch_iimg_workspace_size(umi_batch b, int kind, size_t *bytes) {
     if (!b || !bytes) return set_error(UMI_ERR_INVALID, "umi_batch_iimg_workspace_size: null pointer");
     if (kind != 0 && kind != 2) return set_error(UMI_ERR_INVALID, "umi_batch_iimg_workspace_size: kind %d", kind);
-    *bytes = (size_t)b->ws_elems * (kind == 0 ? sizeof(uint32_t) : sizeof(double));
+    *bytes = kind == 0 ? batch_iimg_ws<IU8>(b) : batch_iimg_ws<IF32>(b);
     return UMI_OK;
 }
 
 umi_status umi_batch_iimg_u8_u32(umi_batch b, const uint8_t *in, uint32_t *out, void *workspace,
-                                 size_t workspace_bytes, umi_stream s) {
-    return launch_iimg<IU8>(b, in, out, workspace, workspace_bytes, as_stream(s), "umi_batch_iimg_u8_u32");
+                                 size_t workspace_bytes, int config, umi_stream s) {
+    return launch_iimg<IU8>(b, in, out, workspace, workspace_bytes, config, as_stream(s), "umi_batch_iimg_u8_u32");
 }
 
 umi_status umi_batch_iimg_f32(umi_batch b, const float *in, float *out, void *workspace, size_t workspace_bytes,
-                              umi_stream s) {
-    return launch_iimg<IF32>(b, in, out, workspace, workspace_bytes, as_stream(s), "umi_batch_iimg_f32");
+                              int config, umi_stream s) {
+    return launch_iimg<IF32>(b, in, out, workspace, workspace_bytes, config, as_stream(s), "umi_batch_iimg_f32");
 }
 
 umi_status umi_batch_box_sum_u32(umi_batch b, const uint32_t *iimg, int ry, int rx, uint32_t *out, umi_stream s) {

@@ -9,9 +9,13 @@
 // float32 -> float32 with row sums inside a tile in fp32 and every vertical
 // sum and carry in fp64.
 //
-// Design: three launches of the tiled reduce-then-scan in iimg_dev.h (tile
-// totals, carries, tile output).  The input is read twice and the output
-// written once; the carries are small (row, column and tile prefix sums).
+// Design: one launch of the single-pass tile kernel in iimg_dev.h (tiles
+// ordered by an atomic ticket; each publishes its column, row and tile sums
+// and reads those of the tiles above and to its left).  The input is read
+// once and the output written once.
+#include <algorithm>
+#include <atomic>
+#include <cstddef>
 #include <cstring>
 #include "common.h"
 #include "iimg_dev.h"
@@ -19,33 +23,67 @@
 namespace {
 using namespace umi;
 
-template <class T>
-__global__ __launch_bounds__(IM_THREADS) void iimg_totals_kernel(const typename T::In *__restrict__ in, int H, int W,
-                                                                 int ip, typename T::Col *__restrict__ ws) {
-    __shared__ typename T::Row lds[IM_TH * IM_TW];
-    __shared__ typename T::Col tmp[4];
-    iimg_tile_totals<T>(lds, tmp, in, H, W, ip, iimg_rows16(in, ip), blockIdx.x, blockIdx.y, iimg_ws<T>(ws, H, W));
+}  // namespace
+int umi::g_iimg_per_cu = 0;
+namespace {
+// Tile height selection: 0 = by size, else RPL (rows per lane: 1, 2 or 4).
+std::atomic<int> g_iimg_rpl{0};
+
+template <class T, int RPL>
+__global__ __launch_bounds__(IM_THREADS) void iimg_kernel(const typename T::In *__restrict__ in, int H, int W, int ip,
+                                                          typename T::Out *__restrict__ out, int op,
+                                                          uint8_t *__restrict__ ws) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t iimg_lds[];
+    auto &sm = *reinterpret_cast<IimgSmem<T, RPL> *>(iimg_lds);
+    auto *ctl = reinterpret_cast<IimgCtl *>(ws);
+    auto *words = reinterpret_cast<uint64_t *>(ws + IM_CTL_BYTES);
+    const IimgGeom g = iimg_geom(H, W, 16 * RPL);
+    const int total = g.ntx * g.nty;
+    const bool rows16 = iimg_rows16(in, ip);
+    iimg_begin(sm, ctl);
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const IimgTileRef d{0, H, W, ip, op, t % g.ntx, t / g.ntx, g.ntx};
+        iimg_tile<T, RPL>(sm, d, in, rows16, out, ctl, words);
+    }
+    iimg_finish(ctl, sm.tag);
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void iimg_carry_kernel(int H, int W, typename T::Col *__restrict__ ws) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;  // whole waves: the item count is a multiple of 64
-    if (k < iimg_carry_items(H, W)) iimg_carry_item<T>(H, W, k, iimg_ws<T>(ws, H, W));
+// Zeroes a workspace the call allocated itself (a kernel on the call's stream).
+__global__ void iimg_zero_kernel(uint4 *__restrict__ p, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = uint4{0, 0, 0, 0};
 }
 
-template <class T>
-__global__ __launch_bounds__(IM_THREADS) void iimg_out_kernel(const typename T::In *__restrict__ in, int H, int W,
-                                                              int ip, typename T::Col *__restrict__ ws,
-                                                              typename T::Out *__restrict__ out, int op) {
-    __shared__ typename T::Row lds[IM_TH * IM_TW];
-    __shared__ typename T::Col rl[IM_TH];
-    iimg_tile_out<T>(lds, rl, in, H, W, ip, iimg_rows16(in, ip), blockIdx.x, blockIdx.y, iimg_ws<T>(ws, H, W), out,
-                     op);
+// RPL for an image: the tallest tile that still gives a tile per CU (256),
+// else RPL 1 (1920 x 1080: RPL 2, 272 tiles, measured fastest of the three).
+int iimg_pick_rpl(int H, int W) {
+    const int want = g_iimg_rpl.load();
+    if (want) return want;
+    for (int rpl : {4, 2})
+        if ((long long)ceil_div(W, IM_TW) * ceil_div(H, 16 * rpl) >= 256) return rpl;
+    return 1;
 }
 
 template <class T>
 size_t iimg_ws_bytes(int H, int W) {
-    return (size_t)iimg_ws_elems(H, W) * sizeof(typename T::Col);
+    // enough for any tile height the call may pick
+    size_t m = 0;
+    for (int rpl : {1, 2, 4}) {
+        const IimgGeom g = iimg_geom(H, W, 16 * rpl);
+        m = std::max(m, iimg_ws_bytes_tiles<typename T::Col>((long long)g.ntx * g.nty, 16 * rpl));
+    }
+    return m;
+}
+
+template <class T, int RPL>
+umi_status iimg_launch_rpl(const typename T::In *in, int H, int W, int ip, typename T::Out *out, int op, uint8_t *ws,
+                           hipStream_t st, const char *name) {
+    const size_t lds = sizeof(IimgSmem<T, RPL>);
+    const IimgGeom g = iimg_geom(H, W, 16 * RPL);
+    const int grid = iimg_grid((const void *)iimg_kernel<T, RPL>, lds, g.ntx * g.nty, g_iimg_per_cu);
+    if (grid < 1) return set_error(UMI_ERR_HIP, "%s: launch setup failed", name);
+    hipLaunchKernelGGL((iimg_kernel<T, RPL>), dim3(grid), dim3(IM_THREADS), lds, st, in, H, W, ip, out, op, ws);
+    return launch_check(name);
 }
 
 template <class T>
@@ -55,7 +93,6 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     if (H < 0 || W < 0 || ip < W || op < W)
         return set_error(UMI_ERR_INVALID, "%s: H=%d W=%d in_pitch=%d out_pitch=%d", name, H, W, ip, op);
     if (H == 0 || W == 0) return UMI_OK;
-    const IimgGeom g = iimg_geom(H, W);
     const size_t need = iimg_ws_bytes<T>(H, W);
     hipStream_t st = as_stream(s);
     void *tmp = ws;
@@ -63,20 +100,24 @@ umi_status iimg_launch(const typename T::In *in, int H, int W, int ip, typename 
     if (!tmp) {
         UMI_TRY(hip_check(hipMalloc(&tmp, need), "hipMalloc (integral image workspace)"));
         own = true;
+        // a fresh allocation may hold an earlier workspace's words: zero all of it
+        const size_t n16 = need / 16;
+        hipLaunchKernelGGL(iimg_zero_kernel, dim3((unsigned)std::min<size_t>((n16 + 255) / 256, 4096)), dim3(256), 0, st,
+                           static_cast<uint4 *>(tmp), n16);
+        umi_status z = launch_check("iimg_zero_kernel");
+        if (z != UMI_OK) {
+            (void)hipFree(tmp);
+            return z;
+        }
     } else if (ws_bytes < need) {
         return set_error(UMI_ERR_INVALID, "%s: workspace %zu B < %zu B", name, ws_bytes, need);
     }
-    auto *w = static_cast<typename T::Col *>(tmp);
-    hipLaunchKernelGGL(iimg_totals_kernel<T>, dim3(g.ntx, g.nty), dim3(IM_THREADS), 0, st, in, H, W, ip, w);
-    umi_status rc = launch_check(name);
-    if (rc == UMI_OK) {
-        hipLaunchKernelGGL(iimg_carry_kernel<T>, dim3(ceil_div(iimg_carry_items(H, W), 256)), dim3(256), 0, st, H, W,
-                           w);
-        rc = launch_check(name);
-    }
-    if (rc == UMI_OK) {
-        hipLaunchKernelGGL(iimg_out_kernel<T>, dim3(g.ntx, g.nty), dim3(IM_THREADS), 0, st, in, H, W, ip, w, out, op);
-        rc = launch_check(name);
+    auto *w = static_cast<uint8_t *>(tmp);
+    umi_status rc;
+    switch (iimg_pick_rpl(H, W)) {
+        case 4: rc = iimg_launch_rpl<T, 4>(in, H, W, ip, out, op, w, st, name); break;
+        case 2: rc = iimg_launch_rpl<T, 2>(in, H, W, ip, out, op, w, st, name); break;
+        default: rc = iimg_launch_rpl<T, 1>(in, H, W, ip, out, op, w, st, name); break;
     }
     if (own) {
         hipError_t e = hipStreamSynchronize(st);
@@ -174,6 +215,32 @@ umi_status umi_iimg_workspace_size(int H, int W, int kind, size_t *bytes) {
         case 2: *bytes = iimg_ws_bytes<IF32>(H, W); break;
         default: return set_error(UMI_ERR_INVALID, "umi_iimg_workspace_size: kind %d", kind);
     }
+    return UMI_OK;
+}
+
+umi_status umi_iimg_workspace_status(void *workspace, int *timed_out) {
+    if (!workspace || !timed_out) return set_error(UMI_ERR_INVALID, "umi_iimg_workspace_status: null pointer");
+    uint32_t f = 0;
+    auto *flag = reinterpret_cast<uint8_t *>(workspace) + offsetof(IimgCtl, timeout);
+    UMI_TRY(hip_check(hipMemcpy(&f, flag, 4, hipMemcpyDeviceToHost), "hipMemcpy (integral image status)"));
+    *timed_out = f != 0;
+    if (f) {
+        const uint32_t z = 0;
+        UMI_TRY(hip_check(hipMemcpy(flag, &z, 4, hipMemcpyHostToDevice), "hipMemcpy (integral image status)"));
+    }
+    return UMI_OK;
+}
+
+umi_status umi_iimg_set_grid(int per_cu) {
+    if (per_cu < 0 || per_cu > 8) return set_error(UMI_ERR_INVALID, "umi_iimg_set_grid: %d (0 .. 8)", per_cu);
+    g_iimg_per_cu = per_cu;
+    return UMI_OK;
+}
+
+umi_status umi_iimg_set_config(int rows_per_lane) {
+    if (rows_per_lane != 0 && rows_per_lane != 1 && rows_per_lane != 2 && rows_per_lane != 4)
+        return set_error(UMI_ERR_INVALID, "umi_iimg_set_config: %d (0, 1, 2 or 4)", rows_per_lane);
+    g_iimg_rpl.store(rows_per_lane);
     return UMI_OK;
 }
 

@@ -1,71 +1,63 @@
-// Device-side bodies of the integral image, shared by the one-image kernels
-// (iimg.hip) and the batch kernels (batch.hip).  Semantics:
+// Device-side bodies of the integral image, shared by the one-image kernel
+// (iimg.hip) and the batch kernel (batch.hip).  Semantics:
 // main/iimg.cu:11-18,37-82,84-112 (johnjohnlin/UMI): out[y][x] is the sum of
 // in[y'][x'] over y' <= y, x' <= x.
 //
-// Tiled reduce-then-scan, three passes over tiles of IM_TH x IM_TW:
-//  A (totals)  per tile: the row totals of its IM_TH rows, L[y][tx]; the
-//              within-tile prefix over x of its column totals, C[ty][x]; the
-//              tile total, T[ty][tx].  Reads the input once.
-//  B (carries) per row y: L becomes the exclusive prefix over tx (the sum of
-//              the row left of the tile); per column x: C becomes the
-//              exclusive prefix over ty (the sum, above the tile, of the
-//              tile-local row prefixes); per tile column tx: K[ty][tx] = the
-//              sum of T over tiles above and to the left.
-//  C (output)  per tile: the within-tile row prefix lrp(y, x), then down each
-//              column acc = K + C + sum over the tile's rows so far of
-//              (L[y] + lrp(y, x)).  Reads the input again, writes the output
-//              once.
-// out(y, x) = K(ty, tx) + C(ty, x) + sum_{y' = ty0 .. y} (L(y', tx) + lrp(y', x))
-// is the summed-area table split into the part above-left of the tile, above
-// it, left of it and inside it.  Integer forms are exact (mod 2^32) in any
-// order; the float form sums rows within a tile in fp32 and everything
-// vertical and every carry in fp64.
+// One launch, input read once.  The image is cut into tiles of TH x 256
+// (TH = 16 * RPL, RPL rows per lane), numbered row-major within an image,
+// images in order; the grid walks them in that order (a workgroup per tile,
+// or a persistent grid taking tiles b, b + G, ...), so a tile waits only for
+// tiles of smaller number, which have started before it.  Per tile, with
+// (ty, tx) its tile row and column and y0 its first row:
+//   1. load (lane: RPL rows x 16 columns), in-tile row prefixes p(y, x) by
+//      16-lane DPP scans into LDS; publish the row totals rt(y);
+//   2. L(y) = sum_{tx' < tx} rt(ty, tx', y), the sum of row y left of the
+//      tile, and its prefix over the tile's rows LP(y);
+//   3. decoupled look-back down the tile column: publish the aggregate
+//      A(x) = sum over the tile's rows of (L(y) + p(y, x)), walk up through
+//      the tiles above adding aggregates until one shows its inclusive value
+//      I(x) = S(y0' + TH - 1, x) (the integral image's value on its last row),
+//      giving U(x) = S(y0 - 1, x); publish I(x) = U(x) + A(x) over A(x);
+//   4. out(y, x) = U(x) + LP(y) + sum_{y'=y0..y} p(y', x): a thread per
+//      column walks the tile's rows in LDS (coalesced 4-byte stores).
+// A published word is (tag << 33 | state << 32 | value), state 0 = aggregate
+// (or row total), 1 = inclusive.  The tag identifies the call (the
+// workspace's epoch + 1), so a word left by an earlier call never passes for
+// a current one and no fence orders the value with a flag.  fp64 values are
+// two words, one per half, each with tag and state.  The stores and loads are
+// relaxed atomics at agent scope, coherent across the XCDs' L2 caches.  The
+// last workgroup to finish (counted in groups on separate cache lines)
+// resets the counters and advances the epoch, so the workspace needs zeroing
+// once, before its first use, and no launch clears anything (graph replays
+// included).  Every wait is bounded by the wall clock (IM_WAIT_TICKS); a wait
+// that runs out sets a flag (umi_iimg_workspace_status) and the kernel still
+// ends.
+// Integer forms are exact (mod 2^32) in any order; the float form sums rows
+// within a tile in fp32 and everything vertical and every carry in fp64.
 #pragma once
+#include <algorithm>
 #include "common.h"
+#include "wave.h"
 
 namespace umi {
 
-template <typename Tin, typename Trow, typename Tcol, typename Tout>
+// In: element; Row: in-tile row sums; Col: vertical sums and carries; Out:
+// output; Lds: in-tile row prefixes as held in LDS (u8 rows of 256 sum to
+// at most 65280, so 16 bits hold them).
+template <typename Tin, typename Trow, typename Tcol, typename Tout, typename Tlds>
 struct IimgTypes {
     using In = Tin;
     using Row = Trow;
     using Col = Tcol;
     using Out = Tout;
+    using Lds = Tlds;
 };
-using IU8 = IimgTypes<uint8_t, uint32_t, uint32_t, uint32_t>;
-using II32 = IimgTypes<int32_t, uint32_t, uint32_t, int32_t>;
-using IF32 = IimgTypes<float, float, double, float>;
+using IU8 = IimgTypes<uint8_t, uint32_t, uint32_t, uint32_t, uint16_t>;
+using II32 = IimgTypes<int32_t, uint32_t, uint32_t, int32_t, uint32_t>;
+using IF32 = IimgTypes<float, float, double, float, float>;
 
-constexpr int IM_THREADS = 256;
-constexpr int IM_TW = 256, IM_TH = 16;  // thread (r = t / 16, s = t % 16) holds row r, columns 16 s .. 16 s + 15
-
-struct IimgGeom {
-    int ntx, nty;
-};
-__host__ __device__ inline IimgGeom iimg_geom(int H, int W) { return {ceil_div(W, IM_TW), ceil_div(H, IM_TH)}; }
-
-// Workspace of one image, in Col elements: L[H][ntx], C[nty][W], T[nty][ntx], K[nty][ntx].
-__host__ __device__ inline long long iimg_ws_elems(int H, int W) {
-    const IimgGeom g = iimg_geom(H, W);
-    return (long long)H * g.ntx + (long long)g.nty * W + 2LL * g.nty * g.ntx;
-}
-
-template <class T>
-struct IimgWs {
-    typename T::Col *L, *C, *T_, *K;
-};
-template <class T>
-__device__ inline IimgWs<T> iimg_ws(typename T::Col *ws, int H, int W) {
-    const IimgGeom g = iimg_geom(H, W);
-    IimgWs<T> w;
-    w.L = ws;
-    w.C = w.L + (size_t)H * g.ntx;
-    w.T_ = w.C + (size_t)g.nty * W;
-    w.K = w.T_ + (size_t)g.nty * g.ntx;
-    return w;
-}
-
+// Element forms: uint8 -> uint32 and int32 -> int32 (exact, wrapping),
+// float -> float with fp32 row sums inside a tile and fp64 vertical sums.
 // True when every row of `in` starts 16-byte aligned.
 template <class In>
 __device__ inline bool iimg_rows16(const In *in, int pitch) {
@@ -150,201 +142,287 @@ __device__ inline void iimg_load16(typename T::Row (&v)[16], const typename T::I
     }
 }
 
-// Inclusive prefix over the 256 threads of the workgroup (tmp: 4 values of
-// LDS).  Contains a barrier; the caller separates reuses of tmp by another.
-template <class V>
-__device__ inline V iimg_block_scan(V v, V *tmp) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// ------------------------------------------------------------ single pass
+constexpr int IM_THREADS = 256, IM_TW = 256;
+constexpr long long IM_WAIT_TICKS = 20000000;  // 0.2 s of the 100 MHz wall clock per wait
+
+// Control block at the start of the workspace: the epoch, the timeout flag,
+// the group counter and the per-group completion counters, each counter on
+// its own 128-byte line.
+constexpr int IM_GROUP = 32, IM_MAX_GROUPS = 64, IM_LINE_U32 = 32;
+constexpr int IM_MAX_GRID = IM_GROUP * IM_MAX_GROUPS;
+struct IimgCtl {
+    uint32_t epoch, timeout, pad0[IM_LINE_U32 - 2];
+    uint32_t top, pad1[IM_LINE_U32 - 1];
+    uint32_t grp[IM_MAX_GROUPS][IM_LINE_U32];
+};
+constexpr size_t IM_CTL_BYTES = sizeof(IimgCtl);
+static_assert(IM_CTL_BYTES % 256 == 0, "control block keeps the words 256-byte aligned");
+
+// Grid of a single-pass launch over `tiles` tiles: per_cu > 0 gives a
+// persistent grid of per_cu workgroups per CU; 0 a workgroup per tile.
+// Sets the kernel's LDS limit; < 1 on failure.
+inline int iimg_grid(const void *kernel, size_t lds, int tiles, int per_cu) {
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 0;
+    if (per_cu <= 0) return tiles;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return std::min(tiles, per_cu * cus);
+}
+extern int g_iimg_per_cu;  // launch shape of the single-pass kernels (umi_iimg_set_grid)
+
+// Published words per value: 1 (32-bit sums) or 2 (fp64 halves).
+template <class Col>
+constexpr int pub_words() { return sizeof(Col) == 8 ? 2 : 1; }
+
+struct IimgGeom {
+    int ntx, nty;
+};
+__host__ __device__ inline IimgGeom iimg_geom(int H, int W, int th) { return {ceil_div(W, IM_TW), ceil_div(H, th)}; }
+
+// Words published per tile: the column words (aggregate, then inclusive) of
+// 256 columns, then the row totals of th rows.
+template <class Col>
+__host__ __device__ inline long long iimg_tile_words(int th) {
+    return (long long)(IM_TW + th) * pub_words<Col>();
+}
+
+// Workspace bytes for `tiles` tiles of height th.
+template <class Col>
+__host__ __device__ inline size_t iimg_ws_bytes_tiles(long long tiles, int th) {
+    const size_t b = IM_CTL_BYTES + (size_t)tiles * (size_t)iimg_tile_words<Col>(th) * sizeof(uint64_t);
+    return (b + 15) & ~(size_t)15;
+}
+
+__device__ inline uint64_t pub_word(uint32_t tag, uint32_t state, uint32_t half) {
+    return ((uint64_t)tag << 33) | ((uint64_t)state << 32) | half;
+}
+__device__ inline void pub_put(uint64_t *p, uint32_t tag, uint32_t state, uint32_t v) {
+    __hip_atomic_store(p, pub_word(tag, state, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void pub_put(uint64_t *p, uint32_t tag, uint32_t state, double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    __hip_atomic_store(p, pub_word(tag, state, (uint32_t)b), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, pub_word(tag, state, (uint32_t)(b >> 32)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t pub_ld(const uint64_t *p) {
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One published value: its words loaded once, then polled until they carry
+// the call's tag and agree on the state (or the wait runs out, which is
+// recorded in ctl->timeout).
+template <class Col>
+struct PubVal {
+    uint64_t w[pub_words<Col>()];
+    __device__ inline void load(const uint64_t *p) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const V n = __shfl_up(v, d);
-        if (lane >= d) v += n;
+        for (int k = 0; k < pub_words<Col>(); ++k) w[k] = pub_ld(p + k);
     }
-    if (lane == 63) tmp[wv] = v;
-    __syncthreads();
-    V off = 0;
-    for (int i = 0; i < wv; ++i) off += tmp[i];
-    return v + off;
-}
+    __device__ inline bool ready(uint32_t tag) const {
+        bool r = (uint32_t)(w[0] >> 33) == tag;
+#pragma unroll
+        for (int k = 1; k < pub_words<Col>(); ++k) r = r && (w[k] >> 32) == (w[0] >> 32);
+        return r;
+    }
+    __device__ inline uint32_t state() const { return (uint32_t)(w[0] >> 32) & 1u; }
+    __device__ inline Col value() const {
+        if constexpr (sizeof(Col) == 8)
+            return __builtin_bit_cast(Col, (w[0] & 0xffffffffu) | (w[1] << 32));
+        else
+            return (Col)(uint32_t)w[0];
+    }
+    __device__ inline void wait(const uint64_t *p, uint32_t tag, IimgCtl *ctl) {
+        if (ready(tag)) return;
+        const uint64_t t0 = wall_clock64();
+        do {
+            __builtin_amdgcn_s_sleep(2);
+            load(p);
+        } while (!ready(tag) && (long long)(wall_clock64() - t0) < IM_WAIT_TICKS);
+        if (!ready(tag)) __hip_atomic_store(&ctl->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
 
-// Pass A for tile (tx, ty).  lds: IM_TH x IM_TW Row values; tmp: 4 Col.
-template <class T>
-__device__ inline void iimg_tile_totals(typename T::Row *lds, typename T::Col *tmp,
-                                        const typename T::In *__restrict__ in, int H, int W, int ip, bool rows16,
-                                        int tx, int ty, const IimgWs<T> &w) {
-    using Row = typename T::Row;
-    using Col = typename T::Col;
-    const int t = threadIdx.x, r = t >> 4, s = t & 15;
-    const int ntx = iimg_geom(H, W).ntx;
-    const int y = ty * IM_TH + r, x = tx * IM_TW + 16 * s;
-    Row v[16];
-    iimg_load16<T>(v, in, H, W, ip, rows16, y, x);
-    Row rs = 0;
+// Sum of n published values at p + i * stride (i < n), loads issued in
+// batches of 16 ahead of the waits.
+template <class Col>
+__device__ inline Col pub_sum(const uint64_t *p, long long stride, int n, uint32_t tag, IimgCtl *ctl) {
+    constexpr int B = 16;
+    Col acc = 0;
+    for (int i0 = 0; i0 < n; i0 += B) {
+        PubVal<Col> v[B];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) rs += v[j];
+        for (int u = 0; u < B; ++u)
+            if (i0 + u < n) v[u].load(p + (i0 + u) * stride);
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) rs += __shfl_xor(rs, m, 16);
-    if (s == 0 && y < H) w.L[(size_t)y * ntx + tx] = (Col)rs;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) lds[r * IM_TW + 16 * s + j] = v[j];
-    __syncthreads();
-    Col c = 0;
-#pragma unroll
-    for (int rr = 0; rr < IM_TH; ++rr) c += (Col)lds[rr * IM_TW + t];
-    const Col incl = iimg_block_scan(c, tmp);
-    const int xc = tx * IM_TW + t;
-    if (xc < W) w.C[(size_t)ty * W + xc] = incl;
-    if (t == IM_THREADS - 1) w.T_[ty * ntx + tx] = incl;  // columns past W hold zeros
-    __syncthreads();
-}
-
-// Pass B items of one image, in whole waves: 64 * ntx corner items (a wave
-// per tile column; lanes over tile rows), H row items padded to a multiple
-// of 64, then 64 * ceil(W / 8) column items (a wave per 8 columns; lane =
-// column c + 8 * segment q, the 8 segments splitting the tile rows).
-__host__ __device__ inline int iimg_carry_items(int H, int W) {
-    return 64 * iimg_geom(H, W).ntx + round_up(H, 64) + 64 * ceil_div(W, 8);
-}
-
-// Pass B, item k of [0, iimg_carry_items(H, W)).  Loads are issued in
-// batches ahead of the dependent sums.
-template <class T>
-__device__ inline void iimg_carry_item(int H, int W, int k, const IimgWs<T> &w) {
-    using Col = typename T::Col;
-    const IimgGeom g = iimg_geom(H, W);
-    constexpr int U = 16;
-    const int nc = 64 * g.ntx, nr = round_up(H, 64);
-    const int lane = k & 63;
-    if (k < nc) {
-        // K[ty][tx] = sum over ty' < ty of P(ty'), P(ty') = sum over tx' < tx of T[ty'][tx']
-        const int tx = k >> 6;
-        const Col *__restrict__ Tp = w.T_;
-        Col *__restrict__ Kp = w.K;
-        Col off = 0;
-        for (int ty0 = 0; ty0 < g.nty; ty0 += 64) {
-            const int ty = ty0 + lane;
-            Col p = 0;
-            if (ty < g.nty)
-                for (int i = 0; i < tx; ++i) p += Tp[ty * g.ntx + i];
-            Col incl = p;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const Col n = __shfl_up(incl, d);
-                if (lane >= d) incl += n;
+        for (int u = 0; u < B; ++u)
+            if (i0 + u < n) {
+                v[u].wait(p + (i0 + u) * stride, tag, ctl);
+                acc += v[u].value();
             }
-            Col excl = __shfl_up(incl, 1);
-            if (lane == 0) excl = 0;
-            if (ty < g.nty) Kp[ty * g.ntx + tx] = off + excl;
-            off += __shfl(incl, 63);
-        }
-    } else if (k < nc + nr) {
-        const int y = k - nc;
-        if (y >= H) return;
-        Col *__restrict__ p = w.L + (size_t)y * g.ntx;
-        Col run = 0;
-        for (int i0 = 0; i0 < g.ntx; i0 += U) {
-            Col v[U];
+    }
+    return acc;
+}
+
+// Decoupled look-back over n predecessors at p - stride, p - 2 stride, ...
+// (nearest first): the sum of their aggregates up to and including the
+// nearest inclusive value.  Loads go out in windows of LB_WIN (16 values of
+// 32 bits, 8 of 64); the first window is loaded by prefetch() ahead of the
+// work that precedes resolve().
+template <class Col>
+struct Lookback {
+    static constexpr int LB_WIN = sizeof(Col) == 8 ? 8 : 16;
+    PubVal<Col> v[LB_WIN];
+    __device__ inline void prefetch(const uint64_t *p, long long stride, int n) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = i0 + u < g.ntx ? p[i0 + u] : (Col)0;
+        for (int u = 0; u < LB_WIN; ++u)
+            if (u < n) v[u].load(p - (u + 1) * stride);
+    }
+    __device__ inline Col resolve(const uint64_t *p, long long stride, int n, uint32_t tag, IimgCtl *ctl) {
+        Col acc = 0;
+        for (int i0 = 0; i0 < n; i0 += LB_WIN) {
+            if (i0 > 0) prefetch(p - i0 * stride, stride, n - i0);
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (i0 + u < g.ntx) {
-                    p[i0 + u] = run;
-                    run += v[u];
+            for (int u = 0; u < LB_WIN; ++u)
+                if (i0 + u < n) {
+                    v[u].wait(p - (i0 + u + 1) * stride, tag, ctl);
+                    acc += v[u].value();
+                    if (v[u].state()) return acc;
                 }
         }
+        return acc;
+    }
+};
+
+// The tile's place in the call: image geometry and the index, among the
+// call's published tiles, of the image's tile (0, 0).
+struct IimgTileRef {
+    long long tbase;
+    int H, W, ip, op, tx, ty, ntx;
+};
+
+template <class T, int RPL>
+struct IimgSmem {
+    static constexpr int TH = 16 * RPL;
+    typename T::Lds p[TH][IM_TW];  // in-tile row prefixes
+    typename T::Col lp[TH];        // prefix of L over the tile's rows
+    uint32_t tag;
+};
+
+// Reads the call's tag (thread 0; broadcast through LDS).
+template <class S>
+__device__ inline void iimg_begin(S &sm, IimgCtl *ctl) {
+    if (threadIdx.x == 0)
+        sm.tag = (__hip_atomic_load(&ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u) & 0x7fffffffu;
+    __syncthreads();
+}
+
+// Counts the workgroup done; the call's last workgroup resets the counters
+// and advances the epoch for the next call.  Workgroups count in groups of
+// at least IM_GROUP consecutive ids, at most IM_MAX_GROUPS groups.
+__device__ inline void iimg_finish(IimgCtl *ctl, uint32_t tag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int G = (int)gridDim.x, gs = max(IM_GROUP, ceil_div(G, IM_MAX_GROUPS));
+        const int gi = (int)blockIdx.x / gs, ng = ceil_div(G, gs);
+        const uint32_t gsize = (uint32_t)min(gs, G - gi * gs);
+        uint32_t *gc = &ctl->grp[gi][0];
+        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&ctl->top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (uint32_t)ng - 1) {
+                __hip_atomic_store(&ctl->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&ctl->epoch, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+// One tile of the single pass (the whole workgroup).  ws: the call's
+// published words (after the control block).
+template <class T, int RPL>
+__device__ inline void iimg_tile(IimgSmem<T, RPL> &sm, const IimgTileRef &d, const typename T::In *__restrict__ in,
+                                 bool rows16, typename T::Out *__restrict__ out, IimgCtl *ctl,
+                                 uint64_t *__restrict__ ws) {
+    using Row = typename T::Row;
+    using Col = typename T::Col;
+    constexpr int TH = 16 * RPL, PW = pub_words<Col>();
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, s = lane & 15;
+    const int grp = wv * 4 + g;  // row group: rows grp * RPL .. + RPL - 1
+    const int y0 = d.ty * TH, x0 = d.tx * IM_TW;
+    const uint32_t tag = sm.tag;
+    const long long tw = iimg_tile_words<Col>(TH);
+    const long long me = d.tbase + (long long)d.ty * d.ntx + d.tx;
+    uint64_t *rec = ws + me * tw;
+
+    // 1. load RPL rows of 16 elements; in-tile row prefixes into LDS; row totals published
+    {
+        Row v[RPL][16];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i)
+            iimg_load16<T>(v[i], in, d.H, d.W, d.ip, rows16, y0 + grp * RPL + i, x0 + 16 * s);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int r = grp * RPL + i;
+#pragma unroll
+            for (int j = 1; j < 16; ++j) v[i][j] += v[i][j - 1];
+            const Row incl = row16_inclusive_scan(v[i][15]);
+            const Row excl =
+                __builtin_bit_cast(Row, dpp_u32<DPP_ROW_SHR1, 0xf, true>(__builtin_bit_cast(uint32_t, incl)));
+            using Lds = typename T::Lds;
+            constexpr int PER16 = 16 / (int)sizeof(Lds);  // LDS values per 16-byte store
+#pragma unroll
+            for (int j = 0; j < 16; j += PER16) {
+                Lds q[PER16];
+#pragma unroll
+                for (int k = 0; k < PER16; ++k) q[k] = (Lds)(excl + v[i][j + k]);
+                *reinterpret_cast<uint4 *>(&sm.p[r][16 * s + j]) = __builtin_bit_cast(uint4, q);
+            }
+            if (s == 15) pub_put(rec + (long long)(IM_TW + r) * PW, tag, 0u, (Col)incl);
+        }
+    }
+    // 2. L(y) from the tiles to the left, and its prefix LP over the tile's rows (one wave);
+    // the first look-back window up the tile column is loaded meanwhile
+    uint64_t *cw = rec + (long long)t * PW;
+    const long long up = (long long)d.ntx * tw;
+    Lookback<Col> lb;
+    lb.prefetch(cw, up, d.ty);
+    if (wv == 0) {
+        Col l = 0;
+        if (lane < TH)
+            l = pub_sum<Col>(ws + (me - d.tx) * tw + (long long)(IM_TW + lane) * PW, tw, d.tx, tag, ctl);
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const Col n = __shfl_up(l, dd);
+            if (lane >= dd) l += n;
+        }
+        if (lane < TH) sm.lp[lane] = l;
+    }
+    __syncthreads();
+    // 3. column t: aggregate A = sum_y (L(y) + p(y, t)) = LP(TH - 1) + sum_y p(y, t); look-back up the tile column
+    Col a = sm.lp[TH - 1];
+#pragma unroll 8
+    for (int y = 0; y < TH; ++y) a += (Col)sm.p[y][t];
+    Col u = 0;
+    if (d.ty == 0) {
+        pub_put(cw, tag, 1u, a);
     } else {
-        // C[ty][x] becomes the exclusive prefix over ty: segment totals, a
-        // scan of the 8 segments across lanes, then the segments rewritten
-        const int q = lane >> 3, x = ((k - nc - nr) >> 6) * 8 + (lane & 7);
-        const int seg = ceil_div(g.nty, 8), j0 = q * seg, j1 = min(g.nty, j0 + seg);
-        Col *__restrict__ p = w.C + min(x, W - 1);
-        const bool own = x < W;
-        Col tot = 0;
-        // segments of at most U tile rows (every height up to 16 * 8 * U) keep
-        // their values in registers between the two passes: one load round trip
-        Col keep[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) keep[u] = 0;
-        const bool one = seg <= U;  // uniform: then every segment fits one batch of loads
-        for (int j = j0; j < j1; j += U) {
-            Col v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = own && j + u < j1 ? p[(size_t)(j + u) * W] : (Col)0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                tot += v[u];
-                keep[u] = v[u];
-            }
-        }
-        Col incl = tot;
-#pragma unroll
-        for (int d = 8; d < 64; d <<= 1) {
-            const Col n = __shfl_up(incl, d);
-            if (lane >= d) incl += n;
-        }
-        Col run = __shfl_up(incl, 8);
-        if (q == 0) run = 0;
-        if (one) {
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (own && j0 + u < j1) {
-                    p[(size_t)(j0 + u) * W] = run;
-                    run += keep[u];
-                }
-            return;
-        }
-        for (int j = j0; j < j1; j += U) {
-            Col v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = own && j + u < j1 ? p[(size_t)(j + u) * W] : (Col)0;
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (own && j + u < j1) {
-                    p[(size_t)(j + u) * W] = run;
-                    run += v[u];
-                }
+        pub_put(cw, tag, 0u, a);
+        u = lb.resolve(cw, up, d.ty, tag, ctl);
+        pub_put(cw, tag, 1u, u + a);
+    }
+    // 4. out(y, t) = U + LP(y) + sum_{y' <= y} p(y', t)
+    const int xc = x0 + t;
+    if (xc < d.W) {
+        const int nr = min(TH, d.H - y0);
+        Col acc = u;
+        typename T::Out *o = out + (size_t)y0 * d.op + xc;
+        for (int y = 0; y < nr; ++y) {
+            acc += (Col)sm.p[y][t];
+            o[(size_t)y * d.op] = (typename T::Out)(acc + sm.lp[y]);
         }
     }
-}
-
-// Pass C for tile (tx, ty).  lds: IM_TH x IM_TW Row values; rl: IM_TH Col.
-template <class T>
-__device__ inline void iimg_tile_out(typename T::Row *lds, typename T::Col *rl, const typename T::In *__restrict__ in,
-                                     int H, int W, int ip, bool rows16, int tx, int ty, const IimgWs<T> &w,
-                                     typename T::Out *__restrict__ out, int op) {
-    using Row = typename T::Row;
-    using Col = typename T::Col;
-    const int t = threadIdx.x, r = t >> 4, s = t & 15;
-    const int ntx = iimg_geom(H, W).ntx;
-    const int y = ty * IM_TH + r, x = tx * IM_TW + 16 * s;
-    Row v[16];
-    iimg_load16<T>(v, in, H, W, ip, rows16, y, x);
-#pragma unroll
-    for (int j = 1; j < 16; ++j) v[j] += v[j - 1];
-    Row incl = v[15];
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-        const Row n = __shfl_up(incl, d, 16);
-        if (s >= d) incl += n;
-    }
-    Row excl = __shfl_up(incl, 1, 16);
-    if (s == 0) excl = 0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) lds[r * IM_TW + 16 * s + j] = excl + v[j];
-    if (t < IM_TH) {
-        const int yy = ty * IM_TH + t;
-        rl[t] = yy < H ? w.L[(size_t)yy * ntx + tx] : (Col)0;
-    }
-    __syncthreads();
-    const int xc = tx * IM_TW + t;
-    Col acc = w.K[ty * ntx + tx] + (xc < W ? w.C[(size_t)ty * W + xc] : (Col)0);
-    const int nr = min(IM_TH, H - ty * IM_TH);
-    for (int rr = 0; rr < nr; ++rr) {
-        acc += rl[rr] + (Col)lds[rr * IM_TW + t];
-        if (xc < W) out[(size_t)(ty * IM_TH + rr) * op + xc] = (typename T::Out)acc;
-    }
-    __syncthreads();
+    __syncthreads();  // LDS reuse by the workgroup's next tile
 }
 
 // ---------------------------------------------------------------- box sums

@@ -27,6 +27,20 @@ __device__ inline uint32_t min8_u32(uint32_t k) {
     return k;
 }
 
+// Inclusive prefix sum within each row of 16 lanes (lane order), 32-bit
+// values; row_shr with bound_ctrl feeds zero bits (0 or 0.0f) into the first
+// lanes of each row.
+template <typename T>
+__device__ inline T row16_inclusive_scan(T v) {
+    static_assert(sizeof(T) == 4, "32-bit values");
+    auto sh = [](T x, auto step) -> T { return __builtin_bit_cast(T, step(__builtin_bit_cast(uint32_t, x))); };
+    v += sh(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR1, 0xf, true>(b); });
+    v += sh(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR2, 0xf, true>(b); });
+    v += sh(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR4, 0xf, true>(b); });
+    v += sh(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR8, 0xf, true>(b); });
+    return v;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave (lane order), with
 // operator `add` on values moved through DPP as 32-bit words.
 template <typename T, typename Add>

@@ -178,6 +178,17 @@ _IIMG = {np.dtype(np.uint8): (0, "umi_iimg_u8_u32", np.uint32), np.dtype(np.int3
          np.dtype(np.float32): (2, "umi_iimg_f32", np.float32)}
 
 
+def iimg_set_config(rows_per_lane: int) -> None:
+    """Tile height of the one-image integral image: rows per lane 1, 2, 4 (TH
+    16, 32, 64) or 0 = chosen by size.  Results do not depend on it."""
+    call("umi_iimg_set_config", rows_per_lane)
+
+
+def iimg_workspace(H: int, W: int, dtype) -> DeviceBuffer:
+    """A zero-filled workspace for iimg() (zeroed once; calls keep it ready)."""
+    return DeviceBuffer.zeros(max(iimg_workspace_size(H, W, dtype), 16))
+
+
 def iimg_workspace_size(H: int, W: int, dtype) -> int:
     import ctypes
 
@@ -199,7 +210,7 @@ def iimg_np(img: np.ndarray, use_workspace: bool = True) -> np.ndarray:
     H, W = img.shape
     _, _, odt = _IIMG[img.dtype]
     d, o = DeviceBuffer.from_array(img), DeviceBuffer(max(H * W * 4, 4))
-    ws = DeviceBuffer(max(iimg_workspace_size(H, W, img.dtype), 4)) if use_workspace else None
+    ws = iimg_workspace(H, W, img.dtype) if use_workspace else None
     iimg(d, H, W, W, o, W, img.dtype, ws)
     return o.download((H, W), odt)
 
