@@ -241,6 +241,17 @@ umi_status umi_iimg_i32(const int32_t *in, int H, int W, int in_pitch, int32_t *
                         void *workspace, size_t workspace_bytes, umi_stream s);
 umi_status umi_iimg_f32(const float *in, int H, int W, int in_pitch, float *out, int out_pitch, void *workspace,
                         size_t workspace_bytes, umi_stream s);
+/* umi_iimg_u8_u32 without a workspace argument, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its workspace is a library array of 16 MiB per GPU
+ * (held until the process ends), enough for images up to about 30 Mpx (a
+ * 16 x 256 tile takes 2176 B); larger images are refused with
+ * UMI_ERR_INVALID.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU (other streams, other threads) race and must
+ * not be issued; consecutive calls on one stream are fine. */
+umi_status umi_iimg_u8_u32_noalloc(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
+                                   umi_stream s);
 
 /* ---------------------------------------------------------------------------
  * Box sums from an inclusive uint32 integral image (as umi_iimg_u8_u32

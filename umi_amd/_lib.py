@@ -64,6 +64,7 @@ SIGNATURES = {
     "umi_iimg_set_grid": [c_int],
     "umi_iimg_workspace_status": [P, C.POINTER(c_int)],
     "umi_iimg_u8_u32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
+    "umi_iimg_u8_u32_noalloc": [P, c_int, c_int, c_int, P, c_int, P],
     "umi_iimg_i32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_iimg_f32": [P, c_int, c_int, c_int, P, c_int, P, c_size_t, P],
     "umi_gemm_elem_bytes": [c_int, C.POINTER(c_int)],

@@ -192,7 +192,9 @@ template <class T, int C>
 umi_status launch_iimg_c(umi_batch b, const typename T::In *in, typename T::Out *out, uint8_t *ws, hipStream_t st) {
     constexpr int RPL = IRPL[C];
     const size_t lds = sizeof(IimgSmem<T, RPL>);
-    const int grid = iimg_grid((const void *)batch_iimg_kernel<T, RPL>, lds, b->nitiles[C], g_iimg_per_cu);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void *)batch_iimg_kernel<T, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int grid = iimg_grid(attr, b->nitiles[C], g_iimg_per_cu);
     if (grid < 1) return set_error(UMI_ERR_HIP, "batch integral image: launch setup failed");
     hipLaunchKernelGGL((batch_iimg_kernel<T, RPL>), dim3(grid), dim3(IM_THREADS), lds, st,
                        b->idesc[C], b->nitiles[C], in, out, ws);

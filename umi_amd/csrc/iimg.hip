@@ -17,6 +17,7 @@
 #include <atomic>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include "common.h"
 #include "iimg_dev.h"
 
@@ -79,8 +80,10 @@ template <class T, int RPL>
 umi_status iimg_launch_rpl(const typename T::In *in, int H, int W, int ip, typename T::Out *out, int op, uint8_t *ws,
                            hipStream_t st, const char *name) {
     const size_t lds = sizeof(IimgSmem<T, RPL>);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void *)iimg_kernel<T, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const IimgGeom g = iimg_geom(H, W, 16 * RPL);
-    const int grid = iimg_grid((const void *)iimg_kernel<T, RPL>, lds, g.ntx * g.nty, g_iimg_per_cu);
+    const int grid = iimg_grid(attr, g.ntx * g.nty, g_iimg_per_cu);
     if (grid < 1) return set_error(UMI_ERR_HIP, "%s: launch setup failed", name);
     hipLaunchKernelGGL((iimg_kernel<T, RPL>), dim3(grid), dim3(IM_THREADS), lds, st, in, H, W, ip, out, op, ws);
     return launch_check(name);
@@ -204,6 +207,37 @@ __global__ __launch_bounds__(256) void haar_kernel(const uint32_t *__restrict__ 
 
 }  // namespace
 
+// Allocation-free integral image: the workspace of umi_iimg_u8_u32_noalloc is
+// this array, one copy per GPU, zero at load (as a workspace must be before
+// its first use); every call leaves it ready for the next, so nothing clears
+// it.  Host code reaches it only through the address that
+// hipGetSymbolAddress gives for the current device.
+constexpr size_t IIMG_SCRATCH_BYTES = 16u << 20;
+static __device__ __align__(256) uint8_t scratch[IIMG_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up on its first use there.
+umi_status scratch_ptr(void **out) {
+    int dev = 0;
+    UMI_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_iimg_u8_u32_noalloc: device %d beyond %d", dev, SCRATCH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        UMI_TRY(hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (integral image scratch)"));
+        if (!p) return set_error(UMI_ERR_HIP, "hipGetSymbolAddress (integral image scratch): null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return UMI_OK;
+}
+}  // namespace
+
 extern "C" {
 
 umi_status umi_iimg_workspace_size(int H, int W, int kind, size_t *bytes) {
@@ -247,6 +281,22 @@ umi_status umi_iimg_set_config(int rows_per_lane) {
 umi_status umi_iimg_u8_u32(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
                            void *workspace, size_t workspace_bytes, umi_stream s) {
     return iimg_launch<IU8>(in, H, W, in_pitch, out, out_pitch, workspace, workspace_bytes, s, "umi_iimg_u8_u32");
+}
+
+umi_status umi_iimg_u8_u32_noalloc(const uint8_t *in, int H, int W, int in_pitch, uint32_t *out, int out_pitch,
+                                   umi_stream s) {
+    const char *name = "umi_iimg_u8_u32_noalloc";
+    if (!in || !out) return set_error(UMI_ERR_INVALID, "%s: null pointer", name);
+    if (H < 0 || W < 0 || in_pitch < W || out_pitch < W)
+        return set_error(UMI_ERR_INVALID, "%s: H=%d W=%d in_pitch=%d out_pitch=%d", name, H, W, in_pitch, out_pitch);
+    if (H == 0 || W == 0) return UMI_OK;
+    const size_t need = iimg_ws_bytes<IU8>(H, W);
+    if (need > IIMG_SCRATCH_BYTES)
+        return set_error(UMI_ERR_INVALID, "%s: %d x %d needs %zu B of scratch, more than its %zu B", name, H, W, need,
+                         IIMG_SCRATCH_BYTES);
+    void *ws = nullptr;
+    UMI_TRY(scratch_ptr(&ws));
+    return iimg_launch<IU8>(in, H, W, in_pitch, out, out_pitch, ws, IIMG_SCRATCH_BYTES, s, name);
 }
 
 umi_status umi_iimg_i32(const int32_t *in, int H, int W, int in_pitch, int32_t *out, int out_pitch,

@@ -161,9 +161,10 @@ static_assert(IM_CTL_BYTES % 256 == 0, "control block keeps the words 256-byte a
 
 // Grid of a single-pass launch over `tiles` tiles: per_cu > 0 gives a
 // persistent grid of per_cu workgroups per CU; 0 a workgroup per tile.
-// Sets the kernel's LDS limit; < 1 on failure.
-inline int iimg_grid(const void *kernel, size_t lds, int tiles, int per_cu) {
-    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return 0;
+// `attr` is the status of setting the kernel's LDS limit (done once per
+// kernel by the caller); < 1 on failure.
+inline int iimg_grid(hipError_t attr, int tiles, int per_cu) {
+    if (attr != hipSuccess) return 0;
     if (per_cu <= 0) return tiles;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
