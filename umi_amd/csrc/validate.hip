@@ -6,7 +6,9 @@
 // is read as 1 (main/iimg.cu:60 passes 3-element ParallelDims, which the
 // reference rejects), and the result is a status code, not abort().
 // Added checks: block sizes >= 1, non-negative totals, view dims in 0..3,
-// input sizes >= 1, plane counts <= 4, Expand nibbles >= 1.
+// input sizes >= 1, plane counts <= 4, Expand nibbles >= 1, at most 1024
+// threads per block, ZeroStrideOpt bits only on inputs whose view does not
+// move along the expanded dim.
 #include "common.h"
 
 extern "C" umi_status umi_validate_execute(const int32_t *p_dims, const int32_t *a_dims, int n_in, int n_out,
@@ -33,7 +35,20 @@ extern "C" umi_status umi_validate_execute(const int32_t *p_dims, const int32_t 
         if (e == 0) return set_error(UMI_ERR_INVALID, "Execute: Expand nibble of dim %d is 0", d);
     }
     if (warp_prod != 32) return set_error(UMI_ERR_INVALID, "Execute: product of warp sizes is %lld, not 32", (long long)warp_prod);
-    (void)zero_stride_opt;  // a load-sharing hint in the reference; results do not depend on it here
+    int64_t threads = 1;
+    for (int d = 0; d < 4; ++d) threads *= p_dims[4 * d + 1];
+    if (threads > 1024) return set_error(UMI_ERR_INVALID, "Execute: %lld threads per block (at most 1024)", (long long)threads);
+    // ZeroStrideOpt bit 4 i + d shares input i's operands across the Expand
+    // points along dim d: only sound when the input's parallel view does not
+    // move along d (the reference would silently read the wrong operands)
+    for (int i = 0; i < n_in && i < 16; ++i)
+        for (int d = 0; d < 4; ++d) {
+            const uint64_t e = (expand >> (4 * (3 - d))) & 0xf;
+            if (((zero_stride_opt >> (4 * i + d)) & 1) && e > 1 && views[24 * i + 3 * d + 2] != 0)
+                return set_error(UMI_ERR_INVALID,
+                                 "Execute: ZeroStrideOpt bit %d set, but input %d's view moves along dim %d (stride %d)",
+                                 4 * i + d, i, d, views[24 * i + 3 * d + 2]);
+        }
     for (int i = 0; i < n_in + n_out; ++i) {
         if (planes[i] < 0 || planes[i] > 4) return set_error(UMI_ERR_INVALID, "Execute: array %d has %d planes", i, planes[i]);
         for (int v = 0; v < 8; ++v) {

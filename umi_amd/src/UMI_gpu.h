@@ -15,23 +15,30 @@
 // read from the row-major array with PAD (-> padding[j]) or clamp (WRAP)
 // boundaries; outputs are written at the same affine address, unclamped,
 // for every valid p (Loop outputs at every accumulation step).
-// PtrNum nibble i = planes of array i (inputs, then outputs).  Expand and
-// ZeroStrideOpt are accepted and validated; they are mapping hints in the
-// reference and do not change results here.
+// PtrNum nibble i = planes of array i (inputs, then outputs).
 //
-// Differences by design: the call validates descriptors and aborts with a
-// message (the reference aborts silently on bad ParallelDims and does not
-// check runtime errors); warp_shuf = 0 is read as 1 (the reference aborts,
-// which makes main/iimg.cu's second pass fail); any element size is accepted.
-//
-// Host side: descriptors are validated by libumi_hip (umi_validate_execute)
-// and the kernel, instantiated here from the user's strategy struct, is
-// launched through umi_launch_kernel.  Execute returns after the kernel
-// completes (the reference is synchronous through its cudaFree);
-// ExecuteAsync takes a stream and does not wait.
+// The kernel (the role of Execute_gpukernel, src/UMI_gpu.h:189-358):
+//  * a workgroup of prod(block_size) threads owns an "eblock" of
+//    Expand_d x block_size_d points per dim; a thread owns the Expand_d
+//    points t_d + e * block_size_d (interleaved, so neighbouring threads take
+//    neighbouring points: conflict-free LDS reads, coalesced writes) and
+//    keeps one strategy object per point in registers (CoreExecution's
+//    umi_ctx, src/UMI_gpu.h:101-187);
+//  * per accumulation block, every Loop input whose footprint pays for it
+//    is staged in LDS (FillShm, src/UMI_gpu.h:33-83): the footprint of an
+//    affine view over the eblock x accumulation block is a box of the same
+//    size for every workgroup; the staging applies PAD / clamp once per
+//    element, so the strategy's operands become plain LDS reads (other inputs
+//    and the Pre / Post inputs are read from device memory with the boundary
+//    rule);
+//  * ZeroStrideOpt bit 4 i + d shares input i's operands across the Expand
+//    points along dim d (one load for all of them); the library checks that
+//    the input's view does not move along dim d, so a hint cannot change
+//    results (the reference would read wrong data).
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cassert>  // the examples use assert() through this header, as with the reference's
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +48,9 @@
 
 namespace UMI {
 namespace engine {
+
+constexpr int MAX_THREADS = 1024;              // threads per workgroup (MAX_BLOCK_SIZE)
+constexpr size_t LDS_BUDGET = 64 * 1024;       // bytes of staged operands per workgroup
 
 // Affine map (p, a) -> physical coordinates plus the array's geometry.
 struct ArrayMap {
@@ -52,9 +62,22 @@ struct ArrayMap {
     int32_t pad_mask;       // bit k: dim k uses PAD
 };
 
+// LDS staging of a Loop input: the footprint box of extent ext over the
+// eblock x accumulation block, planes one after another from lds_ofs.
+struct Staging {
+    int32_t on;             // 1: staged
+    int32_t lds_ofs;        // element offset of plane 0
+    int32_t box;            // elements per plane
+    int32_t ext[DIM];       // box extent per physical dim
+    int32_t lst[DIM];       // LDS stride per physical dim
+    int32_t pcl[DIM];       // LDS-linear coefficient of p_d
+    int32_t acl[DIM];       // LDS-linear coefficient of a_d
+};
+
 template <class E>
 struct InArray {
     ArrayMap m;
+    Staging st;
     const E *ptr[MAX_PTR];
     E pad[MAX_PTR];
 };
@@ -67,10 +90,11 @@ struct OutArray {
 template <class E, int NI, int NO>
 struct Args {
     int32_t ptot[DIM];   // parallel totals
+    int32_t pblk[DIM];   // threads of a workgroup per dim (block_size)
+    int32_t pgrid[DIM];  // workgroups per dim
     int32_t atot[DIM];   // accumulation totals
     int32_t ablk[DIM];   // accumulation block sizes
     int32_t agrid[DIM];  // accumulation blocks per dim
-    int64_t np;          // number of parallel points
     InArray<E> in[SafeArrayLength(NI)];
     OutArray<E> out[SafeArrayLength(NO)];
 };
@@ -91,6 +115,38 @@ inline ArrayMap make_map(const NDTuple<ViewDim> &pv, const NDTuple<ViewDim> &av,
         if (phy[k].mode == PhysicalDim::PAD) m.pad_mask |= 1 << k;
     }
     return m;
+}
+
+// Footprint box of map m over eblock extents eb and accumulation block ab.
+inline Staging make_staging(const ArrayMap &m, const int32_t (&eb)[DIM], const int32_t (&ab)[DIM]) {
+    Staging st{};
+    int64_t box = 1;
+    for (int k = 0; k < DIM; ++k) {
+        int64_t e = 1;
+        for (int d = 0; d < DIM; ++d) {
+            e += (int64_t)(m.pc[k][d] < 0 ? -m.pc[k][d] : m.pc[k][d]) * (eb[d] - 1);
+            e += (int64_t)(m.ac[k][d] < 0 ? -m.ac[k][d] : m.ac[k][d]) * (ab[d] - 1);
+        }
+        st.ext[k] = (int32_t)(e < (1 << 30) ? e : (1 << 30));
+        box *= st.ext[k];
+        if (box > (1 << 30)) box = 1 << 30;
+    }
+    st.box = (int32_t)box;
+    int32_t s = 1;
+    for (int k = DIM - 1; k >= 0; --k) {
+        st.lst[k] = s;
+        s = (int32_t)std::min<int64_t>((int64_t)s * st.ext[k], 1 << 30);
+    }
+    for (int d = 0; d < DIM; ++d) {
+        int64_t pl = 0, al = 0;
+        for (int k = 0; k < DIM; ++k) {
+            pl += (int64_t)m.pc[k][d] * st.lst[k];
+            al += (int64_t)m.ac[k][d] * st.lst[k];
+        }
+        st.pcl[d] = (int32_t)pl;
+        st.acl[d] = (int32_t)al;
+    }
+    return st;
 }
 
 // Physical coordinates of (p, a) for map m; a contributes only in the Loop stage.
@@ -135,88 +191,264 @@ __host__ __device__ constexpr int planes(int array_index) {
     return (int)Slice(PtrNum, 4, array_index);
 }
 
-// Gathers the inputs of one stage: arrays [I0, I0+N) into in[].
-template <bool LOOP, uint64_t PtrNum, int I0, int N, class E, int NI, int NO, size_t S>
-__device__ inline void gather(const Args<E, NI, NO> &g, const int32_t (&p)[DIM], const int32_t (&a)[DIM],
-                              simple_array<E, S> &in) {
+// Expand nibbles (dim d: bits 12 - 4 d .. 15 - 4 d, src/UMI_gpu.h:128-131).
+template <uint64_t Expand>
+struct Ex {
+    static constexpr int e[DIM] = {(int)Slice(Expand, 4, 3), (int)Slice(Expand, 4, 2), (int)Slice(Expand, 4, 1),
+                                   (int)Slice(Expand, 4, 0)};
+    static constexpr int size = e[0] * e[1] * e[2] * e[3];
+    // expansion index -> its coordinate along dim d (dim 3 fastest)
+    __host__ __device__ static constexpr int coord(int id, int d) {
+        int r = id;
+        for (int k = DIM - 1; k > d; --k) r /= e[k];
+        return r % e[d];
+    }
+    // the expansion whose operands input i shares (ZeroStrideOpt bit 4 i + d zeroes dim d)
+    template <uint64_t Opt>
+    __host__ __device__ static constexpr int canon(int i, int id) {
+        int c = 0;
+        for (int d = 0; d < DIM; ++d) c = c * e[d] + (Slice(Opt, 1, DIM * i + d) ? 0 : coord(id, d));
+        return c;
+    }
+};
+
+// Lowest coordinate of the footprint box of map m for eblock origin P0 and
+// accumulation block origin A0 (Loop stage).
+__device__ inline void box_min(const ArrayMap &m, const int32_t (&P0)[DIM], const int32_t (&eb)[DIM],
+                               const int32_t (&A0)[DIM], const int32_t (&ab)[DIM], int32_t (&mn)[DIM]) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        int32_t v = m.base[k];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int32_t pc = m.pc[k][d], ac = m.ac[k][d];
+            v += pc >= 0 ? pc * P0[d] : pc * (P0[d] + eb[d] - 1);
+            v += ac >= 0 ? ac * A0[d] : ac * (A0[d] + ab[d] - 1);
+        }
+        mn[k] = v;
+    }
+}
+
+// Stages input A's footprint box (all its planes) into lds; the whole
+// workgroup, coalesced along the innermost physical dim.
+template <class E>
+__device__ inline void stage(const InArray<E> &A, int npl, E *lds, const int32_t (&mn)[DIM], int nthreads) {
+    const Staging &st = A.st;
+    const int32_t e1 = st.ext[1], e2 = st.ext[2], e3 = st.ext[3];
+    // (row, column) of element idx = row * e3 + column, advanced by nthreads
+    // without dividing: q whole rows and r columns per step
+    const int32_t q = nthreads / e3, r = nthreads - q * e3;
+    int32_t row = (int32_t)threadIdx.x / e3, col = (int32_t)threadIdx.x - row * e3;
+    const bool flat = e1 * st.ext[0] == 1;  // rows are dim 2 alone (2-D boxes)
+    for (int32_t idx = (int32_t)threadIdx.x; idx < st.box; idx += nthreads) {
+        int32_t c[DIM];
+        c[3] = mn[3] + col;
+        if (flat) {
+            c[2] = mn[2] + row;
+            c[1] = mn[1];
+            c[0] = mn[0];
+        } else {
+            const int32_t r12 = row / e2;
+            c[2] = mn[2] + (row - r12 * e2);
+            c[1] = mn[1] + r12 % e1;
+            c[0] = mn[0] + r12 / e1;
+        }
+        for (int j = 0; j < npl; ++j) lds[st.lds_ofs + j * st.box + idx] = load(A, j, c);
+        row += q;
+        col += r;
+        if (col >= e3) {
+            col -= e3;
+            ++row;
+        }
+    }
+}
+
+// Operands of one stage for all Expand points: buf[i][plane][expansion],
+// loaded once per shared (ZeroStrideOpt) group.
+// LDS offsets of the staged Loop inputs: per Expand point (once per
+// accumulation block) and per step (the same for the whole workgroup).
+template <int N, int ES>
+struct LdsOffsets {
+    int32_t pt[SafeArrayLength(N)][ES];
+    int32_t step[SafeArrayLength(N)];
+};
+
+template <bool LOOP, uint64_t PtrNum, uint64_t Expand, uint64_t Opt, int I0, int N, class E, int NI, int NO, class LO>
+__device__ inline void gather_all(const Args<E, NI, NO> &g, const E *lds, const LO &lo, const int32_t (&pt)[DIM],
+                                  const int32_t (&a)[DIM], E (&buf)[SafeArrayLength(N)][MAX_PTR][Ex<Expand>::size]) {
+    using X = Ex<Expand>;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        int32_t c[DIM];
-        coords<LOOP>(g.in[I0 + i].m, p, a, c);
+        const InArray<E> &A = g.in[I0 + i];
 #pragma unroll
-        for (int j = 0; j < planes<PtrNum>(I0 + i); ++j) in[i * MAX_PTR + j] = load(g.in[I0 + i], j, c);
+        for (int id = 0; id < X::size; ++id) {
+            if (X::template canon<Opt>(I0 + i, id) != id) continue;  // shared with an earlier point
+            if (LOOP && A.st.on) {
+                const int32_t off = lo.pt[i][id] + lo.step[i];
+#pragma unroll
+                for (int j = 0; j < planes<PtrNum>(I0 + i); ++j) buf[i][j][id] = lds[off + j * A.st.box];
+            } else {
+                int32_t p[DIM], c[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) p[d] = pt[d] + X::coord(id, d) * g.pblk[d];
+                coords<LOOP>(A.m, p, a, c);
+#pragma unroll
+                for (int j = 0; j < planes<PtrNum>(I0 + i); ++j) buf[i][j][id] = load(A, j, c);
+            }
+        }
     }
 }
 
-template <bool LOOP, uint64_t PtrNum, int TOTAL_IN, int O0, int N, class E, int NI, int NO, size_t S>
-__device__ inline void scatter(const Args<E, NI, NO> &g, const int32_t (&p)[DIM], const int32_t (&a)[DIM],
-                               const simple_array<E, S> &out) {
+// Runs one stage (0 Pre, 1 Loop, 2 Post) for every Expand point of the thread.
+template <int STAGE, uint64_t PtrNum, uint64_t Expand, uint64_t Opt, int I0, int NIN, int O0, int NOUT, int TOTAL_IN,
+          class Umi, class E, class C, int NI, int NO, class LO>
+__device__ inline void run_stage(const Args<E, NI, NO> &g, const E *lds, const LO &lo, const int32_t (&pt)[DIM],
+                                 const int32_t (&a)[DIM], const bool (&valid)[Ex<Expand>::size],
+                                 Umi (&ctx)[Ex<Expand>::size], const C &uc) {
+    using X = Ex<Expand>;
+    E buf[SafeArrayLength(NIN)][MAX_PTR][X::size];
+    gather_all<STAGE == 1, PtrNum, Expand, Opt, I0, NIN>(g, lds, lo, pt, a, buf);
 #pragma unroll
-    for (int o = 0; o < N; ++o) {
-        int32_t c[DIM];
-        coords<LOOP>(g.out[O0 + o].m, p, a, c);
+    for (int id = 0; id < X::size; ++id) {
+        simple_array<E, NIN * MAX_PTR> in;
+        simple_array<E, NOUT * MAX_PTR> out;
 #pragma unroll
-        for (int j = 0; j < planes<PtrNum>(TOTAL_IN + O0 + o); ++j) store(g.out[O0 + o], j, c, out[o * MAX_PTR + j]);
+        for (int i = 0; i < NIN; ++i) {
+            const int src = X::template canon<Opt>(I0 + i, id);
+#pragma unroll
+            for (int j = 0; j < planes<PtrNum>(I0 + i); ++j) in[i * MAX_PTR + j] = buf[i][j][src];
+        }
+        if constexpr (STAGE == 0)
+            ctx[id].Pre(uc, in, out);
+        else if constexpr (STAGE == 1)
+            ctx[id].Loop(uc, in, out);
+        else
+            ctx[id].Post(uc, in, out);
+        if (NOUT > 0 && valid[id]) {
+            int32_t p[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) p[d] = pt[d] + X::coord(id, d) * g.pblk[d];
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) {
+                int32_t c[DIM];
+                coords<STAGE == 1>(g.out[O0 + o].m, p, a, c);
+#pragma unroll
+                for (int j = 0; j < planes<PtrNum>(TOTAL_IN + O0 + o); ++j)
+                    store(g.out[O0 + o], j, c, out[o * MAX_PTR + j]);
+            }
+        }
     }
 }
 
-template <class Umi, uint64_t PtrNum, class E, class C>
-__global__ __launch_bounds__(256) void execute_kernel(const Args<E, Umi::TOTAL_IN, Umi::TOTAL_OUT> g, const C uc) {
+template <class Umi, uint64_t PtrNum, uint64_t Expand, uint64_t Opt, class E, class C, int MAXT>
+__global__ __launch_bounds__(MAXT) void execute_kernel(const Args<E, Umi::TOTAL_IN, Umi::TOTAL_OUT> g, const C uc) {
+    constexpr int NI = Umi::TOTAL_IN;
     constexpr int PRE_I = 0, LOOP_I = Umi::N_PRE_IN, POST_I = LOOP_I + Umi::N_LOOP_IN;
     constexpr int PRE_O = 0, LOOP_O = Umi::N_PRE_OUT, POST_O = LOOP_O + Umi::N_LOOP_OUT;
-    const int64_t flat = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (flat >= g.np) return;
-    int32_t p[DIM];
-    int64_t f = flat;
+    using X = Ex<Expand>;
+    extern __shared__ __attribute__((aligned(16))) char engine_lds[];
+    E *lds = reinterpret_cast<E *>(engine_lds);
+    const int nthreads = (int)blockDim.x;
+    // workgroup origin, thread offset, eblock extents
+    int32_t P0[DIM], pt[DIM], eb[DIM];
+    {
+        int32_t b = (int32_t)blockIdx.x, t = (int32_t)threadIdx.x;
 #pragma unroll
-    for (int d = DIM - 1; d >= 0; --d) {
-        p[d] = (int32_t)(f % g.ptot[d]);
-        f /= g.ptot[d];
+        for (int d = DIM - 1; d >= 0; --d) {
+            eb[d] = X::e[d] * g.pblk[d];
+            P0[d] = (b % g.pgrid[d]) * eb[d];
+            b /= g.pgrid[d];
+            pt[d] = P0[d] + t % g.pblk[d];
+            t /= g.pblk[d];
+        }
+    }
+    bool valid[X::size];
+#pragma unroll
+    for (int id = 0; id < X::size; ++id) {
+        bool v = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v &= pt[d] + X::coord(id, d) * g.pblk[d] < g.ptot[d];
+        valid[id] = v;
     }
     int32_t a[DIM] = {0, 0, 0, 0};
-    Umi ctx;
-    {
-        simple_array<E, Umi::N_PRE_IN * MAX_PTR> in;
-        simple_array<E, Umi::N_PRE_OUT * MAX_PTR> out;
-        gather<false, PtrNum, PRE_I, Umi::N_PRE_IN>(g, p, a, in);
-        ctx.Pre(uc, in, out);
-        scatter<false, PtrNum, Umi::TOTAL_IN, PRE_O, Umi::N_PRE_OUT>(g, p, a, out);
-    }
-    // block-major raster walk of the accumulation space
+    LdsOffsets<Umi::N_LOOP_IN, X::size> lo{};
+    Umi ctx[X::size];
+    run_stage<0, PtrNum, Expand, Opt, PRE_I, Umi::N_PRE_IN, PRE_O, Umi::N_PRE_OUT, NI>(g, lds, lo, pt, a, valid, ctx,
+                                                                                      uc);
+    // Loop: block-major raster walk of the accumulation space
+    bool any_staged = false;
+#pragma unroll
+    for (int i = 0; i < Umi::N_LOOP_IN; ++i) any_staged |= g.in[LOOP_I + i].st.on != 0;
     const int32_t nblk = g.agrid[0] * g.agrid[1] * g.agrid[2] * g.agrid[3];
     const int32_t bsz = g.ablk[0] * g.ablk[1] * g.ablk[2] * g.ablk[3];
     for (int32_t ab = 0; ab < nblk; ++ab) {
-        int32_t abi[DIM];
-        int32_t t = ab;
-#pragma unroll
-        for (int d = DIM - 1; d >= 0; --d) {
-            abi[d] = t % g.agrid[d];
-            t /= g.agrid[d];
-        }
-        for (int32_t as = 0; as < bsz; ++as) {
-            int32_t s = as;
-            bool ok = true;
+        int32_t A0[DIM];
+        {
+            int32_t r = ab;
 #pragma unroll
             for (int d = DIM - 1; d >= 0; --d) {
-                a[d] = abi[d] * g.ablk[d] + s % g.ablk[d];
-                s /= g.ablk[d];
+                A0[d] = (r % g.agrid[d]) * g.ablk[d];
+                r /= g.agrid[d];
+            }
+        }
+        if (any_staged) {
+            __syncthreads();  // the previous block's operands are no longer read
+#pragma unroll
+            for (int i = 0; i < Umi::N_LOOP_IN; ++i) {
+                const InArray<E> &A = g.in[LOOP_I + i];
+                if (!A.st.on) continue;
+                int32_t mn[DIM];
+                box_min(A.m, P0, eb, A0, g.ablk, mn);
+                stage<E>(A, planes<PtrNum>(LOOP_I + i), lds, mn, nthreads);
+                // LDS index of (thread's first point, a = 0) = sum_k (c_k - mn_k) lst_k
+                int32_t c[DIM];
+                coords<false>(A.m, pt, a, c);  // a = 0 here: the Loop term is added per step
+                int32_t off = A.st.lds_ofs;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) off += (c[k] - mn[k]) * A.st.lst[k];
+#pragma unroll
+                for (int id = 0; id < X::size; ++id) {
+                    int32_t o = off;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) o += A.st.pcl[d] * X::coord(id, d) * g.pblk[d];
+                    lo.pt[i][id] = o;
+                }
+            }
+            __syncthreads();
+        }
+        // steps of the block in raster order (dim 3 fastest), an odometer
+        // instead of dividing the step index
+        int32_t l[DIM] = {0, 0, 0, 0};
+        for (int32_t as = 0; as < bsz; ++as) {
+            bool ok = true;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                a[d] = A0[d] + l[d];
                 ok &= a[d] < g.atot[d];
             }
-            if (!ok) continue;
-            simple_array<E, Umi::N_LOOP_IN * MAX_PTR> in;
-            simple_array<E, Umi::N_LOOP_OUT * MAX_PTR> out;
-            gather<true, PtrNum, LOOP_I, Umi::N_LOOP_IN>(g, p, a, in);
-            ctx.Loop(uc, in, out);
-            scatter<true, PtrNum, Umi::TOTAL_IN, LOOP_O, Umi::N_LOOP_OUT>(g, p, a, out);
+#pragma unroll
+            for (int i = 0; i < Umi::N_LOOP_IN; ++i) {
+                const Staging &st = g.in[LOOP_I + i].st;
+                lo.step[i] = st.acl[0] * a[0] + st.acl[1] * a[1] + st.acl[2] * a[2] + st.acl[3] * a[3];
+            }
+            if (ok)
+                run_stage<1, PtrNum, Expand, Opt, LOOP_I, Umi::N_LOOP_IN, LOOP_O, Umi::N_LOOP_OUT, NI>(g, lds, lo, pt, a,
+                                                                                                      valid, ctx, uc);
+            if (++l[3] == g.ablk[3]) {
+                l[3] = 0;
+                if (++l[2] == g.ablk[2]) {
+                    l[2] = 0;
+                    if (++l[1] == g.ablk[1]) {
+                        l[1] = 0;
+                        ++l[0];
+                    }
+                }
+            }
         }
     }
-    {
-        simple_array<E, Umi::N_POST_IN * MAX_PTR> in;
-        simple_array<E, Umi::N_POST_OUT * MAX_PTR> out;
-        gather<false, PtrNum, POST_I, Umi::N_POST_IN>(g, p, a, in);
-        ctx.Post(uc, in, out);
-        scatter<false, PtrNum, Umi::TOTAL_IN, POST_O, Umi::N_POST_OUT>(g, p, a, out);
-    }
+    for (int d = 0; d < DIM; ++d) a[d] = 0;
+    run_stage<2, PtrNum, Expand, Opt, POST_I, Umi::N_POST_IN, POST_O, Umi::N_POST_OUT, NI>(g, lds, lo, pt, a, valid,
+                                                                                          ctx, uc);
 }
 
 inline void pack_dims(const NDTuple<ParallelDim> &p_dims, const NDTuple<AccumulationDim> &a_dims, int32_t (&pd)[16],
@@ -276,16 +508,24 @@ umi_status ExecuteAsync(umi_stream stream, const NDTuple<ParallelDim> &p_dims, c
     umi_status st = umi_validate_execute(pd, ad, NI, NO, views, phys, planes, Expand, ZeroStrideOpt);
     if (st != UMI_OK) return st;
 
+    using X = engine::Ex<Expand>;
     engine::Args<EType, NI, NO> g{};
-    int64_t np = 1;
+    int64_t nwg = 1, nthreads = 1;
+    int32_t eb[DIM];
+    bool empty = false;
     for (int d = 0; d < DIM; ++d) {
         g.ptot[d] = p_dims[d].total_size;
+        g.pblk[d] = p_dims[d].block_size;
+        eb[d] = X::e[d] * g.pblk[d];
+        g.pgrid[d] = CeilDiv(g.ptot[d], eb[d]);
         g.atot[d] = a_dims[d].total_size;
         g.ablk[d] = a_dims[d].block_size;
         g.agrid[d] = g.atot[d] > 0 ? CeilDiv(g.atot[d], g.ablk[d]) : 0;
-        np *= p_dims[d].total_size;
+        nwg *= g.pgrid[d];
+        nthreads *= g.pblk[d];
+        empty |= g.ptot[d] <= 0;
     }
-    g.np = np;
+    if (nthreads > engine::MAX_THREADS || nwg > 0x7fffffff) return UMI_ERR_UNSUPPORTED;
     for (int i = 0; i < NI; ++i) {
         g.in[i].m = engine::make_map(imems[i].p_view_dims, imems[i].a_view_dims, imems[i].physical_dims);
         for (int j = 0; j < MAX_PTR; ++j) {
@@ -293,16 +533,34 @@ umi_status ExecuteAsync(umi_stream stream, const NDTuple<ParallelDim> &p_dims, c
             g.in[i].pad[j] = imems[i].padding[j];
         }
     }
-    for (int o = 0; o < NO; ++o) {
-        g.out[o].m = engine::make_map(omems[o].p_view_dims, omems[o].a_view_dims, omems[o].physical_dims);
-        for (int j = 0; j < MAX_PTR; ++j) g.out[o].ptr[j] = omems[o].ptr[j];
+    // LDS staging of Loop inputs: in order, while the footprint is smaller
+    // than the reads it replaces (operands are reused) and the budget holds
+    {
+        int64_t reads = 1, used = 0;
+        for (int d = 0; d < DIM; ++d) reads *= (int64_t)eb[d] * g.ablk[d];
+        for (int i = Umi::N_PRE_IN; i < Umi::N_PRE_IN + Umi::N_LOOP_IN; ++i) {
+            const int npl = engine::planes<PtrNum>(i);
+            if (npl == 0) continue;
+            engine::Staging st = engine::make_staging(g.in[i].m, eb, g.ablk);
+            const int64_t bytes = (int64_t)st.box * npl * (int64_t)sizeof(EType);
+            if (st.box < reads && used + bytes <= (int64_t)engine::LDS_BUDGET) {
+                st.on = 1;
+                st.lds_ofs = (int32_t)(used / (int64_t)sizeof(EType));
+                used += (bytes + 15) / 16 * 16;
+                g.in[i].st = st;
+            }
+        }
+        for (int o = 0; o < NO; ++o) {
+            g.out[o].m = engine::make_map(omems[o].p_view_dims, omems[o].a_view_dims, omems[o].physical_dims);
+            for (int j = 0; j < MAX_PTR; ++j) g.out[o].ptr[j] = omems[o].ptr[j];
+        }
+        if (empty || nwg == 0) return UMI_OK;
+        void *args[] = {(void *)&g, (void *)&c};
+        const void *fn = nthreads <= 256
+                             ? (const void *)&engine::execute_kernel<Umi, PtrNum, Expand, ZeroStrideOpt, EType, CType, 256>
+                             : (const void *)&engine::execute_kernel<Umi, PtrNum, Expand, ZeroStrideOpt, EType, CType, 1024>;
+        return umi_launch_kernel(fn, (unsigned)nwg, 1, 1, (unsigned)nthreads, 1, 1, (size_t)used, args, stream);
     }
-    if (np <= 0) return UMI_OK;
-    const int64_t blocks = (np + 255) / 256;
-    if (blocks > 0x7fffffff) return UMI_ERR_UNSUPPORTED;
-    void *args[] = {(void *)&g, (void *)&c};
-    return umi_launch_kernel((const void *)&engine::execute_kernel<Umi, PtrNum, EType, CType>, (unsigned)blocks, 1, 1,
-                             256, 1, 1, 0, args, stream);
 }
 
 // UMI::Execute — drop-in for src/UMI_gpu.h:409-623.  Runs on the default
