@@ -10,6 +10,7 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -81,7 +82,7 @@ def _pack(pd, ad):
     return np.array(pd, np.int32).reshape(16), np.array(ad, np.int32).reshape(8)
 
 
-def _validate(pd, ad, expand=0x1111, views=None, phys=None, planes=None, n_in=0, n_out=0):
+def _validate(pd, ad, expand=0x1111, views=None, phys=None, planes=None, n_in=0, n_out=0, zso=0):
     import numpy as np
 
     from umi_amd._lib import lib
@@ -93,7 +94,7 @@ def _validate(pd, ad, expand=0x1111, views=None, phys=None, planes=None, n_in=0,
         ph[1::2] = 0
     pl = np.ones(max(1, n_in + n_out), np.int32) if planes is None else np.asarray(planes, np.int32)
     return lib().umi_validate_execute(p.ctypes.data, a.ctypes.data, n_in, n_out, v.ctypes.data, ph.ctypes.data,
-                                      pl.ctypes.data, expand, 0)
+                                      pl.ctypes.data, expand, zso)
 
 
 GOOD_P = [[1, 1, 1, 1], [1, 1, 1, 1], [300, 16, 4, 1], [500, 16, 8, 1]]
@@ -133,6 +134,29 @@ def test_validate_execute_rejects_bad_views_and_sizes():
     assert _validate(GOOD_P, GOOD_A, planes=[5], n_in=1) == 1
     assert _validate(GOOD_P, GOOD_A, expand=0x1101) == 1
     assert _validate(GOOD_P, [[1, 1], [1, 0], [3, 3], [3, 3]]) == 1
+
+
+def test_validate_zero_stride_opt_against_views():
+    # main/wfilter3x3.cu:39: weights (input 1) do not move with the expanded dims 2, 3 -> 0xc0 is sound
+    from umi_amd._lib import lib
+
+    views = np.zeros(48, np.int32)
+    views[3 * 2:3 * 2 + 3] = (2, 0, 1)  # input 0, p view dim 2 -> physical 2, stride 1
+    views[3 * 3:3 * 3 + 3] = (3, 0, 1)  # input 0, p view dim 3 -> physical 3, stride 1
+    assert _validate(GOOD_P, GOOD_A, expand=0x1122, views=views, n_in=2, zso=0xc0) == 0
+    # the same bit pattern on input 0, whose view moves along dims 2 and 3: refused, not silently wrong
+    assert _validate(GOOD_P, GOOD_A, expand=0x1122, views=views, n_in=2, zso=0x0c) == 1
+    assert b"ZeroStrideOpt" in lib().umi_last_error()
+    # without expansion along the dim the bit shares nothing and is accepted
+    assert _validate(GOOD_P, GOOD_A, expand=0x1111, views=views, n_in=2, zso=0x0c) == 0
+
+
+def test_validate_block_size_limit():
+    from umi_amd._lib import lib
+
+    big = [[1, 1, 1, 1], [1, 1, 1, 1], [300, 64, 4, 1], [500, 32, 8, 1]]  # 64 x 32 = 2048 threads
+    assert _validate(big, GOOD_A) == 1
+    assert b"1024" in lib().umi_last_error()
 
 
 def test_batch_create_validates_on_host():
