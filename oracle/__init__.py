@@ -32,6 +32,9 @@ _SIGS = {
     "oracle_iimg_u8_u32": [_P, _I, _I, _I, _P],
     "oracle_iimg_i32": [_P, _I, _I, _I, _P],
     "oracle_iimg_f64": [_P, _I, _I, _I, _P],
+    "oracle_set_threads": [_I],
+    "oracle_gemm_f32_rows": [_P, _P, _P, _I, _I, _I, _I, _I],
+    "oracle_conv_f32": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I],
 }
 _lib = None
 
@@ -48,6 +51,34 @@ def lib() -> C.CDLL:
             f.restype = None
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the row-parallel loops (1 = the reference's single thread)."""
+    lib().oracle_set_threads(int(n))
+
+
+def gemm_f32_rows(A, B, r0, r1):
+    """Rows [r0, r1) of A @ B by the loop of main/gemm.cu:69-78 in float (fmaf)."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.zeros((M, N), np.float32)
+    lib().oracle_gemm_f32_rows(_ptr(A), _ptr(B), _ptr(C), M, N, K, r0, r1)
+    return C
+
+
+def conv_f32(x, w, stride, pad, n0=0, n1=None):
+    """Images [n0, n1) of the direct convolution of main/fprop.cu:84-108 in float (fmaf)."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    N, C, H, W = x.shape
+    K, _, R, S = w.shape
+    OH, OW = (H + 2 * pad - R) // stride + 1, (W + 2 * pad - S) // stride + 1
+    out = np.zeros((N, K, OH, OW), np.float32)
+    lib().oracle_conv_f32(_ptr(x), _ptr(w), _ptr(out), N, C, H, W, K, R, S, stride, pad, n0, N if n1 is None else n1)
+    return out
 
 
 def _ptr(a: np.ndarray | None):

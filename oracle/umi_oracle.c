@@ -12,7 +12,12 @@
  * (closed forms, brute force, invariants) is listed in the tests that load it.
  * Build: oracle/Makefile (gcc, -O2, -ffp-contract=off: float expressions are
  * evaluated exactly as written, fused multiply-adds only where fmaf() is
- * spelled out).
+ * spelled out; -fopenmp).
+ *
+ * Threads: oracle_set_threads(n) spreads the outer loop (block rows, image
+ * rows) of the ME, cost-volume, filter, bilateral, GEMM and convolution
+ * loops over n OpenMP threads (default 1, the reference's single thread);
+ * rows are independent, so the results do not depend on n.
  */
 #include <math.h>
 #include <stdint.h>
@@ -20,6 +25,9 @@
 #include <string.h>
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static int g_threads = 1;
+void oracle_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 /*
  * Full-search block motion estimation.
@@ -34,6 +42,7 @@ static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ?
 void oracle_me_full_search_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int B, int R,
                               int by_begin, int by_end, int32_t *mv_sad, int32_t *volume) {
     const int nbx = W / B, S = 2 * R + 1;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
     for (int by = by_begin; by < by_end; ++by) {
         for (int bx = 0; bx < nbx; ++bx) {
             const int y = by * B, x = bx * B;
@@ -98,6 +107,7 @@ void oracle_sad_volume_i32(const int32_t *cur, const int32_t *ref, int H, int W,
  */
 void oracle_cost_volume_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step,
                            int KY, int KX, int y_begin, int y_end, int32_t *out) {
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
     for (int y = y_begin; y < y_end; ++y)
         for (int x = 0; x < W; ++x)
             for (int i = 0; i < D; ++i) {
@@ -307,6 +317,7 @@ void oracle_wfilter3x3_i32(const int32_t *in, int H, int W, int ip, const int32_
 
 void oracle_wfilter3x3_f32(const float *in, int H, int W, int ip, const float *w, float init, float pad, int mode,
                            float *out, int op) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             float sum = init;
@@ -343,6 +354,7 @@ void oracle_wfilter3x3_f64(const float *in, int H, int W, int ip, const float *w
  */
 void oracle_bilateral_f32(const float *in, int H, int W, int ip, int r, const float *ws, float norm, float pad,
                           int mode, float *out, int op) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             const float c = in[(size_t)y * ip + x];
@@ -411,4 +423,47 @@ void oracle_iimg_f64(const float *in, int H, int W, int ip, double *out) {
             out[(size_t)y * W + x] = rowsum + (y ? out[(size_t)(y - 1) * W + x] : 0.0);
         }
     }
+}
+
+/*
+ * Dense GEMM, the loop of main/gemm.cu:69-78 (for y, for x: sum over k of
+ * A[y][k] * B[k][x]) in float, for rows [r0, r1) of C: the CPU baseline of
+ * the precision config (the FP64 oracle of the tests is numpy).
+ */
+void oracle_gemm_f32_rows(const float *A, const float *B, float *C, int M, int N, int K, int r0, int r1) {
+    (void)M;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int y = r0; y < r1; ++y)
+        for (int x = 0; x < N; ++x) {
+            float acc = 0.f;
+            for (int k = 0; k < K; ++k) acc = fmaf(A[(size_t)y * K + k], B[(size_t)k * N + x], acc);
+            C[(size_t)y * N + x] = acc;
+        }
+}
+
+/*
+ * Direct convolution, the loop of main/fprop.cu:84-108 (NCHW input, KCRS
+ * filter, zero padding) in float, for images [n0, n1): the CPU baseline of
+ * the precision config.
+ */
+void oracle_conv_f32(const float *x, const float *w, float *out, int N, int C, int H, int W, int K, int R, int S,
+                     int stride, int pad, int n0, int n1) {
+    (void)N;
+    const int OH = (H + 2 * pad - R) / stride + 1, OW = (W + 2 * pad - S) / stride + 1;
+#pragma omp parallel for num_threads(g_threads) schedule(static) collapse(2)
+    for (int n = n0; n < n1; ++n)
+        for (int k = 0; k < K; ++k)
+            for (int oy = 0; oy < OH; ++oy)
+                for (int ox = 0; ox < OW; ++ox) {
+                    float acc = 0.f;
+                    for (int c = 0; c < C; ++c)
+                        for (int r = 0; r < R; ++r)
+                            for (int s2 = 0; s2 < S; ++s2) {
+                                const int iy = oy * stride + r - pad, ix = ox * stride + s2 - pad;
+                                if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
+                                acc = fmaf(w[(((size_t)k * C + c) * R + r) * S + s2],
+                                           x[(((size_t)n * C + c) * H + iy) * W + ix], acc);
+                            }
+                    out[(((size_t)n * K + k) * OH + oy) * OW + ox] = acc;
+                }
 }

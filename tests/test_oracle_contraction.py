@@ -43,3 +43,44 @@ def test_decoders():
     # every finite e4m3 value appears once per sign; 448 is the largest
     fin = oracle.E4M3[np.isfinite(oracle.E4M3)]
     assert fin.max() == 448.0 and len(np.unique(np.abs(fin))) == 127
+
+
+def test_gemm_and_conv_f32_loops():
+    # the float restatements of main/gemm.cu:69-78 and main/fprop.cu:84-108 used as CPU baselines:
+    # exact on small integers (equal to the int32 loops), close to FP64 on N(0,1), and the same
+    # for any thread count and row range
+    g = np.random.default_rng(11)
+    A = g.integers(-3, 4, size=(40, 33)).astype(np.float32)
+    B = g.integers(-3, 4, size=(33, 27)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.gemm_f32_rows(A, B, 0, 40), (A.astype(np.int64) @ B.astype(np.int64)))
+    part = oracle.gemm_f32_rows(A, B, 5, 9)
+    np.testing.assert_array_equal(part[5:9], (A[5:9].astype(np.int64) @ B.astype(np.int64)))
+    assert not part[:5].any() and not part[9:].any()
+    x = g.integers(0, 11, size=(2, 5, 9, 8))
+    w = g.integers(0, 11, size=(4, 5, 3, 3))
+    for stride, pad in ((1, 1), (2, 1), (1, 0)):
+        np.testing.assert_array_equal(oracle.conv_f32(x, w, stride, pad).astype(np.int64),
+                                      oracle.conv_direct_i32(x, w, stride, pad).astype(np.int64))
+    xf = g.standard_normal((2, 6, 10, 7)).astype(np.float32)
+    wf = g.standard_normal((3, 6, 3, 3)).astype(np.float32)
+    one = oracle.conv_f32(xf, wf, 1, 1)
+    assert np.abs(one - oracle.conv_f64(xf, wf, 1, 1)).max() < 1e-4
+    try:
+        oracle.set_threads(4)
+        np.testing.assert_array_equal(oracle.conv_f32(xf, wf, 1, 1), one)
+        np.testing.assert_array_equal(oracle.gemm_f32_rows(A, B, 0, 40), oracle.gemm_f32_rows(A, B, 0, 40))
+    finally:
+        oracle.set_threads(1)
+
+
+def test_me_oracle_threads_do_not_change_results():
+    from umi_amd import workloads
+
+    cur, ref, _ = workloads.me_pair(96, 160, 8, seed=4)
+    one, _ = oracle.me_full_search(cur, ref, 8, 8)
+    try:
+        oracle.set_threads(4)
+        many, _ = oracle.me_full_search(cur, ref, 8, 8)
+    finally:
+        oracle.set_threads(1)
+    np.testing.assert_array_equal(one, many)

@@ -76,6 +76,34 @@ class DeviceBuffer:
             pass
 
 
+class HostBuffer:
+    """Pinned host memory (umi_host_alloc) of ``nbytes`` bytes, viewed as numpy."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        call("umi_host_alloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p.value or 0
+
+    def array(self, shape, dtype) -> np.ndarray:
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        if n > self.nbytes:
+            raise ValueError(f"view of {n} B on a {self.nbytes} B buffer")
+        buf = (C.c_char * n).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", 0):
+            lib().umi_host_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     """A non-blocking HIP stream (default priority)."""
 
