@@ -188,3 +188,22 @@ def test_conv_direct_equals_implicit_gemm(shape, mode, gpu):
     f64 = oracle.conv_f64(x, w, 1, 1)
     scale = np.sqrt(64 * 9) * np.abs(x).max() * np.abs(w).max()
     assert np.abs(out - f64).max() <= PRECISION[mode] * 40 * scale
+
+
+@pytest.mark.parametrize("mode", MODES, ids=[ops.MODE_NAMES[m] for m in MODES])
+@pytest.mark.parametrize("M,N,K", [(4096, 4096, 256), (4000, 4100, 320)])
+def test_gemm_direct_lds_equals_register_staging(mode, M, N, K, gpu):
+    # 256 x 256 tiles staged by direct-to-LDS loads use the register-staged kernel's
+    # operands, LDS layout and MFMA order: identical bits (ragged M, N included)
+    from umi_amd._lib import call
+
+    g = workloads.rng(M + N + K + mode)
+    A = g.standard_normal((M, K)).astype(np.float32)
+    B = g.standard_normal((K, N)).astype(np.float32)
+    print(f"[gemm] direct-LDS vs register staging {ops.MODE_NAMES[mode]} {M}x{N}x{K}", flush=True)
+    try:
+        call("umi_gemm_set_lds_direct", 0)
+        ref = ops.gemm_np(A, B, mode)
+    finally:
+        call("umi_gemm_set_lds_direct", 1)
+    np.testing.assert_array_equal(ops.gemm_np(A, B, mode), ref)

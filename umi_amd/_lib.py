@@ -53,6 +53,7 @@ SIGNATURES = {
     "umi_haar_features_u32": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, P, P],
     "umi_me_subpel_refine_u8": [P, P, c_int, c_int, c_int, P, P, P],
     "umi_conv_set_direct": [c_int],
+    "umi_gemm_set_lds_direct": [c_int],
     "umi_batch_box_sum_u32": [P, P, c_int, c_int, P, P],
     "umi_filter3x3_f32": [P, c_int, c_int, c_int, c_float, c_float, c_float, c_int, P, c_int, P],
     "umi_filter3x3_i32": [P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_int, P],

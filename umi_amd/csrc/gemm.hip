@@ -428,6 +428,189 @@ umi_status gemm_launch_t(ASrc ah, ASrc al, BSrc bh, BSrc bl, Dst d, int M, int N
     return launch_check("gemm_kernel");
 }
 
+// Dense 256 x 256 tiles with direct-to-LDS loads (global_load_lds_dwordx4):
+// the same operands, LDS layout (64-byte rows, 16-byte chunks swizzled by
+// (row >> 2) & 3), MFMA sequence and epilogue as gemm_kernel<TileGemm>, so
+// the results are bit-identical; what changes is the staging.  A wave's
+// direct load writes 1 KB of LDS in lane order (16 rows x 64 B), so a lane
+// fetches the global chunk that its LDS slot holds after the swizzle; no
+// register ring, no LDS stores.  NS stages of LDS (4 x 32 KB; 2 x 64 KB for
+// the split forms' hi and lo panels) keep NS - 1 stages of loads in flight:
+// step st waits (s_waitcnt vmcnt) for its own stage only, one barrier, then
+// refills the buffer freed by step st - 1 with stage st + NS - 1.
+bool gemm_lds_direct = true;  // umi_gemm_set_lds_direct
+
+template <int N>
+__device__ inline void wait_vm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// WGM x WGN waves of TM x TN MFMA blocks (32 x 32): 2 x 2 waves of 4 x 4
+// (one wave per SIMD) or 2 x 4 waves of 4 x 2 (two per SIMD, half the
+// accumulators each); both cover 256 x 256.
+template <int WGM_, int WGN_, int TM_, int TN_>
+struct LdsTile {
+    static constexpr int WGM = WGM_, WGN = WGN_, TM = TM_, TN = TN_;
+    static constexpr int NT = WGM * WGN * 64, BM = WGM * TM * 32, BN = WGN * TN * 32;
+    static_assert(BM == 256 && BN == 256, "256 x 256 tiles");
+};
+using LdsTile4 = LdsTile<2, 2, 4, 4>;
+using LdsTile8 = LdsTile<2, 4, 4, 2>;
+
+template <int MODE, class TL>
+__global__ __launch_bounds__(TL::NT) void gemm_lds_kernel(const uint8_t *__restrict__ a_hi,
+                                                            const uint8_t *__restrict__ a_lo, long lda,
+                                                            const uint8_t *__restrict__ b_hi,
+                                                            const uint8_t *__restrict__ b_lo, long ldb, DenseDst dst,
+                                                            int M, int N, int K, int tiles_m, int tiles_n,
+                                                            const float *qa, const float *qb) {
+    using OP = Mfma<MODE>;
+    constexpr bool SPLIT = OP::NOP == 4;
+    constexpr int NS = SPLIT ? 2 : 4;
+    constexpr int BMt = TL::BM, BNt = TL::BN, TM = TL::TM, TN = TL::TN;
+    constexpr int OFF_B = BMt * LDSROW, OFF_LO = (BMt + BNt) * LDSROW;
+    constexpr int STAGE = (SPLIT ? 2 : 1) * (BMt + BNt) * LDSROW;
+    constexpr int NW = TL::NT / 64;           // waves
+    constexpr int WL = BMt / 16 / NW;         // direct loads of 16 rows per wave per panel
+    constexpr int LPS = (SPLIT ? 4 : 2) * WL; // direct loads per thread per stage
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wv / TL::WGN, wn = wv % TL::WGN;
+
+    // tile order as gemm_kernel: XCD-aware, then grouped raster
+    const int ntiles = tiles_m * tiles_n;
+    int pid = blockIdx.x;
+    if (ntiles % 8 == 0) pid = (pid % 8) * (ntiles / 8) + pid / 8;
+    const int GROUP = 4;
+    const int group = pid / (GROUP * tiles_n), first_m = group * GROUP;
+    const int gsz = min(GROUP, tiles_m - first_m);
+    const int tm = first_m + (pid % (GROUP * tiles_n)) % gsz;
+    const int tn = (pid % (GROUP * tiles_n)) / gsz;
+    const int m0 = tm * BMt, n0 = tn * BNt;
+    const int nstages = K * OP::ESZ / ROWB;
+
+    // lane's source rows: load i of the wave covers panel rows (wv * WL + i) * 16 .. + 15;
+    // the lane's LDS slot (row r, chunk slot lane & 3) holds global chunk slot ^ sw(r)
+    const int lrow = lane >> 2, lslot = lane & 3;
+    const uint8_t *srcA[WL], *srcB[WL], *srcAl[WL], *srcBl[WL];
+#pragma unroll
+    for (int i = 0; i < WL; ++i) {
+        const int r = (wv * WL + i) * 16 + lrow;
+        const int chunk = (lslot ^ ((r >> 2) & 3)) * 16;
+        const long ra = (long)min(m0 + r, M - 1) * lda + chunk, rb = (long)min(n0 + r, N - 1) * ldb + chunk;
+        srcA[i] = a_hi + ra;
+        srcB[i] = b_hi + rb;
+        if constexpr (SPLIT) {
+            srcAl[i] = a_lo + ra;
+            srcBl[i] = b_lo + rb;
+        }
+    }
+    auto issue = [&](int st, int buf) {
+        uint8_t *base = smem + buf * STAGE;
+        const long kb = (long)st * ROWB;
+#pragma unroll
+        for (int i = 0; i < WL; ++i) {
+            const int row0 = (wv * WL + i) * 16 * LDSROW;
+            __builtin_amdgcn_global_load_lds((const void *)(srcA[i] + kb),
+                                             (__attribute__((address_space(3))) void *)(base + row0), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(srcB[i] + kb),
+                                             (__attribute__((address_space(3))) void *)(base + OFF_B + row0), 16, 0, 0);
+            if constexpr (SPLIT) {
+                __builtin_amdgcn_global_load_lds((const void *)(srcAl[i] + kb),
+                                                 (__attribute__((address_space(3))) void *)(base + OFF_LO + row0), 16,
+                                                 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(srcBl[i] + kb),
+                    (__attribute__((address_space(3))) void *)(base + OFF_LO + OFF_B + row0), 16, 0, 0);
+            }
+        }
+    };
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    constexpr int UNIT = 127, LO = 127 - 4;
+    const int lr = lane & 31, lk = (lane >> 5) * OP::LANEB;
+    const int rsw = (lr >> 2) & 3;
+    auto compute = [&](int buf) {
+        const uint8_t *base = smem + buf * STAGE;
+        const uint8_t *pa = base + (wm * TM * 32 + lr) * LDSROW;
+        const uint8_t *pb = base + OFF_B + (wn * TN * 32 + lr) * LDSROW;
+#pragma unroll
+        for (int kc = 0; kc < OP::CH; ++kc) {
+            const int off = kc * 2 * OP::LANEB + lk;
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const uint8_t *ai = pa + i * 32 * LDSROW;
+                    const uint8_t *bj = pb + j * 32 * LDSROW;
+                    acc[i][j] = OP::run(ai, bj, off, rsw, acc[i][j], UNIT, UNIT);
+                    if constexpr (SPLIT) {
+                        constexpr int S1 = (MODE == GM_FP8X3) ? LO : UNIT;
+                        acc[i][j] = OP::run(ai + OFF_LO, bj, off, rsw, acc[i][j], S1, UNIT);
+                        acc[i][j] = OP::run(ai, bj + OFF_LO, off, rsw, acc[i][j], UNIT, S1);
+                    }
+                }
+        }
+    };
+    for (int st = 0; st < NS - 1 && st < nstages; ++st) issue(st, st);
+    for (int st = 0; st < nstages; ++st) {
+        // stage st has landed once at most the loads of the later stages issued so far are outstanding
+        if (st + NS - 2 < nstages) wait_vm<LPS * (NS - 2)>();
+        else wait_vm<0>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();  // every wave's stage st is in LDS; buffer (st - 1) % NS is free
+        asm volatile("" ::: "memory");
+        if (st + NS - 1 < nstages) issue(st + NS - 1, (st + NS - 1) % NS);
+        compute(st % NS);
+    }
+
+    const float alpha = (qa ? 1.0f / *qa : 1.0f) * (qb ? 1.0f / *qb : 1.0f);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const auto cj = dst.col(n0 + wn * TN * 32 + j * 32 + lr);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * TM * 32 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                dst.put(m, cj, acc[i][j][r] * alpha);
+            }
+    }
+}
+
+int gemm_lds_waves = 8;  // umi_gemm_set_lds_direct: 4 or 8 waves per workgroup
+
+template <int MODE, class TL>
+umi_status gemm_lds_launch_t(const uint8_t *ah, const uint8_t *al, long lda, const uint8_t *bh, const uint8_t *bl,
+                             long ldb, DenseDst d, int M, int N, int K, const float *qa, const float *qb,
+                             hipStream_t st) {
+    using OP = Mfma<MODE>;
+    constexpr int NS = OP::NOP == 4 ? 2 : 4;
+    const size_t lds = (size_t)NS * (OP::NOP / 2) * (TL::BM + TL::BN) * LDSROW;
+    static const hipError_t attr = hipFuncSetAttribute((const void *)gemm_lds_kernel<MODE, TL>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    UMI_TRY(hip_check(attr, "hipFuncSetAttribute(gemm LDS)"));
+    const int tm = ceil_div(M, TL::BM), tn = ceil_div(N, TL::BN);
+    hipLaunchKernelGGL((gemm_lds_kernel<MODE, TL>), dim3(tm * tn), dim3(TL::NT), lds, st, ah, al, lda, bh, bl, ldb, d,
+                       M, N, K, tm, tn, qa, qb);
+    return launch_check("gemm_lds_kernel");
+}
+
+template <int MODE>
+umi_status gemm_lds_launch(const uint8_t *ah, const uint8_t *al, long lda, const uint8_t *bh, const uint8_t *bl,
+                           long ldb, DenseDst d, int M, int N, int K, const float *qa, const float *qb,
+                           hipStream_t st) {
+    if (gemm_lds_waves == 8) return gemm_lds_launch_t<MODE, LdsTile8>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
+    return gemm_lds_launch_t<MODE, LdsTile4>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
+}
+
 // Tile choice: the 256 x 256 tile when it yields at least one workgroup per
 // CU, otherwise the 128 x 128 one; the 64 x 128 tile when M <= 64.
 template <int MODE, class ASrc, class BSrc, class Dst>
@@ -678,6 +861,13 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
 
 extern "C" {
 
+umi_status umi_gemm_set_lds_direct(int enable) {
+    if (enable < 0 || enable > 8) return set_error(UMI_ERR_INVALID, "umi_gemm_set_lds_direct: %d", enable);
+    gemm_lds_direct = enable != 0;
+    if (enable == 4 || enable == 8) gemm_lds_waves = enable;
+    return UMI_OK;
+}
+
 umi_status umi_conv_set_direct(int enable) {
     direct_conv_enabled = enable != 0;
     return UMI_OK;
@@ -731,6 +921,21 @@ umi_status umi_gemm(int mode, int M, int N, int K, const void *a_hi, const void 
     const DenseSrc bh{(const uint8_t *)b_hi, (long)ldb * e, N}, bl{(const uint8_t *)b_lo, (long)ldb * e, N};
     const DenseDst d{C, ldc, M, N};
     hipStream_t st = as_stream(s);
+    if (gemm_lds_direct && M > 64 && (long)ceil_div(M, 256) * ceil_div(N, 256) >= 256 &&
+        ((reinterpret_cast<uintptr_t>(a_hi) | reinterpret_cast<uintptr_t>(b_hi) |
+          (split(mode) ? reinterpret_cast<uintptr_t>(a_lo) | reinterpret_cast<uintptr_t>(b_lo) : 0)) & 15) == 0) {
+        const auto *pah = (const uint8_t *)a_hi, *pal = (const uint8_t *)a_lo;
+        const auto *pbh = (const uint8_t *)b_hi, *pbl = (const uint8_t *)b_lo;
+        const long la = (long)lda * e, lb = (long)ldb * e;
+        switch (mode) {
+            case GM_F32: return gemm_lds_launch<GM_F32>(pah, pal, la, pbh, pbl, lb, d, M, N, K, nullptr, nullptr, st);
+            case GM_BF16: return gemm_lds_launch<GM_BF16>(pah, pal, la, pbh, pbl, lb, d, M, N, K, nullptr, nullptr, st);
+            case GM_BF16X3:
+                return gemm_lds_launch<GM_BF16X3>(pah, pal, la, pbh, pbl, lb, d, M, N, K, nullptr, nullptr, st);
+            case GM_FP8: return gemm_lds_launch<GM_FP8>(pah, pal, la, pbh, pbl, lb, d, M, N, K, qa, qb, st);
+            default: return gemm_lds_launch<GM_FP8X3>(pah, pal, la, pbh, pbl, lb, d, M, N, K, qa, qb, st);
+        }
+    }
     switch (mode) {
         case GM_F32: return gemm_launch<GM_F32>(ah, al, bh, bl, d, M, N, K, nullptr, nullptr, st);
         case GM_BF16: return gemm_launch<GM_BF16>(ah, al, bh, bl, d, M, N, K, nullptr, nullptr, st);
