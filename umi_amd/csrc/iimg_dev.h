@@ -268,22 +268,26 @@ __device__ inline Col pub_sum(const uint64_t *p, long long stride, int n, uint32
 
 // Decoupled look-back over n predecessors at p - stride, p - 2 stride, ...
 // (nearest first): the sum of their aggregates up to and including the
-// nearest inclusive value.  Loads go out in windows of LB_WIN (16 values of
-// 32 bits, 8 of 64); the first window is loaded by prefetch() ahead of the
-// work that precedes resolve().
+// nearest inclusive value.  prefetch() loads the nearest one ahead of the
+// work that precedes resolve() (it usually already shows its inclusive value
+// or soon does); further predecessors go out in windows of LB_WIN, so a tile
+// rarely reads more than the one value per column.
 template <class Col>
 struct Lookback {
-    static constexpr int LB_WIN = sizeof(Col) == 8 ? 8 : 16;
+    static constexpr int LB_WIN = 8;
     PubVal<Col> v[LB_WIN];
     __device__ inline void prefetch(const uint64_t *p, long long stride, int n) {
-#pragma unroll
-        for (int u = 0; u < LB_WIN; ++u)
-            if (u < n) v[u].load(p - (u + 1) * stride);
+        if (n > 0) v[0].load(p - stride);
     }
     __device__ inline Col resolve(const uint64_t *p, long long stride, int n, uint32_t tag, IimgCtl *ctl) {
-        Col acc = 0;
-        for (int i0 = 0; i0 < n; i0 += LB_WIN) {
-            if (i0 > 0) prefetch(p - i0 * stride, stride, n - i0);
+        if (n <= 0) return 0;
+        v[0].wait(p - stride, tag, ctl);
+        Col acc = v[0].value();
+        if (v[0].state()) return acc;
+        for (int i0 = 1; i0 < n; i0 += LB_WIN) {
+#pragma unroll
+            for (int u = 0; u < LB_WIN; ++u)
+                if (i0 + u < n) v[u].load(p - (i0 + u + 1) * stride);
 #pragma unroll
             for (int u = 0; u < LB_WIN; ++u)
                 if (i0 + u < n) {
