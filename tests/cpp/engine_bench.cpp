@@ -34,10 +34,18 @@ static double time_us(F &&launch, int reps) {
     return ms * 1e3 / reps;
 }
 
-static void report(const char *name, double us, double bytes, double ops, const char *ops_unit) {
+static void report(const char *name, double us, double bytes, double ops, const char *ops_unit,
+                   const TuneResult *t = nullptr) {
     std::printf("{\"case\": \"%s\", \"us\": %.2f, \"bytes\": %.0f, \"gbs\": %.1f, \"ops\": %.0f, \"ops_per_s\": %.4g, "
-                "\"ops_unit\": \"%s\"}\n",
+                "\"ops_unit\": \"%s\"",
                 name, us, bytes, bytes / (us * 1e-6) / 1e9, ops, ops / (us * 1e-6), ops_unit);
+    if (t)
+        std::printf(", \"tuned_us\": %.2f, \"tuned_gbs\": %.1f, \"tuned_ops_per_s\": %.4g, \"tuned_expand\": \"%04llx\", "
+                    "\"tuned_block\": [%d, %d, %d, %d], \"candidates\": %d",
+                    t->us, bytes / (t->us * 1e-6) / 1e9, ops / (t->us * 1e-6), (unsigned long long)t->expand,
+                    t->p_dims[0].block_size, t->p_dims[1].block_size, t->p_dims[2].block_size, t->p_dims[3].block_size,
+                    t->candidates);
+    std::printf("}\n");
     std::fflush(stdout);
 }
 
@@ -71,7 +79,8 @@ int main() {
         }};
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<WeightedFilter, 0x111, 0x1122, 0xc0>(nullptr, pd, ad, I, O, {-5})); }, R);
-        report("wfilter3x3 int32 1024x1024 (Expand 1122, ZeroStrideOpt c0)", us, 8.0 * h * w, 9.0 * h * w, "MAC");
+        const TuneResult t = Autotune<WeightedFilter, 0x111, 0xc0, 0x1111, 0x1112, 0x1122, 0x1124, 0x1142, 0x1144>(nullptr, pd, ad, I, O, {-5}, R);
+        report("wfilter3x3 int32 1024x1024 (Expand 1122, ZeroStrideOpt c0)", us, 8.0 * h * w, 9.0 * h * w, "MAC", &t);
     }
     {  // main/filter3x3.cu:36-58 at 1024 x 1024
         const int h = 1024, w = 1024;
@@ -82,7 +91,8 @@ int main() {
                                                 {{{1, PAD}, {1, PAD}, {h, PAD}, {w, PAD}}}, DEFAULT_SHM, {in}, {999}}}};
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<BoxFilter, 0x11, 0x1111, 0x1122>(nullptr, pd, ad, I, O, {-5, 2})); }, R);
-        report("filter3x3 int32 1024x1024", us, 8.0 * h * w, 9.0 * h * w, "add");
+        const TuneResult t = Autotune<BoxFilter, 0x11, 0, 0x1111, 0x1112, 0x1122, 0x1124, 0x1142, 0x1144>(nullptr, pd, ad, I, O, {-5, 2}, R);
+        report("filter3x3 int32 1024x1024", us, 8.0 * h * w, 9.0 * h * w, "add", &t);
     }
     {  // main/bilateral.cu:54-99, 5 x 5 window, 1024 x 1024
         const int h = 1024, w = 1024, ofs = 2, k = 5;
@@ -97,7 +107,8 @@ int main() {
         }};
         const std::array<UnrolledO<float>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<BilateralF, 0x11111>(nullptr, pd, ad, I, O, {-50.f})); }, R);
-        report("bilateral f32 1024x1024 5x5", us, 8.0 * h * w, 25.0 * h * w, "tap");
+        const TuneResult t = Autotune<BilateralF, 0x11111, 0, 0x1111, 0x1112, 0x1121, 0x1122, 0x1124>(nullptr, pd, ad, I, O, {-50.f}, R);
+        report("bilateral f32 1024x1024 5x5", us, 8.0 * h * w, 25.0 * h * w, "tap", &t);
     }
     {  // main/iimg.cu:37-82, both passes, 1080 x 1920
         const int h = 1080, w = 1920;
@@ -127,8 +138,9 @@ int main() {
         }};
         const std::array<UnrolledO<int>, 1> O{{{{{{0, 0, 1}, {1, 0, 1}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{h / b, WRAP}, {w / b, WRAP}, {sr, WRAP}, {sr, WRAP}}}, {sad}}}};
         const double us = time_us([&] { ok(ExecuteAsync<Sad, 0x111, 0x2211>(nullptr, pd, ad, I, O)); }, 5);
+        const TuneResult t = Autotune<Sad, 0x111, 0xc0, 0x1111, 0x2211, 0x1122, 0x2222, 0x4411>(nullptr, pd, ad, I, O, {}, 5);
         const double nsad = (double)(h / b) * (w / b) * sr * sr;
-        report("full_me int32 1920x1080, 8x8, 15x15 SAD volume", us, 4.0 * h * w + 4.0 * nsad, nsad * b * b, "absdiff");
+        report("full_me int32 1920x1080, 8x8, 15x15 SAD volume", us, 4.0 * h * w + 4.0 * nsad, nsad * b * b, "absdiff", &t);
     }
     {  // main/cost_volume.cu:35-65 at 1242 x 375, 7-wide window, 15 shifts
         const int h = 375, w = 1242, k = 7, sr = 15, k_ofs = k / 2, sr_ofs = sr / 2;
@@ -141,7 +153,8 @@ int main() {
         }};
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {1, 0, 1}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{1, WRAP}, {h, WRAP}, {w, WRAP}, {sr, WRAP}}}, {cv}}}};
         const double us = time_us([&] { ok(ExecuteAsync<Sad, 0x111>(nullptr, pd, ad, I, O)); }, R);
-        report("cost_volume int32 1242x375, 7-wide, 15 shifts", us, 4.0 * h * w + 4.0 * h * w * sr, (double)h * w * sr * k, "absdiff");
+        const TuneResult t = Autotune<Sad, 0x111, 0, 0x1111, 0x1211, 0x1121, 0x1221, 0x1411>(nullptr, pd, ad, I, O, {}, R);
+        report("cost_volume int32 1242x375, 7-wide, 15 shifts", us, 4.0 * h * w + 4.0 * h * w * sr, (double)h * w * sr * k, "absdiff", &t);
     }
     {  // main/fprop.cu:53-82: N=16, C=64, 56x56, K=64, 3x3, stride 1, pad 1 (int32)
         const int bs = 16, ich = 64, och = 64, ih = 56, iw = 56, f = 3, s = 1, ofs = -1, oh = 56, ow = 56;
@@ -155,8 +168,9 @@ int main() {
         }};
         const std::array<UnrolledO<int>, 1> O{{{{{{0, 0, 1}, {1, 0, 1}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{bs, PAD}, {och, PAD}, {oh, PAD}, {ow, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<Dot>(nullptr, pd, ad, I, O)); }, 5);
+        const TuneResult t = Autotune<Dot, 0x111, 0xc2, 0x1111, 0x1211, 0x1112, 0x1212, 0x1412, 0x1214, 0x1414>(nullptr, pd, ad, I, O, {}, 5);
         const double macs = (double)bs * och * oh * ow * ich * f * f;
-        report("fprop int32 16x64x56x56 -> 64, 3x3", us, 4.0 * (bs * ich * ih * iw + och * ich * f * f + (double)bs * och * oh * ow), macs, "MAC");
+        report("fprop int32 16x64x56x56 -> 64, 3x3", us, 4.0 * (bs * ich * ih * iw + och * ich * f * f + (double)bs * och * oh * ow), macs, "MAC", &t);
     }
     {  // main/gemm.cu:37-67 at 1024^3 (int32)
         const int m = 1024, k = 1024, n = 1024;
@@ -169,7 +183,8 @@ int main() {
         }};
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {m, PAD}, {n, PAD}}}, {C}}}};
         const double us = time_us([&] { ok(ExecuteAsync<Dot, 0x111, 0x1124, 0x48>(nullptr, pd, ad, I, O)); }, 5);
-        report("gemm int32 1024^3 (Expand 1124, ZeroStrideOpt 48)", us, 4.0 * (2.0 * m * k + (double)m * n), (double)m * n * k, "MAC");
+        const TuneResult t = Autotune<Dot, 0x111, 0x48, 0x1111, 0x1124, 0x1142, 0x1144, 0x1148, 0x1184>(nullptr, pd, ad, I, O, {}, 5);
+        report("gemm int32 1024^3 (Expand 1124, ZeroStrideOpt 48)", us, 4.0 * (2.0 * m * k + (double)m * n), (double)m * n * k, "MAC", &t);
     }
     return 0;
 }

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <numeric>
 #include <random>
+#include <utility>
 #include <vector>
 
 #include "SyncedMemory.h"
@@ -317,6 +318,60 @@ static void case_planes(int h, int w, int depth) {
     compare("planes: second Post output", post2.s.get_cpu_ro(), c2.data(), c2.size(), 0);
 }
 
+static void case_tuned_wfilter_gemm(int h, int w, int m, int k, int n) {
+    // Autotune over Expand x block shapes, then ExecuteTuned: the same outputs as the interpreter
+    Buf<int> in(w * h), out(w * h), wt(9);
+    fill_int(in.host(), in.n, -1000, 1000);
+    fill_int(wt.host(), 9, -5, 5);
+    const NDTuple<ParallelDim> pd{{{1, 1, 1, 1}, {1, 1, 1, 1}, {h, 16, 4, 1}, {w, 16, 8, 1}}};
+    const NDTuple<AccumulationDim> ad{{{1, 1}, {1, 1}, {3, 3}, {3, 3}}};
+    auto wf_io = [&](const int *ip, const int *wp, int *op) {
+        const std::array<UnrolledI<int>, 2> I{{
+            {{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {{{}, {}, {2, -1, 1}, {3, -1, 1}}}, {{{1, PAD}, {1, PAD}, {h, PAD}, {w, PAD}}}, DEFAULT_SHM, {ip}, {56}},
+            {{{{}, {}, {}, {}}}, {{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {{{1, PAD}, {1, PAD}, {3, PAD}, {3, PAD}}}, DEFAULT_SHM, {wp}, {}},
+        }};
+        const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {op}}}};
+        return std::make_pair(I, O);
+    };
+    {
+        auto io = wf_io(in.s.get_gpu_ro(), wt.s.get_gpu_ro(), out.s.get_gpu_wo());
+        const TuneResult t = Autotune<WeightedFilter, 0x111, 0xc0, 0x1111, 0x1122, 0x1124, 0x1142>(nullptr, pd, ad, io.first, io.second, {-5}, 2);
+        const umi_status st = ExecuteTuned<WeightedFilter, 0x111, 0xc0, 0x1111, 0x1122, 0x1124, 0x1142>(t, nullptr, ad, io.first, io.second, {-5});
+        umi_stream_synchronize(nullptr);
+        std::printf("[engine] tuned wfilter: %d candidates, Expand %04llx, block %d x %d, %.1f us (status %d)\n",
+                    t.candidates, (unsigned long long)t.expand, t.p_dims[2].block_size, t.p_dims[3].block_size, t.us, st);
+        auto cio = wf_io(in.s.get_cpu_ro(), wt.s.get_cpu_ro(), out.cpu_out.data());
+        umi_oracle::Execute<WeightedFilter, 0x111>(pd, ad, cio.first, cio.second, {-5});
+        if (st != UMI_OK || t.candidates < 8) ++g_fail;
+        compare("wfilter3x3 with the tuned mapping", out.s.get_cpu_ro(), out.cpu_out.data(), out.n, 0);
+    }
+    Buf<int> A(m * k), B(k * n), C(m * n);
+    fill_int(A.host(), A.n, -100, 100);
+    fill_int(B.host(), B.n, -100, 100);
+    const NDTuple<ParallelDim> gpd{{{1, 1, 1, 1}, {1, 1, 1, 1}, {m, 16, 2, 1}, {n, 16, 16, 1}}};
+    const NDTuple<AccumulationDim> gad{{{1, 1}, {1, 1}, {1, 1}, {k, 16}}};
+    auto g_io = [&](const int *ap, const int *bp, int *cp) {
+        const std::array<UnrolledI<int>, 2> I{{
+            {{{{}, {}, {2, 0, 1}, {}}}, {{{}, {}, {}, {3, 0, 1}}}, {{{1, PAD}, {1, PAD}, {m, PAD}, {k, PAD}}}, DEFAULT_SHM, {ap}, {}},
+            {{{{}, {}, {}, {3, 0, 1}}}, {{{}, {}, {}, {2, 0, 1}}}, {{{1, PAD}, {1, PAD}, {k, PAD}, {n, PAD}}}, DEFAULT_SHM, {bp}, {}},
+        }};
+        const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {m, PAD}, {n, PAD}}}, {cp}}}};
+        return std::make_pair(I, O);
+    };
+    {
+        auto io = g_io(A.s.get_gpu_ro(), B.s.get_gpu_ro(), C.s.get_gpu_wo());
+        const TuneResult t = Autotune<Dot, 0x111, 0x48, 0x1111, 0x1124, 0x1142, 0x1144>(nullptr, gpd, gad, io.first, io.second, {}, 2);
+        const umi_status st = ExecuteTuned<Dot, 0x111, 0x48, 0x1111, 0x1124, 0x1142, 0x1144>(t, nullptr, gad, io.first, io.second, {});
+        umi_stream_synchronize(nullptr);
+        std::printf("[engine] tuned gemm: %d candidates, Expand %04llx, block %d x %d, %.1f us (status %d)\n",
+                    t.candidates, (unsigned long long)t.expand, t.p_dims[2].block_size, t.p_dims[3].block_size, t.us, st);
+        auto cio = g_io(A.s.get_cpu_ro(), B.s.get_cpu_ro(), C.cpu_out.data());
+        umi_oracle::Execute<Dot, 0x111>(gpd, gad, cio.first, cio.second, {});
+        if (st != UMI_OK || t.candidates < 8) ++g_fail;
+        compare("gemm with the tuned mapping", C.s.get_cpu_ro(), C.cpu_out.data(), C.n, 0);
+    }
+}
+
 static void case_invalid_dims() {
     // Product of warp sizes 16 != 32: a status, not an abort, from ExecuteAsync.
     const NDTuple<ParallelDim> pd{{{1, 1, 1, 1}, {1, 1, 1, 1}, {4, 4, 2, 1}, {4, 8, 8, 1}}};
@@ -344,6 +399,7 @@ int main() {
     case_fprop(2, 13, 9, 20, 24, 7, 2);
     case_gemm(100, 100, 200);
     case_planes(21, 19, 8);
+    case_tuned_wfilter_gemm(123, 301, 96, 80, 200);
     case_invalid_dims();
     std::printf("[engine] %s\n", g_fail ? "FAILED" : "all cases passed");
     return g_fail ? 1 : 0;

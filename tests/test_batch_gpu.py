@@ -47,22 +47,25 @@ def test_batch_bilateral_equals_single(gpu, config):
         np.testing.assert_array_equal(got, ops.bilateral_np(im, 2, 2.0, 0.1))
 
 
-def test_batch_iimg_u8_and_f32(gpu):
+@pytest.mark.parametrize("config", range(3))
+def test_batch_iimg_u8_and_f32(gpu, config):
     imgs, u8 = small_batch(seed=5)
     bu = batch.ImageBatch(u8, out_dtype=np.uint32)
-    bu.iimg(bu.iimg_workspace())
+    bu.iimg(bu.iimg_workspace(), config=config)
     for im, got in zip(u8, bu.outputs()):
         np.testing.assert_array_equal(got, oracle.iimg(im))
     bf = batch.ImageBatch(imgs)
-    bf.iimg(bf.iimg_workspace())
+    bf.iimg(bf.iimg_workspace(), config=config)
     for im, got in zip(imgs, bf.outputs()):
         ref = oracle.iimg(im)
         assert np.max(np.abs(got - ref) / ref) <= 4e-6
 
 
-def test_full_uneven_batch_all_ops_in_graphs(gpu):
+def test_full_uneven_batch_all_ops_in_graphs(gpu, tmp_path, monkeypatch):
     # 256 float32 images, sides log-uniform in [64, 1024]: bilateral 5x5 (sigma_s 2, sigma_r 0.1),
     # wfilter3x3 and integral images of uint8 and float copies, each op one graph-captured launch
+    # at the tile configuration the autotuner picks for this batch
+    monkeypatch.setenv("UMI_TUNE_CACHE", str(tmp_path / "tune.json"))
     imgs, u8 = workloads.uneven_batch(256, 64, 1024, seed=1)
     print(f"[batch] {sum(i.size for i in imgs)} px in 256 images", flush=True)
     bf, bu = batch.ImageBatch(imgs), batch.ImageBatch(u8, out_dtype=np.uint32)
@@ -72,12 +75,20 @@ def test_full_uneven_batch_all_ops_in_graphs(gpu):
     wsp_u, wsp_f = bu.iimg_workspace(), bf.iimg_workspace()
     outs = {k: DeviceBuffer(bf.total * 4) for k in ("wf", "bl", "if")}
     st = Stream()
+    pick = {
+        "wf": batch.autotune(batch.OP_FILTER, bf, lambda c, s: bf.wfilter3x3(w, 0.0, 0.0, 0, c, s, out=outs["wf"]))[0],
+        "bl": batch.autotune(batch.OP_BILATERAL, bf,
+                             lambda c, s: bf.bilateral(2, ws, norm, 0.0, 0, c, s, out=outs["bl"]))[0],
+        "if": batch.autotune(batch.OP_IIMG, bf, lambda c, s: bf.iimg(wsp_f, s, out=outs["if"], config=c))[0],
+        "iu": batch.autotune(batch.OP_IIMG, bu, lambda c, s: bu.iimg(wsp_u, s, config=c))[0],
+    }
+    print(f"[batch] tuned configurations {pick}", flush=True)
 
     def run(s):
-        bf.wfilter3x3(w, 0.0, 0.0, 0, -1, s, out=outs["wf"])
-        bf.bilateral(2, ws, norm, 0.0, 0, -1, s, out=outs["bl"])
-        bf.iimg(wsp_f, s, out=outs["if"])
-        bu.iimg(wsp_u, s)
+        bf.wfilter3x3(w, 0.0, 0.0, 0, pick["wf"], s, out=outs["wf"])
+        bf.bilateral(2, ws, norm, 0.0, 0, pick["bl"], s, out=outs["bl"])
+        bf.iimg(wsp_f, s, out=outs["if"], config=pick["if"])
+        bu.iimg(wsp_u, s, config=pick["iu"])
 
     run(st)
     st.synchronize()

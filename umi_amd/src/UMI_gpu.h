@@ -42,6 +42,7 @@
 #include <cassert>  // the examples use assert() through this header, as with the reference's
 #include <cstdio>
 #include <cstdlib>
+#include <utility>
 
 #include "UMI.h"
 #include "umi_hip.h"
@@ -580,6 +581,117 @@ void Execute(const NDTuple<ParallelDim> &p_dims, const NDTuple<AccumulationDim> 
         std::fprintf(stderr, "UMI::Execute failed (status %d): %s\n", st, umi_last_error());
         std::abort();
     }
+}
+
+// ---------------------------------------------------------------- tuning
+// Autotuner over the template parameters of a UMI program (no reference
+// counterpart; the reference's mapping is whatever the caller wrote into
+// Expand and the ParallelDim block / warp sizes, src/UMI_gpu.h:409-423,
+// src/UMI.cpp:39-66).  Candidates: every Expand of the list, crossed with
+// workgroup shapes for the two innermost parallel dims (the caller's own
+// first, then 256 threads as 16 x 16, 8 x 32, 32 x 8, 4 x 64, 1 x 256 and 128
+// threads as 8 x 16, 4 x 32), each with warp sizes that keep the reference's
+// rules (product of warp sizes 32, block a multiple of warp x shuf); outer
+// dims keep the caller's values.  A candidate the validator refuses (for
+// instance a ZeroStrideOpt hint that is unsound under a larger Expand) is
+// skipped.  Each is timed with events over `reps` launches on `stream`
+// after one warm-up; results never depend on the choice (the engine's
+// mapping does not change results), so ExecuteTuned may run any of them.
+struct TuneResult {
+    int expand_index = -1;             // position in the Expand list
+    uint64_t expand = 0;
+    NDTuple<ParallelDim> p_dims{};     // block / warp sizes of the fastest candidate
+    float us = 0;                      // its time per launch
+    int candidates = 0;                // candidates timed
+};
+
+namespace engine {
+inline bool shape_pd(const NDTuple<ParallelDim> &base, int b2, int b3, NDTuple<ParallelDim> &out) {
+    out = base;
+    const int w3 = b3 < 32 ? b3 : 32, w2 = 32 / w3;
+    if (32 % w3 != 0 || b2 % w2 != 0) return false;
+    for (int d = 0; d < 2; ++d) out[d] = ParallelDim{base[d].total_size, 1, 1, 1};
+    out[2] = ParallelDim{base[2].total_size, b2, w2, 1};
+    out[3] = ParallelDim{base[3].total_size, b3, w3, 1};
+    return true;
+}
+template <class F>
+inline float time_launches(F &&launch, umi_stream stream, int reps) {
+    if (launch() != UMI_OK) return -1.f;  // warm-up; a refused candidate is skipped
+    umi_timer t;
+    if (umi_timer_create(&t) != UMI_OK) return -1.f;
+    umi_timer_start(t, stream);
+    for (int i = 0; i < reps; ++i) launch();
+    umi_timer_stop(t, stream);
+    float ms = -1.f;
+    umi_timer_elapsed_ms(t, &ms);
+    umi_timer_destroy(t);
+    return ms * 1e3f / reps;
+}
+}  // namespace engine
+
+template <class Umi, uint64_t PtrNum, uint64_t ZeroStrideOpt, uint64_t... Expands>
+struct Tuner {
+    using EType = typename remove_cv<typename Umi::ElementType>::type;
+    using CType = typename remove_cv<typename Umi::UserConstantType>::type;
+    static constexpr uint64_t list[sizeof...(Expands)] = {Expands...};
+
+    template <size_t... I>
+    static umi_status run_index(int idx, umi_stream s, const NDTuple<ParallelDim> &pd, const NDTuple<AccumulationDim> &ad,
+                                const array<UnrolledI<EType>, Umi::TOTAL_IN> &in,
+                                const array<UnrolledO<EType>, Umi::TOTAL_OUT> &out, const CType &c,
+                                std::index_sequence<I...>) {
+        umi_status st = UMI_ERR_INVALID;
+        ((idx == (int)I ? (st = ExecuteAsync<Umi, PtrNum, list[I], ZeroStrideOpt>(s, pd, ad, in, out, c), 0) : 0), ...);
+        return st;
+    }
+    static umi_status run(int idx, umi_stream s, const NDTuple<ParallelDim> &pd, const NDTuple<AccumulationDim> &ad,
+                          const array<UnrolledI<EType>, Umi::TOTAL_IN> &in,
+                          const array<UnrolledO<EType>, Umi::TOTAL_OUT> &out, const CType &c) {
+        return run_index(idx, s, pd, ad, in, out, c, std::make_index_sequence<sizeof...(Expands)>{});
+    }
+};
+
+// Times the candidates and returns the fastest; the outputs hold the results
+// of the last launch (identical for every candidate).
+template <class Umi, uint64_t PtrNum, uint64_t ZeroStrideOpt, uint64_t... Expands>
+TuneResult Autotune(umi_stream stream, const NDTuple<ParallelDim> &p_dims, const NDTuple<AccumulationDim> &a_dims,
+                    const array<UnrolledI<typename remove_cv<typename Umi::ElementType>::type>, Umi::TOTAL_IN> &imems,
+                    const array<UnrolledO<typename remove_cv<typename Umi::ElementType>::type>, Umi::TOTAL_OUT> &omems,
+                    const typename remove_cv<typename Umi::UserConstantType>::type &c =
+                        typename remove_cv<typename Umi::UserConstantType>::type(),
+                    int reps = 10) {
+    using T = Tuner<Umi, PtrNum, ZeroStrideOpt, Expands...>;
+    static const int shapes[][2] = {{0, 0}, {16, 16}, {8, 32}, {32, 8}, {4, 64}, {1, 256}, {8, 16}, {4, 32}};
+    TuneResult best;
+    for (int e = 0; e < (int)sizeof...(Expands); ++e)
+        for (const auto &sh : shapes) {
+            NDTuple<ParallelDim> pd = p_dims;
+            if (sh[0] && !engine::shape_pd(p_dims, sh[0], sh[1], pd)) continue;
+            const float us = engine::time_launches([&] { return T::run(e, stream, pd, a_dims, imems, omems, c); },
+                                                   stream, reps);
+            if (us < 0) continue;
+            ++best.candidates;
+            if (best.expand_index < 0 || us < best.us) {
+                best.expand_index = e;
+                best.expand = T::list[e];
+                best.p_dims = pd;
+                best.us = us;
+            }
+        }
+    return best;
+}
+
+// Runs the program with the mapping Autotune picked (same Expand list).
+template <class Umi, uint64_t PtrNum, uint64_t ZeroStrideOpt, uint64_t... Expands>
+umi_status ExecuteTuned(const TuneResult &t, umi_stream stream, const NDTuple<AccumulationDim> &a_dims,
+                        const array<UnrolledI<typename remove_cv<typename Umi::ElementType>::type>, Umi::TOTAL_IN> &imems,
+                        const array<UnrolledO<typename remove_cv<typename Umi::ElementType>::type>, Umi::TOTAL_OUT> &omems,
+                        const typename remove_cv<typename Umi::UserConstantType>::type &c =
+                            typename remove_cv<typename Umi::UserConstantType>::type()) {
+    if (t.expand_index < 0) return UMI_ERR_INVALID;
+    return Tuner<Umi, PtrNum, ZeroStrideOpt, Expands...>::run(t.expand_index, stream, t.p_dims, a_dims, imems, omems,
+                                                               c);
 }
 
 }  // namespace UMI
