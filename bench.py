@@ -311,6 +311,26 @@ def sec_stereo(st, g, cpu):
         "us": us, "vs_volume": ms * 1e3 / us, "gpx_s": 375 * 1242 / (us * 1e-6) / 1e9,
         "equals_volume_argmin": ok,
         "roofline": roofline("valu", ach, VALU_LANE_OPS * 4 / 1e12, "T abs-diff/s")}
+    # semi-global matching, 4 paths, P1 = 10, P2 = 120: the uint16 partial sums
+    # S [H][W][D] are written 3 times and read 3 times across the two launches
+    ws = DeviceBuffer(ops.sgm_workspace_bytes(375, 1242, 128))
+    sg = DeviceBuffer(375 * 1242 * 8)
+    us_sgm = timed(lambda: ops.sgm(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, 10, 120, ws, sg, stream=st),
+                   st, 20) * 1e3
+    sgm_bytes = 6 * 375 * 1242 * 128 * 2 + 2 * 375 * 1242 + 375 * 1242 * 8
+    out["sgm_1242x375_d128_5x5_4paths"] = {
+        "us": us_sgm, "gpx_s": 375 * 1242 / (us_sgm * 1e-6) / 1e9,
+        "roofline": roofline("hbm", sgm_bytes / (us_sgm * 1e-6) / 1e9, HBM_PEAK_GBS, "GB/s",
+                             algorithmic_bytes=sgm_bytes)}
+    if cpu:
+        try:
+            oracle.set_threads(cpu_info()["threads_used"])
+            reps, el = cpu_timed(lambda: oracle.sgm(L, R, 128, 0, -1, 2, 2, 10, 120), 1.0, 3)
+        finally:
+            oracle.set_threads(1)
+        out["sgm_1242x375_d128_5x5_4paths"]["cpu_baseline"] = {
+            "value": reps * 375 * 1242 / el, "unit": "px/s", "cores": cpu_info()["threads_used"], "kind": "port",
+            "sample": f"{reps} whole pairs (volume on all threads, the four paths on one), {el:.1f} s"}
     if cpu:
         rows = 8
         try:

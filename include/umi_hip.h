@@ -166,6 +166,30 @@ umi_status umi_stereo_wta_u8(const uint8_t *L, const uint8_t *R, int H, int W, i
                              int s0, int step, int KY, int KX, int32_t *out, umi_stream s);
 
 /* ---------------------------------------------------------------------------
+ * Semi-global matching over the same cost volume.  No reference counterpart:
+ * the aggregation step downstream of the volume that the MotionEst launch of
+ * main/cost_volume.cu:35-65 writes.  Four paths r (from the left, right, top
+ * and bottom), with C = cost[y][x][i] as in umi_cost_volume_u8:
+ *
+ *   L_r(p, i) = C(p, i) + min(L_r(q, i), L_r(q, i-1) + P1, L_r(q, i+1) + P1,
+ *                            M_r(q) + P2) - M_r(q),   q = p - r,
+ *   M_r(q) = min_k L_r(q, k); L_r = C on the first pixel of a path;
+ *   neighbours outside [0, D) are left out;  S = sum over the four paths;
+ *   out[y][x][0] = the smallest i with S minimal, out[y][x][1] = that S.
+ *
+ * The volume is recomputed from the pair, never stored; workspace holds the
+ * partial sums (uint16 [H][W][D], umi_sgm_workspace_size bytes, no initial
+ * contents needed).  Two launches on stream s.  Integer arithmetic, exact.
+ * Limits: 0 <= KX, KY <= 3; step = +1 or -1; 1 <= D <= 128;
+ * 0 <= P1 <= P2; 4 * (255 (2KY+1)(2KX+1) + P2) < 65536;
+ * 2 (4 + 2KY) W <= 160 KiB (W <= 8192 at KY = 3).
+ * ------------------------------------------------------------------------- */
+umi_status umi_sgm_workspace_size(int H, int W, int D, size_t *bytes);
+umi_status umi_sgm_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step,
+                      int KY, int KX, int P1, int P2, void *workspace, size_t workspace_bytes, int32_t *out,
+                      umi_stream s);
+
+/* ---------------------------------------------------------------------------
  * 3x3 filters.  Pitches are in elements.  mode: 0 = PAD (out-of-image taps
  * read `pad`, the reference's PAD boundary), 1 = clamp to edge (its WRAP).
  * Taps are accumulated dy outer, dx inner, starting from `init`; the weighted

@@ -33,6 +33,7 @@ _SIGS = {
     "oracle_iimg_i32": [_P, _I, _I, _I, _P],
     "oracle_iimg_f64": [_P, _I, _I, _I, _P],
     "oracle_set_threads": [_I],
+    "oracle_sgm_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
     "oracle_gemm_f32_rows": [_P, _P, _P, _I, _I, _I, _I, _I],
     "oracle_conv_f32": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I],
 }
@@ -125,6 +126,17 @@ def stereo_wta(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     y0, y1 = y_range if y_range is not None else (0, H)
     out = np.zeros((H, W, 2), np.int32)
     lib().oracle_stereo_wta_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
+    return out
+
+
+def sgm(L, R, D, s0=0, step=-1, KY=2, KX=2, P1=10, P2=120):
+    """Semi-global matching over the cost volume (four paths): per pixel (first
+    index with the least aggregated cost, that cost), int32 [H][W][2]."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    out = np.zeros((H, W, 2), np.int32)
+    lib().oracle_sgm_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, P1, P2, _ptr(out))
     return out
 
 

@@ -467,3 +467,66 @@ void oracle_conv_f32(const float *x, const float *w, float *out, int N, int C, i
                     out[(((size_t)n * K + k) * OH + oy) * OW + ox] = acc;
                 }
 }
+
+/*
+ * Semi-global matching over the cost volume of oracle_cost_volume_u8 (the
+ * volume main/cost_volume.cu:35-65 computes, generalised as above).  No
+ * reference loop computes it: it is the aggregation step downstream of that
+ * volume.  Four paths r (from the left, right, top, bottom):
+ *   L_r(p, i) = C(p, i) + min(L_r(q, i), L_r(q, i-1) + P1, L_r(q, i+1) + P1,
+ *                            M_r(q) + P2) - M_r(q),   q = p - r,
+ *   M_r(q) = min_k L_r(q, k); the first pixel of a path has L_r = C;
+ *   neighbours outside [0, D) are left out.  S = sum of the four L_r; out =
+ *   (first i with S minimal, that S).  Integer arithmetic, exact.
+ */
+static void sgm_path(const int32_t *C, int H, int W, int D, int P1, int P2, int dy, int dx, int64_t *S) {
+    int32_t *prev = (int32_t *)malloc(sizeof(int32_t) * D), *cur = (int32_t *)malloc(sizeof(int32_t) * D);
+    /* walk each path line: rows for horizontal paths, columns for vertical */
+    const int nlines = dy ? W : H, len = dy ? H : W;
+    for (int l = 0; l < nlines; ++l) {
+        for (int t = 0; t < len; ++t) {
+            const int s = (dx > 0 || dy > 0) ? t : len - 1 - t;
+            const int y = dy ? s : l, x = dy ? l : s;
+            const int32_t *c = C + ((size_t)y * W + x) * D;
+            if (t == 0) {
+                for (int i = 0; i < D; ++i) cur[i] = c[i];
+            } else {
+                int32_t m = prev[0];
+                for (int i = 1; i < D; ++i) m = prev[i] < m ? prev[i] : m;
+                for (int i = 0; i < D; ++i) {
+                    int32_t best = prev[i];
+                    if (i > 0 && prev[i - 1] + P1 < best) best = prev[i - 1] + P1;
+                    if (i + 1 < D && prev[i + 1] + P1 < best) best = prev[i + 1] + P1;
+                    if (m + P2 < best) best = m + P2;
+                    cur[i] = c[i] + best - m;
+                }
+            }
+            for (int i = 0; i < D; ++i) S[((size_t)y * W + x) * D + i] += cur[i];
+            int32_t *tmp = prev;
+            prev = cur;
+            cur = tmp;
+        }
+    }
+    free(prev);
+    free(cur);
+}
+
+void oracle_sgm_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step, int KY,
+                   int KX, int P1, int P2, int32_t *out) {
+    int32_t *C = (int32_t *)malloc(sizeof(int32_t) * (size_t)H * W * D);
+    int64_t *S = (int64_t *)calloc((size_t)H * W * D, sizeof(int64_t));
+    oracle_cost_volume_u8(L, R, H, W, pitch, D, s0, step, KY, KX, 0, H, C);
+    sgm_path(C, H, W, D, P1, P2, 0, 1, S);
+    sgm_path(C, H, W, D, P1, P2, 0, -1, S);
+    sgm_path(C, H, W, D, P1, P2, 1, 0, S);
+    sgm_path(C, H, W, D, P1, P2, -1, 0, S);
+    for (size_t p = 0; p < (size_t)H * W; ++p) {
+        int best = 0;
+        for (int i = 1; i < D; ++i)
+            if (S[p * D + i] < S[p * D + best]) best = i;
+        out[2 * p] = best;
+        out[2 * p + 1] = (int32_t)S[p * D + best];
+    }
+    free(C);
+    free(S);
+}

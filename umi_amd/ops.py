@@ -7,6 +7,8 @@ the tests use.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from ._lib import call
@@ -107,6 +109,36 @@ def stereo_wta_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int =
     dl, dr = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R)
     out = DeviceBuffer(H * W * 8)
     stereo_wta(dl, dr, H, W, W, D, s0, step, KY, KX, out)
+    return out.download((H, W, 2), np.int32)
+
+
+def sgm_workspace_bytes(H: int, W: int, D: int) -> int:
+    n = ctypes.c_size_t()
+    call("umi_sgm_workspace_size", H, W, D, ctypes.byref(n))
+    return n.value
+
+
+def sgm(L: DeviceBuffer, R: DeviceBuffer, H: int, W: int, pitch: int, D: int, s0: int, step: int, KY: int,
+        KX: int, P1: int, P2: int, workspace: DeviceBuffer, out: DeviceBuffer, stream=None) -> None:
+    """Semi-global matching (four paths, penalties P1/P2) over the cost volume
+    of `cost_volume`, without storing it: out int32 [H][W][2] = (i, aggregated
+    cost) with i the first minimum; workspace of `sgm_workspace_bytes` bytes."""
+    if out.nbytes < H * W * 8:
+        raise ValueError("SGM output buffer too small")
+    if L.nbytes < (H - 1) * pitch + W or R.nbytes < (H - 1) * pitch + W:
+        raise ValueError("image buffer too small")
+    call("umi_sgm_u8", L.ptr, R.ptr, H, W, pitch, D, s0, step, KY, KX, P1, P2, workspace.ptr, workspace.nbytes,
+         out.ptr, _stream(stream))
+
+
+def sgm_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int = -1, KY: int = 2, KX: int = 2,
+           P1: int = 10, P2: int = 120) -> np.ndarray:
+    assert L.dtype == np.uint8 and R.dtype == np.uint8 and L.shape == R.shape
+    H, W = L.shape
+    dl, dr = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R)
+    ws = DeviceBuffer(max(sgm_workspace_bytes(H, W, D), 4))
+    out = DeviceBuffer(H * W * 8)
+    sgm(dl, dr, H, W, W, D, s0, step, KY, KX, P1, P2, ws, out)
     return out.download((H, W, 2), np.int32)
 
 
