@@ -313,7 +313,7 @@ def sec_stereo(st, g, cpu):
         "roofline": roofline("valu", ach, VALU_LANE_OPS * 4 / 1e12, "T abs-diff/s")}
     # semi-global matching, 4 paths, P1 = 10, P2 = 120: the uint16 partial sums
     # S [H][W][D] are written 3 times and read 3 times across the two launches
-    ws = DeviceBuffer(ops.sgm_workspace_bytes(375, 1242, 128))
+    ws = DeviceBuffer(ops.sgm_workspace_bytes(375, 1242, 128, 0, -1, 2, 2))
     sg = DeviceBuffer(375 * 1242 * 8)
     us_sgm = timed(lambda: ops.sgm(dl, dr, 375, 1242, 1242, 128, 0, -1, 2, 2, 10, 120, ws, sg, stream=st),
                    st, 20) * 1e3

@@ -178,13 +178,15 @@ umi_status umi_stereo_wta_u8(const uint8_t *L, const uint8_t *R, int H, int W, i
  *   out[y][x][0] = the smallest i with S minimal, out[y][x][1] = that S.
  *
  * The volume is recomputed from the pair, never stored; workspace holds the
- * partial sums (uint16 [H][W][D], umi_sgm_workspace_size bytes, no initial
- * contents needed).  Two launches on stream s.  Integer arithmetic, exact.
+ * partial sums (uint16 per pixel and disparity) and packed copies of the
+ * pair: umi_sgm_workspace_size bytes for the same H, W, D, s0, step, KY, KX,
+ * no initial contents needed.  Three launches on stream s (pack, the four
+ * paths, sum and argmin).  Integer arithmetic, exact.
  * Limits: 0 <= KX, KY <= 3; step = +1 or -1; 1 <= D <= 128;
  * 0 <= P1 <= P2; 4 * (255 (2KY+1)(2KX+1) + P2) < 65536;
  * 2 (4 + 2KY) W <= 160 KiB (W <= 8192 at KY = 3).
  * ------------------------------------------------------------------------- */
-umi_status umi_sgm_workspace_size(int H, int W, int D, size_t *bytes);
+umi_status umi_sgm_workspace_size(int H, int W, int D, int s0, int step, int KY, int KX, size_t *bytes);
 umi_status umi_sgm_u8(const uint8_t *L, const uint8_t *R, int H, int W, int pitch, int D, int s0, int step,
                       int KY, int KX, int P1, int P2, void *workspace, size_t workspace_bytes, int32_t *out,
                       umi_stream s);

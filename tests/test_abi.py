@@ -183,10 +183,14 @@ def test_sgm_validates_on_host():
 
     L = lib()
     ws_n = ctypes.c_size_t()
-    assert L.umi_sgm_workspace_size(10, 20, 30, ctypes.byref(ws_n)) == 0 and ws_n.value == 10 * 20 * 30 * 2
-    keep, (l, r, ws, out) = _buffers(16 * 16, 16 * 16, 16 * 16 * 128 * 2, 16 * 16 * 8)
+    assert L.umi_sgm_workspace_size(10, 20, 30, 0, -1, 2, 2, ctypes.byref(ws_n)) == 0
+    assert ws_n.value >= 10 * 20 * 30 * 2  # at least the uint16 partial sums
+    assert L.umi_sgm_workspace_size(10, 20, 30, 0, 3, 2, 2, ctypes.byref(ws_n)) == 1  # step 3
+    assert L.umi_sgm_workspace_size(16, 16, 8, 0, -1, 2, 2, ctypes.byref(ws_n)) == 0
+    need = ws_n.value
+    keep, (l, r, ws, out) = _buffers(16 * 16, 16 * 16, need, 16 * 16 * 8)
     ok = (l, r, 16, 16, 16, 8, 0, -1, 2, 2)
-    big = 16 * 16 * 128 * 2
+    big = need
     assert L.umi_sgm_u8(*ok, 10, 120, None, big, out, None) == 1  # no workspace
     assert L.umi_sgm_u8(l, r, 16, 16, 16, 129, 0, -1, 2, 2, 10, 120, ws, big, out, None) == 3  # D > 128
     assert L.umi_sgm_u8(l, r, 16, 16, 16, 8, 0, 2, 2, 2, 10, 120, ws, big, out, None) == 1  # step 2
@@ -194,5 +198,5 @@ def test_sgm_validates_on_host():
     assert L.umi_sgm_u8(*ok, 130, 120, ws, big, out, None) == 1  # P1 > P2
     assert L.umi_sgm_u8(*ok, 10, 12000, ws, big, out, None) == 1  # partial sums would overflow uint16
     assert b"65536" in L.umi_last_error()
-    assert L.umi_sgm_u8(*ok, 10, 120, ws, 16 * 16 * 8 * 2 - 2, out, None) == 1  # workspace one element short
+    assert L.umi_sgm_u8(*ok, 10, 120, ws, need - 1, out, None) == 1  # workspace one byte short
     assert b"workspace" in L.umi_last_error()

@@ -71,7 +71,7 @@ def test_sgm_pitched_rows_and_reused_workspace(gpu):
     H, W, pitch, D = 70, 100, 128, 50
     L, R = _pair(H, pitch, 77)
     dl, dr = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R)
-    ws = DeviceBuffer(ops.sgm_workspace_bytes(H, W, D))
+    ws = DeviceBuffer(ops.sgm_workspace_bytes(H, W, D, 0, -1, 2, 2))
     ws.fill(0xAB)
     out = DeviceBuffer(H * W * 8)
     ref = oracle.sgm(np.ascontiguousarray(L[:, :W]), np.ascontiguousarray(R[:, :W]), D, 0, -1, 2, 2, 10, 120)
@@ -86,3 +86,12 @@ def test_sgm_kitti_size_vs_oracle(gpu):
     print("[sgm] 375x1242 D=128", flush=True)
     out = ops.sgm_np(L, R, 128, 0, -1, 2, 2, 10, 120)
     np.testing.assert_array_equal(out, oracle.sgm(L, R, 128, 0, -1, 2, 2, 10, 120))
+
+
+@pytest.mark.parametrize("s0,step,D", [(25, 1, 40), (-200, 1, 16), (300, -1, 20), (-5, -1, 64), (3, 1, 33)])
+def test_sgm_shifts_past_the_image(gpu, s0, step, D):
+    # shifts beyond the image width read clamped columns only
+    L, R = _pair(21, 30, 600 + D)
+    print(f"[sgm] s0={s0} step={step} D={D}", flush=True)
+    out = ops.sgm_np(L, R, D, s0, step, 1, 2, 6, 70)
+    np.testing.assert_array_equal(out, oracle.sgm(L, R, D, s0, step, 1, 2, 6, 70))

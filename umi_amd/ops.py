@@ -112,9 +112,9 @@ def stereo_wta_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int =
     return out.download((H, W, 2), np.int32)
 
 
-def sgm_workspace_bytes(H: int, W: int, D: int) -> int:
+def sgm_workspace_bytes(H: int, W: int, D: int, s0: int = 0, step: int = -1, KY: int = 2, KX: int = 2) -> int:
     n = ctypes.c_size_t()
-    call("umi_sgm_workspace_size", H, W, D, ctypes.byref(n))
+    call("umi_sgm_workspace_size", H, W, D, s0, step, KY, KX, ctypes.byref(n))
     return n.value
 
 
@@ -136,7 +136,7 @@ def sgm_np(L: np.ndarray, R: np.ndarray, D: int, s0: int = 0, step: int = -1, KY
     assert L.dtype == np.uint8 and R.dtype == np.uint8 and L.shape == R.shape
     H, W = L.shape
     dl, dr = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R)
-    ws = DeviceBuffer(max(sgm_workspace_bytes(H, W, D), 4))
+    ws = DeviceBuffer(sgm_workspace_bytes(H, W, D, s0, step, KY, KX))
     out = DeviceBuffer(H * W * 8)
     sgm(dl, dr, H, W, W, D, s0, step, KY, KX, P1, P2, ws, out)
     return out.download((H, W, 2), np.int32)
