@@ -192,15 +192,17 @@ struct LineWalk {
         }
     }
     __device__ int pos(int s) const { return DIR > 0 ? s : len - 1 - s; }
-    // the L word is the same for every lane (lane offset 0)
-    __device__ uint2 lw(int q) const {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rL, 0, lq0 + q * rqs, 0);
+    // The L word is the same for every lane (lane offset 0).  imm: a constant
+    // byte offset, which the compiler folds into the instruction.
+    __device__ uint2 lw(int q, int imm = 0) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rL, imm, lq0 + q * RQS(), 0);
         return make_uint2(v[0], v[1]);
     }
-    __device__ uint2 rw(int q, int k) const {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rR, (int)roffb[k], rq0 + q * rqs, 0);
+    __device__ uint2 rw(int q, int k, int imm = 0) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rR, (int)roffb[k] + imm, rq0 + q * RQS(), 0);
         return make_uint2(v[0], v[1]);
     }
+    __device__ int RQS() const { return VERT ? rqs : 8; }
     __device__ uint32_t pload(int p) const { return __builtin_amdgcn_raw_buffer_load_b32(rP, (int)laneb, pq0 + p * pqs, 0); }
     __device__ void pstore(int p, uint32_t v) const { __builtin_amdgcn_raw_buffer_store_b32(v, rP, (int)laneb, pq0 + p * pqs, 0); }
     __device__ uint32_t win(int q, int k) const { return win_sad(lw(q), rw(q, k)); }
@@ -220,8 +222,10 @@ struct LineWalk {
 
     template <int MODE>
     __device__ void emit(int p, uint32_t pv) {
-        const uint32_t packed = (Lp[0] & 0xffffu) | (Lp[1] << 16);
-        // no carry between the halves: each stays below 2^16
+        // Lp[0] < 2^16 wherever it is stored; an invalid Lp[1] (D odd) only
+        // fills the unused upper half.  No carry between the halves: each
+        // stays below 2^16
+        const uint32_t packed = Lp[0] | (Lp[1] << 16);
         if (valid[0]) pstore(p, MODE == SGM_ADD ? pv + packed : packed);
     }
 
@@ -248,19 +252,34 @@ struct LineWalk {
         const int qa = DIR * KA, qs = -DIR * (KA + 1);  // entering / leaving window offsets
         uint2 la[SGM_U], ls[SGM_U], ra[SGM_U][2], rs[SGM_U][2];
         uint32_t pv[SGM_U];
+        // L and R words and partial sums of steps s .. s + SGM_U - 1.  Rows:
+        // one scalar offset per block and array, the step in the
+        // instruction's immediate (from the lowest address of the block up);
+        // columns: a scalar offset per load.
         auto issue = [&](int s) {
-            const int p0 = pos(s);
+            const int p0 = pos(s), plo = DIR > 0 ? p0 : p0 - (SGM_U - 1);
 #pragma unroll
             for (int u = 0; u < SGM_U; ++u) {
-                const int p = p0 + DIR * u;
-                la[u] = lw(p + qa);
-                ls[u] = lw(p + qs);
+                if (VERT) {
+                    const int p = p0 + DIR * u;
+                    la[u] = lw(p + qa);
+                    ls[u] = lw(p + qs);
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    ra[u][k] = rw(p + qa, k);
-                    rs[u][k] = rw(p + qs, k);
+                    for (int k = 0; k < 2; ++k) {
+                        ra[u][k] = rw(p + qa, k);
+                        rs[u][k] = rw(p + qs, k);
+                    }
+                } else {
+                    const int imm = 8 * (DIR > 0 ? u : SGM_U - 1 - u);
+                    la[u] = lw(plo + qa, imm);
+                    ls[u] = lw(plo + qs, imm);
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        ra[u][k] = rw(plo + qa, k, imm);
+                        rs[u][k] = rw(plo + qs, k, imm);
+                    }
                 }
-                if (NEEDP) pv[u] = pload(p);
+                if (NEEDP) pv[u] = pload(p0 + DIR * u);
             }
         };
         auto block = [&](int s, int n) {  // steps s .. s + n - 1 from the loaded block
@@ -274,10 +293,11 @@ struct LineWalk {
             if (n == SGM_U) {
                 // The next block's loads reuse this block's registers: its
                 // start position passes through empty asm statements that
-                // consume the differences, so no load is placed before them.
+                // consume the differences and partial sums, so no load is
+                // placed before them.
                 int sn = s + SGM_U;
 #pragma unroll
-                for (int u = 0; u < SGM_U; ++u) asm volatile("" : "+s"(sn) : "v"(dc[u][0]), "v"(dc[u][1]));
+                for (int u = 0; u < SGM_U; ++u) asm volatile("" : "+s"(sn) : "v"(dc[u][0]), "v"(dc[u][1]), "v"(pc[u]));
                 issue(sn);  // in-bounds within the slack
             }
             const int p0 = pos(s);
