@@ -194,9 +194,21 @@ def sec_subpel(st, g, cpu):
     t_sp = timed(lambda: ops.me_subpel_refine(dc, dr, 1080, 1920, 1920, mvi, mvq, stream=st), st, 50)
     q = mvq.download((135, 240, 3), np.int32)
     i = mvi.download((135, 240, 3), np.int32)
-    return {"us": t_sp * 1e3, "full_search_us": t_fs * 1e3, "blocks_per_s": 32400 / (t_sp * 1e-3),
-            "mean_sad_integer": float(i[..., 2].mean()), "mean_sad_quarter_pel": float(q[..., 2].mean()),
-            "blocks_moved": float(np.mean((q[..., 0] != 4 * i[..., 0]) | (q[..., 1] != 4 * i[..., 1])))}
+    out = {"us": t_sp * 1e3, "full_search_us": t_fs * 1e3, "blocks_per_s": 32400 / (t_sp * 1e-3),
+           "mean_sad_integer": float(i[..., 2].mean()), "mean_sad_quarter_pel": float(q[..., 2].mean()),
+           "blocks_moved": float(np.mean((q[..., 0] != 4 * i[..., 0]) | (q[..., 1] != 4 * i[..., 1])))}
+    # NV12 1080p: luma + chroma full search (+-16) and the FFmpeg-record export
+    cy, cuv, ry, ruv, _ = workloads.nv12_pair(1080, 1920, seed=2)
+    planes = [DeviceBuffer.from_array(a) for a in (cy, cuv, ry, ruv)]
+    mvn, rec = DeviceBuffer(32400 * 12), DeviceBuffer(32400 * 40)
+    t_nv = timed(lambda: ops.me_full_search_nv12(*planes, 1080, 1920, 1920, 1920, 16, mvn, stream=st), st, 50)
+    t_ex = timed(lambda: ops.export_mvs(mvq, 135, 240, 8, 4, ops.MV_AVMOTIONVECTOR, rec, stream=st), st, 50)
+    absdiff = 32400 * 1089 * 96
+    out["nv12_full_search_1080p_r16"] = {
+        "us": t_nv * 1e3, "blocks_per_s": 32400 / (t_nv * 1e-3), "vs_luma_only": t_nv / t_fs,
+        "roofline": roofline("valu", absdiff / (t_nv * 1e-3) / 1e12, VALU_LANE_OPS * 4 / 1e12, "T abs-diff/s")}
+    out["mv_export_avmotionvector_us"] = t_ex * 1e3
+    return out
 
 
 def sec_gemm(st, g, cpu):

@@ -422,6 +422,39 @@ umi_status umi_batch_box_sum_u32(umi_batch b, const uint32_t *iimg, int ry, int 
 umi_status umi_me_subpel_refine_u8(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch,
                                    const int32_t *mv_in, int32_t *mv_out, umi_stream s);
 
+/* ---------------------------------------------------------------------------
+ * Full-search ME over NV12 frames (luma plane + interleaved UV plane, 4:2:0).
+ * No reference counterpart: it extends the MotionEst search of
+ * main/full_me.cu:34-64 (clamp boundary, first minimum in raster order) to
+ * the chroma of a video frame.  For each 8x8 luma block (nby = H/8,
+ * nbx = W/8) and integer vector (dy, dx) in [-R, R]^2:
+ *   cost = SAD_Y(8x8) + SAD_U(4x4) + SAD_V(4x4),
+ * the chroma predicted as in H.264 8.4.2.2.2 from the vector (4dy, 4dx) in
+ * quarter-luma units: integer offset floor(d/2), eighth-pel fraction
+ * 4 (d & 1); samples clamped to the (H/2) x (W/2) chroma plane; U = byte 2x,
+ * V = byte 2x + 1 of a uv row.
+ * mv_cost: int32 [nby][nbx][3] = (dy, dx, cost).  H, W even and >= 8;
+ * pitch_y, pitch_uv >= W bytes; 1 <= R <= 16.
+ * ------------------------------------------------------------------------- */
+umi_status umi_me_full_search_nv12(const uint8_t *cur_y, const uint8_t *cur_uv, const uint8_t *ref_y,
+                                   const uint8_t *ref_uv, int H, int W, int pitch_y, int pitch_uv, int R,
+                                   int32_t *mv_cost, umi_stream s);
+
+/* ---------------------------------------------------------------------------
+ * Motion vectors in formats a video codec or filter graph consumes.  No
+ * reference counterpart.  mv: int32 [nby][nbx][3] (dy, dx, cost) from the
+ * searches above, in pels (units = 1) or quarter pels (units = 4, from
+ * umi_me_subpel_refine_u8).  B: block size.
+ *   format 0: int16 [nby][nbx][2] = (mvx, mvy) in quarter pels, the
+ *             motion-hint layout of hardware encoders;
+ *   format 1: FFmpeg AVMotionVector records (libavutil/motion_vector.h,
+ *             40 bytes each, one per block in raster order): source = -1,
+ *             w = h = B, dst = block centre, motion = (mvx, mvy) with
+ *             motion_scale = 4, src = dst + (motion >> 2), flags = 0.
+ * ------------------------------------------------------------------------- */
+umi_status umi_me_export_mvs(const int32_t *mv, int nby, int nbx, int B, int units, int format, void *out,
+                             umi_stream s);
+
 /* Full search of the listed 8x8/16x16 blocks only (block index by*nbx+bx in
  * list[0 .. *count)), writing their (dy, dx, SAD) into mv_sad; semantics of
  * umi_me_full_search_u8.  max_count bounds *count. */

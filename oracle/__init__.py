@@ -34,6 +34,8 @@ _SIGS = {
     "oracle_iimg_f64": [_P, _I, _I, _I, _P],
     "oracle_set_threads": [_I],
     "oracle_sgm_u8": [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "oracle_me_nv12_u8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
+    "oracle_nv12_chroma_pred": [_P, _I, _I, _I, _I, _I, _I, _I, _I],
     "oracle_gemm_f32_rows": [_P, _P, _P, _I, _I, _I, _I, _I],
     "oracle_conv_f32": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I],
 }
@@ -127,6 +129,27 @@ def stereo_wta(L, R, D, s0=0, step=-1, KY=2, KX=2, y_range=None):
     out = np.zeros((H, W, 2), np.int32)
     lib().oracle_stereo_wta_u8(_ptr(L), _ptr(R), H, W, W, D, s0, step, KY, KX, y0, y1, _ptr(out))
     return out
+
+
+def me_nv12(cur_y, cur_uv, ref_y, ref_uv, R=16):
+    """Full-search ME over NV12 frames (luma 8x8 + chroma 4x4 U and V, H.264
+    chroma interpolation at half-pel): int32 [H/8][W/8][3] = (dy, dx, cost)."""
+    cur_y, ref_y = (np.ascontiguousarray(a, np.uint8) for a in (cur_y, ref_y))
+    cur_uv, ref_uv = (np.ascontiguousarray(a, np.uint8) for a in (cur_uv, ref_uv))
+    H, W = cur_y.shape
+    assert cur_uv.shape == (H // 2, W) and ref_uv.shape == cur_uv.shape and ref_y.shape == cur_y.shape
+    out = np.zeros((H // 8, W // 8, 3), np.int32)
+    lib().oracle_me_nv12_u8(_ptr(cur_y), _ptr(cur_uv), _ptr(ref_y), _ptr(ref_uv), H, W, W, W, R, _ptr(out))
+    return out
+
+
+def nv12_chroma_pred(uv, ch, Y, X, fy, fx):
+    """The chroma predictor of me_nv12: eighth-pel fractions fy, fx at integer
+    chroma position (Y, X) of plane ch (0 = U, 1 = V) of an NV12 uv plane."""
+    uv = np.ascontiguousarray(uv, np.uint8)
+    f = lib().oracle_nv12_chroma_pred
+    f.restype = C.c_int
+    return f(_ptr(uv), uv.shape[0], uv.shape[1] // 2, uv.shape[1], ch, Y, X, fy, fx)
 
 
 def sgm(L, R, D, s0=0, step=-1, KY=2, KX=2, P1=10, P2=120):

@@ -469,6 +469,68 @@ void oracle_conv_f32(const float *x, const float *w, float *out, int N, int C, i
 }
 
 /*
+ * Full-search ME over NV12 frames: luma SAD of the 8x8 block plus the SAD of
+ * its 4x4 U and V blocks.  No reference loop computes it (main/full_me.cu is
+ * luma-only); it extends that search to the chroma planes of a 4:2:0 frame.
+ * The chroma predictor of an integer luma vector (dy, dx) follows H.264
+ * 8.4.2.2.2 (chroma sample interpolation) with the vector in quarter-luma
+ * units (4dy, 4dx): chroma offset (4d) >> 3 = floor(d / 2), eighth-pel
+ * fraction f = (4d) & 7 in {0, 4},
+ *   pred = ((8-fx)(8-fy) A + fx (8-fy) B + (8-fx) fy C + fx fy D + 32) >> 6,
+ * A..D = ref chroma at (Y,X), (Y,X+1), (Y+1,X), (Y+1,X+1), clamped to the
+ * chroma plane ((H/2) x (W/2) samples, U at byte 2x and V at byte 2x + 1 of
+ * a uv row).  cost = SAD_Y + SAD_U + SAD_V; out int32 [H/8][W/8][3] = (dy,
+ * dx, cost) of the first minimum in raster order over (dy, dx) in [-R, R]^2.
+ */
+int oracle_nv12_chroma_pred(const uint8_t *uv, int HC, int WC, int pitch, int ch, int Y, int X, int fy, int fx) {
+    const int y0 = clampi(Y, 0, HC - 1), y1 = clampi(Y + 1, 0, HC - 1);
+    const int x0 = clampi(X, 0, WC - 1), x1 = clampi(X + 1, 0, WC - 1);
+    const int A = uv[(size_t)y0 * pitch + 2 * x0 + ch], B = uv[(size_t)y0 * pitch + 2 * x1 + ch];
+    const int C = uv[(size_t)y1 * pitch + 2 * x0 + ch], D = uv[(size_t)y1 * pitch + 2 * x1 + ch];
+    return ((8 - fx) * (8 - fy) * A + fx * (8 - fy) * B + (8 - fx) * fy * C + fx * fy * D + 32) >> 6;
+}
+
+void oracle_me_nv12_u8(const uint8_t *cy, const uint8_t *cuv, const uint8_t *ry, const uint8_t *ruv, int H, int W,
+                       int pitch_y, int pitch_uv, int R, int32_t *out) {
+    const int nby = H / 8, nbx = W / 8, HC = H / 2, WC = W / 2, S = 2 * R + 1;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+    for (int by = 0; by < nby; ++by)
+        for (int bx = 0; bx < nbx; ++bx) {
+            int32_t best = INT32_MAX, bdy = 0, bdx = 0;
+            for (int sy = 0; sy < S; ++sy)
+                for (int sx = 0; sx < S; ++sx) {
+                    const int dy = sy - R, dx = sx - R;
+                    int32_t sum = 0;
+                    for (int py = 0; py < 8; ++py)
+                        for (int px = 0; px < 8; ++px) {
+                            const int c = cy[(size_t)(8 * by + py) * pitch_y + 8 * bx + px];
+                            const int r = ry[(size_t)clampi(8 * by + py + dy, 0, H - 1) * pitch_y +
+                                             clampi(8 * bx + px + dx, 0, W - 1)];
+                            sum += abs(c - r);
+                        }
+                    const int oy = (4 * dy) >> 3, fy = (4 * dy) & 7, ox = (4 * dx) >> 3, fx = (4 * dx) & 7;
+                    for (int ch = 0; ch < 2; ++ch)
+                        for (int j = 0; j < 4; ++j)
+                            for (int i = 0; i < 4; ++i) {
+                                const int c = cuv[(size_t)(4 * by + j) * pitch_uv + 2 * (4 * bx + i) + ch];
+                                const int p = oracle_nv12_chroma_pred(ruv, HC, WC, pitch_uv, ch, 4 * by + j + oy,
+                                                               4 * bx + i + ox, fy, fx);
+                                sum += abs(c - p);
+                            }
+                    if (sum < best) {
+                        best = sum;
+                        bdy = dy;
+                        bdx = dx;
+                    }
+                }
+            int32_t *o = out + ((size_t)by * nbx + bx) * 3;
+            o[0] = bdy;
+            o[1] = bdx;
+            o[2] = best;
+        }
+}
+
+/*
  * Semi-global matching over the cost volume of oracle_cost_volume_u8 (the
  * volume main/cost_volume.cu:35-65 computes, generalised as above).  No
  * reference loop computes it: it is the aggregation step downstream of that

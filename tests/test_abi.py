@@ -200,3 +200,18 @@ def test_sgm_validates_on_host():
     assert b"65536" in L.umi_last_error()
     assert L.umi_sgm_u8(*ok, 10, 120, ws, need - 1, out, None) == 1  # workspace one byte short
     assert b"workspace" in L.umi_last_error()
+
+
+def test_nv12_and_export_validate_on_host():
+    from umi_amd._lib import lib
+
+    L = lib()
+    keep, (y, uv, mv, out) = _buffers(64 * 64, 32 * 64, 64 * 12, 64 * 40)
+    assert L.umi_me_full_search_nv12(y, uv, y, uv, 64, 64, 64, 64, 17, mv, None) == 3  # R > 16
+    assert L.umi_me_full_search_nv12(y, uv, y, uv, 63, 64, 64, 64, 8, mv, None) == 1  # odd height
+    assert L.umi_me_full_search_nv12(y, uv, y, uv, 64, 64, 64, 32, 8, mv, None) == 1  # uv pitch < W
+    assert L.umi_me_full_search_nv12(y, None, y, uv, 64, 64, 64, 64, 8, mv, None) == 1
+    assert L.umi_me_export_mvs(mv, 8, 8, 8, 2, 0, out, None) == 1  # units 2
+    assert L.umi_me_export_mvs(mv, 8, 8, 8, 1, 2, out, None) == 1  # format 2
+    assert L.umi_me_export_mvs(mv, 8, 8, 0, 1, 0, out, None) == 1  # block 0
+    assert L.umi_me_export_mvs(mv, 4096, 8, 16, 1, 1, out, None) == 3  # positions past int16

@@ -49,6 +49,8 @@ SIGNATURES = {
     "umi_me_full_search_u8": [P, P, c_int, c_int, c_int, c_int, c_int, P, P, P],
     "umi_cost_volume_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
     "umi_stereo_wta_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P],
+    "umi_me_full_search_nv12": [P, P, P, P, c_int, c_int, c_int, c_int, c_int, P, P],
+    "umi_me_export_mvs": [P, c_int, c_int, c_int, c_int, c_int, P, P],
     "umi_sgm_workspace_size": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P],
     "umi_sgm_u8": [P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, c_size_t, P, P],
     "umi_box_sum_iimg_u32": [P, c_int, c_int, c_int, c_int, c_int, P, c_int, P],

@@ -69,6 +69,69 @@ def me_subpel_refine_np(cur: np.ndarray, ref: np.ndarray, mv: np.ndarray) -> np.
 
 
 # ---------------------------------------------------------------------------
+# NV12 frames: full search over luma + chroma, and codec-facing MV formats
+# ---------------------------------------------------------------------------
+MV_Q4_INT16, MV_AVMOTIONVECTOR = 0, 1
+
+# FFmpeg's AVMotionVector (libavutil/motion_vector.h), 40 bytes per record
+AVMOTIONVECTOR_DTYPE = np.dtype({
+    "names": ["source", "w", "h", "src_x", "src_y", "dst_x", "dst_y", "flags", "motion_x", "motion_y",
+              "motion_scale"],
+    "formats": [np.int32, np.uint8, np.uint8, np.int16, np.int16, np.int16, np.int16, np.uint64, np.int32, np.int32,
+                np.uint16],
+    "offsets": [0, 4, 5, 6, 8, 10, 12, 16, 24, 28, 32],
+    "itemsize": 40})
+
+
+def me_full_search_nv12(cur_y: DeviceBuffer, cur_uv: DeviceBuffer, ref_y: DeviceBuffer, ref_uv: DeviceBuffer,
+                        H: int, W: int, pitch_y: int, pitch_uv: int, R: int, mv_cost: DeviceBuffer,
+                        stream=None) -> None:
+    """Full search over NV12 frames: cost = luma 8x8 SAD + chroma 4x4 U and V
+    SAD (H.264 chroma interpolation at half-pel); mv_cost int32 [H/8][W/8][3]."""
+    if mv_cost.nbytes < (H // 8) * (W // 8) * 12:
+        raise ValueError("mv_cost buffer too small")
+    for b, rows, pitch in ((cur_y, H, pitch_y), (ref_y, H, pitch_y), (cur_uv, H // 2, pitch_uv),
+                           (ref_uv, H // 2, pitch_uv)):
+        if b.nbytes < (rows - 1) * pitch + W:
+            raise ValueError("plane buffer too small")
+    call("umi_me_full_search_nv12", cur_y.ptr, cur_uv.ptr, ref_y.ptr, ref_uv.ptr, H, W, pitch_y, pitch_uv, R,
+         mv_cost.ptr, _stream(stream))
+
+
+def me_full_search_nv12_np(cur_y: np.ndarray, cur_uv: np.ndarray, ref_y: np.ndarray, ref_uv: np.ndarray,
+                           R: int = 16) -> np.ndarray:
+    H, W = cur_y.shape
+    assert cur_uv.shape == (H // 2, W) and ref_uv.shape == cur_uv.shape and ref_y.shape == cur_y.shape
+    bufs = [DeviceBuffer.from_array(np.ascontiguousarray(a, np.uint8)) for a in (cur_y, cur_uv, ref_y, ref_uv)]
+    out = DeviceBuffer(max((H // 8) * (W // 8) * 12, 4))
+    me_full_search_nv12(*bufs, H, W, W, W, R, out)
+    return out.download((H // 8, W // 8, 3), np.int32)
+
+
+def export_mvs(mv: DeviceBuffer, nby: int, nbx: int, B: int, units: int, fmt: int, out: DeviceBuffer,
+               stream=None) -> None:
+    """Motion vectors (dy, dx, cost) in pels (units 1) or quarter pels (units 4)
+    to int16 quarter-pel (mvx, mvy) pairs (fmt 0) or FFmpeg AVMotionVector
+    records (fmt 1)."""
+    need = nby * nbx * (4 if fmt == MV_Q4_INT16 else 40)
+    if out.nbytes < need or mv.nbytes < nby * nbx * 12:
+        raise ValueError("motion vector buffers too small")
+    call("umi_me_export_mvs", mv.ptr, nby, nbx, B, units, fmt, out.ptr, _stream(stream))
+
+
+def export_mvs_np(mv: np.ndarray, B: int, units: int, fmt: int) -> np.ndarray:
+    nby, nbx = mv.shape[:2]
+    dm = DeviceBuffer.from_array(np.ascontiguousarray(mv, np.int32))
+    if fmt == MV_Q4_INT16:
+        out = DeviceBuffer(max(nby * nbx * 4, 4))
+        export_mvs(dm, nby, nbx, B, units, fmt, out)
+        return out.download((nby, nbx, 2), np.int16)
+    out = DeviceBuffer(max(nby * nbx * 40, 4))
+    export_mvs(dm, nby, nbx, B, units, fmt, out)
+    return out.download((nby * nbx * 40,), np.uint8).view(AVMOTIONVECTOR_DTYPE).reshape(nby, nbx)
+
+
+# ---------------------------------------------------------------------------
 # SAD cost volume (main/cost_volume.cu:35-80, johnjohnlin/UMI)
 # ---------------------------------------------------------------------------
 def cost_volume(L: DeviceBuffer, R: DeviceBuffer, H: int, W: int, pitch: int, D: int, s0: int, step: int,

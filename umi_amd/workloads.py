@@ -48,6 +48,26 @@ def me_pair(H: int = 1080, W: int = 1920, R: int = 16, tile: int = 16, sigma: fl
     return f1, f0, mv
 
 
+def nv12_pair(H: int = 1080, W: int = 1920, R: int = 16, tile: int = 16, sigma: float = 2.0, seed: int = 1):
+    """NV12 frame pair: the luma of me_pair and a 4:2:0 chroma plane pair
+    moved with it (half the luma vector, floor), each plane smoothed noise
+    plus N(0, sigma^2).  Returns (cur_y, cur_uv, ref_y, ref_uv, true_mv)."""
+    cy, ry, mv = me_pair(H, W, R, tile, sigma, seed)
+    g = rng(seed + 7)
+    HC, WC = H // 2, W // 2
+    yy, xx = np.mgrid[0:HC, 0:WC]
+    my = mv[(2 * yy) // tile, (2 * xx) // tile, 0] >> 1
+    mx = mv[(2 * yy) // tile, (2 * xx) // tile, 1] >> 1
+    ruv = np.empty((HC, 2 * WC), np.uint8)
+    cuv = np.empty_like(ruv)
+    for ch in range(2):
+        p0 = smooth_noise_u8(HC, WC, g)
+        src = p0[np.clip(yy + my, 0, HC - 1), np.clip(xx + mx, 0, WC - 1)].astype(np.float64)
+        ruv[:, ch::2] = p0
+        cuv[:, ch::2] = np.clip(np.rint(src + g.normal(0.0, sigma, size=(HC, WC))), 0, 255).astype(np.uint8)
+    return cy, cuv, ry, ruv, mv
+
+
 def smooth_motion_pair(H: int, W: int, R: int = 16, sigma: float = 2.0, seed: int = 1):
     """Frame pair with a smooth affine motion field: frame1(y, x) =
     clip(round(frame0[cl(y+my)][cl(x+mx)] + N(0, sigma^2))) with
