@@ -66,6 +66,17 @@ struct Sad {  // main/full_me.cu:8-15, main/cost_volume.cu:8-15
     STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = sad; }
 };
 
+// SAD over two planes at once (the U and V planes of a chroma array):
+// the loop of main/full_me.cu:8-15 summed over the planes of its inputs.
+struct Sad2 {
+    int sad;
+    struct Constant {};
+    STRATEGY_TYPEDEF_STUB(int, Constant)
+    STRATEGY_PRE_STUB(0, 0) { sad = 0; }
+    STRATEGY_LOOP_STUB(2, 0) { sad += abs(IMEM(0, 0) - IMEM(1, 0)) + abs(IMEM(0, 1) - IMEM(1, 1)); }
+    STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = sad; }
+};
+
 struct Dot {  // main/gemm.cu:8-15, main/fprop.cu:10-17
     int accum;
     struct Constant {};

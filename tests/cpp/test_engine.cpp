@@ -10,6 +10,7 @@
 // planes per array (PtrNum), Loop outputs, Pre/Post inputs and outputs,
 // negative strides, a partially filled accumulation block, clamp vs pad.
 // Build: __graft_entry__.build() (hipcc -x hip).  Run: tests/test_engine_gpu.py.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -209,6 +210,56 @@ static void case_full_me(int h, int w, int b, int sr) {
     compare("full-search SAD volume (clamp boundary)", sad.s.get_cpu_ro(), sad.cpu_out.data(), n, 0);
 }
 
+static void case_nv12_chroma_planes(int h, int w, int sr) {
+    // NV12 chroma: U and V interleaved in one [hc][wc][2] array, read as two
+    // planes of one input (pointers uv and uv + 1, the channel index left at
+    // 0 by the views); SAD volume of the 4x4 chroma blocks over a search of
+    // sr x sr offsets, clamp boundary.
+    const int hc = h / 2, wc = w / 2, b = 4, ofs = -(sr / 2);
+    const size_t n = (size_t)(wc / b) * (hc / b) * sr * sr;
+    Buf<int> cur(hc * wc * 2), ref(hc * wc * 2), sad(n);
+    fill_int(cur.host(), cur.n, 0, 255);
+    fill_int(ref.host(), ref.n, 0, 255);
+    auto run = [&](const int *cp, const int *rp, int *sp, bool gpu) {
+        const NDTuple<ParallelDim> pd{{{hc / b, 1, 1, 1}, {wc / b, 1, 1, 1}, {sr, 16, 2, 1}, {sr, 16, 16, 1}}};
+        const NDTuple<AccumulationDim> ad{{{1, 1}, {1, 1}, {b, b}, {b, b}}};
+        const NDTuple<PhysicalDim> phys{{{1, WRAP}, {hc, WRAP}, {wc, WRAP}, {2, WRAP}}};
+        const std::array<UnrolledI<int>, 2> I{{
+            {{{{1, 0, b}, {2, 0, b}, {1, ofs, 1}, {2, ofs, 1}}}, {{{}, {}, {1, 0, 1}, {2, 0, 1}}}, phys, DEFAULT_SHM, {rp, rp + 1}, {}},
+            {{{{1, 0, b}, {2, 0, b}, {}, {}}}, {{{}, {}, {1, 0, 1}, {2, 0, 1}}}, phys, DEFAULT_SHM, {cp, cp + 1}, {}},
+        }};
+        const std::array<UnrolledO<int>, 1> O{{{{{{0, 0, 1}, {1, 0, 1}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{hc / b, WRAP}, {wc / b, WRAP}, {sr, WRAP}, {sr, WRAP}}}, {sp}}}};
+        constexpr uint64_t PN = 0x122;  // inputs: 2 planes each; output: 1
+        if (gpu)
+            Execute<Sad2, PN, 0x2211>(pd, ad, I, O);
+        else
+            umi_oracle::Execute<Sad2, PN>(pd, ad, I, O);
+    };
+    run(cur.s.get_gpu_ro(), ref.s.get_gpu_ro(), sad.s.get_gpu_wo(), true);
+    run(cur.s.get_cpu_ro(), ref.s.get_cpu_ro(), sad.cpu_out.data(), false);
+    compare("NV12 chroma planes: engine vs interpreter", sad.s.get_cpu_ro(), sad.cpu_out.data(), n, 0);
+    // direct loop over the interleaved array
+    std::vector<int> direct(n);
+    const int *c = cur.s.get_cpu_ro(), *r = ref.s.get_cpu_ro();
+    size_t o = 0;
+    for (int by = 0; by < hc / b; ++by)
+        for (int bx = 0; bx < wc / b; ++bx)
+            for (int sy = 0; sy < sr; ++sy)
+                for (int sx = 0; sx < sr; ++sx) {
+                    int sum = 0;
+                    for (int j = 0; j < b; ++j)
+                        for (int i = 0; i < b; ++i) {
+                            const int y = by * b + j, x = bx * b + i;
+                            const int yy = std::min(std::max(y + sy + ofs, 0), hc - 1);
+                            const int xx = std::min(std::max(x + sx + ofs, 0), wc - 1);
+                            for (int ch = 0; ch < 2; ++ch)
+                                sum += std::abs(c[(y * wc + x) * 2 + ch] - r[(yy * wc + xx) * 2 + ch]);
+                        }
+                    direct[o++] = sum;
+                }
+    compare("NV12 chroma planes: engine vs direct loop", sad.s.get_cpu_ro(), direct.data(), n, 0);
+}
+
 static void case_cost_volume(int h, int w, int k, int sr) {
     const int k_ofs = k / 2, sr_ofs = sr / 2;
     Buf<int> frame(w * h), cv((size_t)w * h * sr);
@@ -395,6 +446,7 @@ int main() {
     case_bilateral(50, 40, 4.0f, 10.0f);
     case_iimg(300, 500);
     case_full_me(96, 160, 8, 15);
+    case_nv12_chroma_planes(72, 104, 9);
     case_cost_volume(40, 200, 7, 15);
     case_fprop(2, 13, 9, 20, 24, 7, 2);
     case_gemm(100, 100, 200);
