@@ -1,15 +1,16 @@
-"""Recipe for oracle/_ref/: the reference's own example programs
+"""Recipe for tests/dropin/bin/: the reference's own example programs
 (main/*.cu, read where they lie under /root/reference) compiled unchanged
-against this project's UMI headers (umi_amd/src) and library.
+against this project's UMI headers (umi_amd/src) and library -- the drop-in
+test of the C++ layer.
 
 TEST INFRASTRUCTURE ONLY.  Each program checks its results with its own CPU
 loop; tests/test_engine_gpu.py runs them on the GPU.  main/bilateral.cu calls
 two CUDA runtime functions outside the UMI API (bilateral.cu:101); the recipe
 maps them to their HIP equivalents on the command line.  Outputs go only to
-oracle/_ref/ (kept out of version control).  Nothing is built when
-/root/reference is absent.
+tests/dropin/bin/ (kept out of version control; they travel to the GPU box
+with the tree).  Nothing is built when /root/reference is absent.
 
-    python oracle/build_ref.py
+    python tests/dropin/build_dropin.py
 """
 from __future__ import annotations
 
@@ -20,9 +21,9 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-ROOT = HERE.parent
+ROOT = HERE.parents[1]
 REF_MAIN = Path("/root/reference/main")
-OUT = HERE / "_ref"
+OUT = HERE / "bin"
 LIB = ROOT / "umi_amd" / "lib" / "libumi_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-x", "hip", f"--offload-arch={os.environ.get('UMI_OFFLOAD_ARCH', 'gfx950')}", "-O2", "-std=c++17",

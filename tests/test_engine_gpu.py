@@ -2,7 +2,7 @@
 
 * tests/cpp/bin/test_engine: strategy structs run by the engine and by the CPU
   interpreter (oracle/umi_interp.h), every output compared;
-* oracle/_ref/<name>_gpu: the reference's own example programs (main/*.cu)
+* tests/dropin/bin/<name>_gpu: the reference's own example programs (main/*.cu)
   compiled unchanged against this project's UMI.h / UMI_gpu.h /
   SyncedMemory.h; each asserts its results with its own CPU loop.
 """
@@ -33,7 +33,7 @@ def test_engine_vs_interpreter(gpu):
 
 @pytest.mark.parametrize("name", EXAMPLES)
 def test_reference_example_drop_in(gpu, name):
-    exe = ROOT / "oracle" / "_ref" / f"{name}_gpu"
+    exe = ROOT / "tests" / "dropin" / "bin" / f"{name}_gpu"
     if not exe.exists():
         pytest.skip("drop-in build needs the reference sources at build time")
     r = run(exe)

@@ -94,13 +94,13 @@ def build_tests() -> list[Path]:
 
 
 def build_dropin() -> list[Path]:
-    """The reference's example programs against this library (oracle/build_ref.py)."""
-    sys.path.insert(0, str(ROOT / "oracle"))
+    """The reference's example programs against this library (tests/dropin/build_dropin.py)."""
+    sys.path.insert(0, str(ROOT / "tests" / "dropin"))
     try:
-        import build_ref
+        import build_dropin as recipe
     finally:
         sys.path.pop(0)
-    return build_ref.build()
+    return recipe.build()
 
 
 def build_oracle() -> Path:
