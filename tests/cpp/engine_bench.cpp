@@ -2,8 +2,10 @@
 // strategies and descriptors (main/*.cu, johnjohnlin/UMI), at larger sizes:
 // hipEvents around repeated ExecuteAsync launches on one stream, inputs
 // resident.  One JSON line per case: time per launch, algorithmic bytes
-// (inputs read once + outputs written once) and the rate.  Not a test: the
-// results are checked by test_engine (same engine, interpreter oracle).
+// (inputs read once + outputs written once) and the rate; the tuner's pick;
+// for the two stencils also the time of the same call routed to the
+// specialised kernel.  Not a test: the results are checked by test_engine
+// (same engine and routes, interpreter oracle).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -35,7 +37,7 @@ static double time_us(F &&launch, int reps) {
 }
 
 static void report(const char *name, double us, double bytes, double ops, const char *ops_unit,
-                   const TuneResult *t = nullptr) {
+                   const TuneResult *t = nullptr, double routed_us = -1) {
     std::printf("{\"case\": \"%s\", \"us\": %.2f, \"bytes\": %.0f, \"gbs\": %.1f, \"ops\": %.0f, \"ops_per_s\": %.4g, "
                 "\"ops_unit\": \"%s\"",
                 name, us, bytes, bytes / (us * 1e-6) / 1e9, ops, ops / (us * 1e-6), ops_unit);
@@ -45,6 +47,8 @@ static void report(const char *name, double us, double bytes, double ops, const 
                     t->us, bytes / (t->us * 1e-6) / 1e9, ops / (t->us * 1e-6), (unsigned long long)t->expand,
                     t->p_dims[0].block_size, t->p_dims[1].block_size, t->p_dims[2].block_size, t->p_dims[3].block_size,
                     t->candidates);
+    if (routed_us >= 0)
+        std::printf(", \"routed_us\": %.2f, \"routed_gbs\": %.1f", routed_us, bytes / (routed_us * 1e-6) / 1e9);
     std::printf("}\n");
     std::fflush(stdout);
 }
@@ -66,8 +70,24 @@ static void ok(umi_status s) {
     }
 }
 
+// Times ExecuteAsync with routing to the specialised kernels on; aborts if
+// the call was not routed.
+template <class F>
+static double routed_time_us(F &&launch, int reps) {
+    route::enable(true);
+    const long before = route::routed();
+    const double us = time_us(launch, reps);
+    if (route::routed() == before) {
+        std::fprintf(stderr, "expected the call to be routed\n");
+        std::abort();
+    }
+    route::enable(false);
+    return us;
+}
+
 int main() {
     const int R = 20;
+    route::enable(false);  // the generic engine (and its tuner) unless a case asks for routing
     {  // main/wfilter3x3.cu:39-68 at 1024 x 1024
         const int h = 1024, w = 1024;
         int *in = dev<int>(h * w, 1000), *wt = dev<int>(9, 5), *out = dev<int>(h * w, 1);
@@ -80,7 +100,8 @@ int main() {
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<WeightedFilter, 0x111, 0x1122, 0xc0>(nullptr, pd, ad, I, O, {-5})); }, R);
         const TuneResult t = Autotune<WeightedFilter, 0x111, 0xc0, 0x1111, 0x1112, 0x1122, 0x1124, 0x1142, 0x1144>(nullptr, pd, ad, I, O, {-5}, R);
-        report("wfilter3x3 int32 1024x1024 (Expand 1122, ZeroStrideOpt c0)", us, 8.0 * h * w, 9.0 * h * w, "MAC", &t);
+        const double ru = routed_time_us([&] { ok(ExecuteAsync<WeightedFilter, 0x111, 0x1122, 0xc0>(nullptr, pd, ad, I, O, {-5})); }, R);
+        report("wfilter3x3 int32 1024x1024 (Expand 1122, ZeroStrideOpt c0)", us, 8.0 * h * w, 9.0 * h * w, "MAC", &t, ru);
     }
     {  // main/filter3x3.cu:36-58 at 1024 x 1024
         const int h = 1024, w = 1024;
@@ -92,7 +113,8 @@ int main() {
         const std::array<UnrolledO<int>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {out}}}};
         const double us = time_us([&] { ok(ExecuteAsync<BoxFilter, 0x11, 0x1111, 0x1122>(nullptr, pd, ad, I, O, {-5, 2})); }, R);
         const TuneResult t = Autotune<BoxFilter, 0x11, 0, 0x1111, 0x1112, 0x1122, 0x1124, 0x1142, 0x1144>(nullptr, pd, ad, I, O, {-5, 2}, R);
-        report("filter3x3 int32 1024x1024", us, 8.0 * h * w, 9.0 * h * w, "add", &t);
+        const double ru = routed_time_us([&] { ok(ExecuteAsync<BoxFilter, 0x11, 0x1111, 0x1122>(nullptr, pd, ad, I, O, {-5, 2})); }, R);
+        report("filter3x3 int32 1024x1024", us, 8.0 * h * w, 9.0 * h * w, "add", &t, ru);
     }
     {  // main/bilateral.cu:54-99, 5 x 5 window, 1024 x 1024
         const int h = 1024, w = 1024, ofs = 2, k = 5;

@@ -29,6 +29,50 @@ struct WeightedFilter {  // main/wfilter3x3.cu:8-15
     STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = accum; }
 };
 
+// Float forms of the two filters (the same bodies over float).
+struct BoxFilterF {
+    float accum;
+    struct Constant {
+        float init, mul;
+    };
+    STRATEGY_TYPEDEF_STUB(float, Constant)
+    STRATEGY_PRE_STUB(0, 0) { accum = c.init; }
+    STRATEGY_LOOP_STUB(1, 0) { accum += IMEM(0, 0); }
+    STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = accum * c.mul; }
+};
+struct WeightedFilterF {
+    float accum;
+    struct Constant {
+        float init;
+    };
+    STRATEGY_TYPEDEF_STUB(float, Constant)
+    STRATEGY_PRE_STUB(0, 0) { accum = c.init; }
+    STRATEGY_LOOP_STUB(2, 0) { accum += IMEM(0, 0) * IMEM(1, 0); }
+    STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = accum; }
+};
+// Look-alikes that a specialised kernel does not compute: constants in the
+// other order, and one more per tap.  Both must stay on the generic engine.
+struct BoxFilterSwapped {
+    int accum;
+    struct Constant {
+        int mul, init;
+    };
+    STRATEGY_TYPEDEF_STUB(int, Constant)
+    STRATEGY_PRE_STUB(0, 0) { accum = c.init; }
+    STRATEGY_LOOP_STUB(1, 0) { accum += IMEM(0, 0); }
+    STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = accum * c.mul; }
+};
+struct BoxFilterPlusOne {
+    int accum;
+    struct Constant {
+        int init, mul;
+    };
+    STRATEGY_TYPEDEF_STUB(int, Constant)
+    STRATEGY_PRE_STUB(0, 0) { accum = c.init; }
+    STRATEGY_LOOP_STUB(1, 0) { accum += IMEM(0, 0) + 1; }
+    STRATEGY_POST_STUB(0, 1) { OMEM(0, 0) = accum * c.mul; }
+};
+
 struct BilateralF {  // main/bilateral.cu:9-24
     float wsum, wxsum, center;
     struct Constant {

@@ -438,7 +438,64 @@ static void case_invalid_dims() {
     }
 }
 
+// Routing (UMI_route.h): the same call with routing on and off.  The routed
+// result must equal the generic engine's bit for bit (float included: both
+// run the fused device arithmetic) and, for int, the interpreter; the
+// look-alike strategies must not be routed.
+template <class S, uint64_t PN, class E>
+static void case_route(const char *name, int h, int w, int mode, const typename S::UserConstantType &c,
+                       bool expect_route, int ablk3 = 3) {
+    constexpr bool WEIGHTED = S::N_LOOP_IN == 2;
+    Buf<E> in(w * h), out(w * h), oute(w * h), wt(9);
+    if constexpr (std::is_same<E, float>::value) {
+        fill_float(in.host(), in.n);
+        const float wv[9] = {0.0625f, 0.125f, 0.0625f, 0.125f, 0.25f, 0.125f, 0.0625f, 0.125f, 0.0625f};
+        std::copy(wv, wv + 9, wt.host());
+    } else {
+        fill_int(in.host(), in.n, -1000, 1000);
+        fill_int(wt.host(), 9, -5, 5);
+    }
+    const auto M = mode ? WRAP : PAD;
+    auto run = [&](const E *ip, const E *wp, E *op, int how) {  // 0 engine, 1 routed, 2 interpreter
+        const NDTuple<ParallelDim> pd{{{1, 1, 1, 1}, {1, 1, 1, 1}, {h, 16, 4, 1}, {w, 16, 8, 1}}};
+        const NDTuple<AccumulationDim> ad{{{1, 1}, {1, 1}, {3, 3}, {3, ablk3}}};
+        const UnrolledI<E> img{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {{{}, {}, {2, -1, 1}, {3, -1, 1}}},
+                               {{{1, M}, {1, M}, {h, M}, {w, M}}}, DEFAULT_SHM, {ip}, {(E)7}};
+        const std::array<UnrolledO<E>, 1> O{{{{{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {}, {{{}, {}, {h, PAD}, {w, PAD}}}, {op}}}};
+        auto go = [&](const auto &I) {
+            if (how == 2) {
+                umi_oracle::Execute<S, PN>(pd, ad, I, O, c);
+            } else {
+                route::enable(how == 1);
+                const long before = route::routed();
+                Execute<S, PN>(pd, ad, I, O, c);
+                if (how == 1 && (route::routed() - before == 1) != expect_route) {
+                    std::printf("[engine] %s: routed %ld, expected %d\n", name, route::routed() - before, (int)expect_route);
+                    g_fail = 1;
+                }
+                route::enable(false);
+            }
+        };
+        if constexpr (WEIGHTED) {
+            const UnrolledI<E> wts{{{{}, {}, {}, {}}}, {{{}, {}, {2, 0, 1}, {3, 0, 1}}}, {{{1, PAD}, {1, PAD}, {3, PAD}, {3, PAD}}},
+                                   DEFAULT_SHM, {wp}, {}};
+            go(std::array<UnrolledI<E>, 2>{{img, wts}});
+        } else {
+            go(std::array<UnrolledI<E>, 1>{{img}});
+        }
+    };
+    run(in.s.get_gpu_ro(), wt.s.get_gpu_ro(), oute.s.get_gpu_wo(), 0);
+    run(in.s.get_gpu_ro(), wt.s.get_gpu_ro(), out.s.get_gpu_wo(), 1);
+    run(in.s.get_cpu_ro(), wt.s.get_cpu_ro(), out.cpu_out.data(), 2);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s: routing on vs off", name);
+    compare(buf, out.s.get_cpu_ro(), oute.s.get_cpu_ro(), out.n, 0);
+    std::snprintf(buf, sizeof buf, "%s: vs interpreter", name);
+    compare(buf, out.s.get_cpu_ro(), out.cpu_out.data(), out.n, std::is_same<E, float>::value ? 1e-5 : 0);
+}
+
 int main() {
+    route::enable(false);  // the generic engine; routing has its own cases below
     case_box(300, 500, 0);
     case_box(37, 53, 1);
     case_wfilter(250, 300);
@@ -453,6 +510,14 @@ int main() {
     case_planes(21, 19, 8);
     case_tuned_wfilter_gemm(123, 301, 96, 80, 200);
     case_invalid_dims();
+    case_route<BoxFilter, 0x11, int>("route: 3x3 filter int32 PAD", 300, 500, 0, {-5, 2}, true);
+    case_route<BoxFilter, 0x11, int>("route: 3x3 filter int32 clamp", 37, 53, 1, {7, -3}, true);
+    case_route<WeightedFilter, 0x111, int>("route: weighted 3x3 int32", 250, 300, 0, {-5}, true);
+    case_route<BoxFilterF, 0x11, float>("route: 3x3 filter f32 clamp", 123, 301, 1, {0.5f, 0.25f}, true);
+    case_route<WeightedFilterF, 0x111, float>("route: weighted 3x3 f32 (fma)", 129, 257, 0, {-0.125f}, true);
+    case_route<BoxFilterSwapped, 0x11, int>("route: constants in the other order stay on the engine", 40, 50, 0, {2, -5}, false);
+    case_route<BoxFilterPlusOne, 0x11, int>("route: a different body stays on the engine", 40, 50, 1, {-5, 2}, false);
+    case_route<BoxFilterF, 0x11, float>("route: taps split over accumulation blocks stay on the engine", 40, 50, 0, {0.5f, 0.25f}, false, 2);
     std::printf("[engine] %s\n", g_fail ? "FAILED" : "all cases passed");
     return g_fail ? 1 : 0;
 }

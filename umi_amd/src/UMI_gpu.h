@@ -480,6 +480,11 @@ inline void pack_phys(const NDTuple<PhysicalDim> &v, int32_t *o) {
 }
 
 }  // namespace engine
+}  // namespace UMI
+
+#include "UMI_route.h"
+
+namespace UMI {
 
 // Status-returning form of Execute: validates, launches on `stream`, does not wait.
 template <class Umi, uint64_t PtrNum = 0x1111111111111111llu, uint64_t Expand = 0x1111, uint64_t ZeroStrideOpt = 0,
@@ -508,6 +513,9 @@ umi_status ExecuteAsync(umi_stream stream, const NDTuple<ParallelDim> &p_dims, c
     }
     umi_status st = umi_validate_execute(pd, ad, NI, NO, views, phys, planes, Expand, ZeroStrideOpt);
     if (st != UMI_OK) return st;
+    // a call a specialised kernel computes exactly goes to it (UMI_route.h)
+    if (route::enabled() && route::try_stencil<Umi, PtrNum, EType, CType>(stream, p_dims, a_dims, imems, omems, c, st))
+        return st;
 
     using X = engine::Ex<Expand>;
     engine::Args<EType, NI, NO> g{};
