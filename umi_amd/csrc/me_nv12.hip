@@ -38,14 +38,13 @@ constexpr int NV_LR = 8 + 2 * NV_RMAX;    // luma window rows
 constexpr int NV_LW = NV_TBX * 8 + 2 * NV_RMAX + 8;  // luma window bytes per row (qsad reads 8 past a group)
 constexpr int NV_CR = 4 + NV_RMAX;        // chroma window rows (floor(dy/2) in [-8, 8])
 constexpr int NV_CW = NV_TBX * 4 + NV_RMAX + 8;      // chroma window bytes per row
-constexpr int NV_RAWR = NV_CR + 1, NV_RAWW = NV_CW + 1;  // raw chroma window (one more row and column)
+constexpr int NV_PS = NV_CW / 4 + 1;      // dwords per chroma plane row (one more for the +1 column)
 static_assert(NV_THREADS >= NV_TBX * (2 * NV_RMAX + 1) && NV_THREADS % 64 == 0, "workgroup too small");
 static_assert(NV_LW % 4 == 0 && NV_CW % 4 == 0, "rows must be whole dwords");
 
 struct NvSmem {
     uint32_t lum[NV_LR * NV_LW / 4];                // luma window
-    uint32_t phase[4][2][NV_CR * NV_CW / 4];        // [2 fy/4 + fx/4][U,V][row][byte]
-    uint8_t raw[2][NV_RAWR][NV_RAWW];               // clamped chroma samples
+    uint32_t pl[4][2][(NV_CR + 1) * NV_PS];         // [2 fy/4 + fx/4][U,V][row][dword]; phase 0 = the samples
     uint32_t curl[NV_TBX][8][2];                    // current luma, 8 bytes per row
     uint32_t curc[NV_TBX][2][4];                    // current chroma, 4 bytes per row and plane
     uint32_t key[NV_TBX];
@@ -65,19 +64,41 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
     const int cy0 = by * 4 - NV_RMAX / 2, cx0 = bx0 * 4 - NV_RMAX / 2;  // chroma window origin
 
     // ---- staging (clamped coordinates)
+    const bool y_aligned = ((reinterpret_cast<uintptr_t>(ry) | (uintptr_t)pitch_y) & 3) == 0;
     for (int e = tid; e < NV_LR * NV_LW / 4; e += NV_THREADS) {
         const int rr = e / (NV_LW / 4), c4 = (e - rr * (NV_LW / 4)) * 4;
         const uint8_t *row = ry + (size_t)clampi(y0 + rr, 0, H - 1) * pitch_y;
+        const int x = x0 + c4;  // a multiple of 4
         uint32_t v = 0;
+        if (y_aligned && x >= 0 && x + 3 < W) {
+            v = *reinterpret_cast<const uint32_t *>(row + x);
+        } else {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(x0 + c4 + b, 0, W - 1)] << (8 * b);
+            for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(x + b, 0, W - 1)] << (8 * b);
+        }
         sm.lum[e] = v;
     }
-    for (int e = tid; e < 2 * NV_RAWR * NV_RAWW; e += NV_THREADS) {
-        const int ch = e / (NV_RAWR * NV_RAWW), r = e - ch * (NV_RAWR * NV_RAWW);
-        const int rr = r / NV_RAWW, cc = r - rr * NV_RAWW;
-        sm.raw[ch][rr][cc] =
-            ruv[(size_t)clampi(cy0 + rr, 0, HC - 1) * pitch_uv + 2 * clampi(cx0 + cc, 0, WC - 1) + ch];
+    // clamped chroma samples, U and V apart, 4 per dword (8 interleaved bytes per load inside the plane)
+    const bool uv_aligned = ((reinterpret_cast<uintptr_t>(ruv) | (uintptr_t)pitch_uv) & 7) == 0;
+    for (int e = tid; e < (NV_CR + 1) * NV_PS; e += NV_THREADS) {
+        const int rr = e / NV_PS, c4 = (e - rr * NV_PS) * 4;
+        const uint8_t *row = ruv + (size_t)clampi(cy0 + rr, 0, HC - 1) * pitch_uv;
+        const int x = cx0 + c4;
+        uint32_t u = 0, v = 0;
+        if (uv_aligned && x >= 0 && x + 3 < WC) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(row + 2 * x);
+            u = __builtin_amdgcn_perm(q.y, q.x, 0x06040200u);  // bytes 0, 2, 4, 6
+            v = __builtin_amdgcn_perm(q.y, q.x, 0x07050301u);  // bytes 1, 3, 5, 7
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint8_t *sp = row + 2 * clampi(x + b, 0, WC - 1);
+                u |= (uint32_t)sp[0] << (8 * b);
+                v |= (uint32_t)sp[1] << (8 * b);
+            }
+        }
+        sm.pl[0][0][e] = u;
+        sm.pl[0][1][e] = v;
     }
     for (int e = tid; e < NV_TBX * 16; e += NV_THREADS) {  // current luma: [k][r][w]
         const int k = e >> 4, r = (e >> 1) & 7, w = e & 1, bx = bx0 + k;
@@ -101,29 +122,28 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
     }
     if (tid < NV_TBX) sm.key[tid] = 0xffffffffu;
     __syncthreads();
-    // the four half-pel phases of U and V, one dword (4 samples) per step
-    for (int e = tid; e < 4 * 2 * NV_CR * (NV_CW / 4); e += NV_THREADS) {
-        const int ph = e / (2 * NV_CR * (NV_CW / 4)), r0 = e - ph * (2 * NV_CR * (NV_CW / 4));
+    // the three half-pel phases of U and V from the samples, 4 per dword:
+    // (A+B+1)>>1 and (A+C+1)>>1 bytewise as (a|b) - (((a^b) & 0xfe..) >> 1),
+    // (A+B+C+D+2)>>2 in 16-bit lanes
+    for (int e = tid; e < 3 * 2 * NV_CR * (NV_CW / 4); e += NV_THREADS) {
+        const int ph = 1 + e / (2 * NV_CR * (NV_CW / 4)), r0 = e % (2 * NV_CR * (NV_CW / 4));
         const int ch = r0 / (NV_CR * (NV_CW / 4)), r1 = r0 - ch * (NV_CR * (NV_CW / 4));
-        const int rr = r1 / (NV_CW / 4), c4 = (r1 - rr * (NV_CW / 4)) * 4;
-        const int fy = ph >> 1, fx = ph & 1;
-        uint32_t v = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int A = sm.raw[ch][rr][c4 + b], B = sm.raw[ch][rr][c4 + b + 1];
-            const int C = sm.raw[ch][rr + 1][c4 + b], D = sm.raw[ch][rr + 1][c4 + b + 1];
-            int p;
-            if (!fy && !fx)
-                p = A;
-            else if (!fy)
-                p = (A + B + 1) >> 1;
-            else if (!fx)
-                p = (A + C + 1) >> 1;
-            else
-                p = (A + B + C + D + 2) >> 2;
-            v |= (uint32_t)p << (8 * b);
+        const int rr = r1 / (NV_CW / 4), c = r1 - rr * (NV_CW / 4);
+        const uint32_t *s0 = sm.pl[0][ch] + rr * NV_PS + c, *s1 = s0 + NV_PS;
+        const uint32_t A = s0[0], B = __builtin_amdgcn_alignbyte(s0[1], s0[0], 1);
+        const uint32_t C = s1[0], D = __builtin_amdgcn_alignbyte(s1[1], s1[0], 1);
+        uint32_t v;
+        if (ph == 1) {
+            v = (A | B) - (((A ^ B) & 0xfefefefeu) >> 1);
+        } else if (ph == 2) {
+            v = (A | C) - (((A ^ C) & 0xfefefefeu) >> 1);
+        } else {
+            const uint32_t m = 0x00ff00ffu;
+            const uint32_t lo = (A & m) + (B & m) + (C & m) + (D & m) + 0x00020002u;
+            const uint32_t hi = ((A >> 8) & m) + ((B >> 8) & m) + ((C >> 8) & m) + ((D >> 8) & m) + 0x00020002u;
+            v = ((lo >> 2) & m) | (((hi >> 2) & m) << 8);
         }
-        sm.phase[ph][ch][rr * (NV_CW / 4) + c4 / 4] = v;
+        sm.pl[ph][ch][rr * NV_PS + c] = v;
     }
     __syncthreads();
 
@@ -134,7 +154,7 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
         uint64_t acc[9];
 #pragma unroll
         for (int g = 0; g < 9; ++g) acc[g] = 0;
-#pragma unroll
+#pragma unroll 2
         for (int r = 0; r < 8; ++r) {
             const uint32_t *wrow = sm.lum + (r + dy + NV_RMAX) * (NV_LW / 4) + 2 * k;
             uint32_t w[11];
@@ -154,15 +174,15 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
         for (int f = 0; f < 2; ++f)
 #pragma unroll
             for (int m = 0; m < 5; ++m) cacc[f][m] = 0;
-#pragma unroll
+#pragma unroll 1
         for (int ch = 0; ch < 2; ++ch)
-#pragma unroll
+#pragma unroll 2
             for (int j = 0; j < 4; ++j) {
                 const uint32_t cc = sm.curc[k][ch][j];
                 const int rr = j + oy + NV_RMAX / 2;
 #pragma unroll
                 for (int fx = 0; fx < 2; ++fx) {
-                    const uint32_t *prow = sm.phase[2 * fy + fx][ch] + rr * (NV_CW / 4) + k;
+                    const uint32_t *prow = sm.pl[2 * fy + fx][ch] + rr * NV_PS + k;
                     uint32_t d[6];
 #pragma unroll
                     for (int j2 = 0; j2 < 6; ++j2) d[j2] = prow[j2];
