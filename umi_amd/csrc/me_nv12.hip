@@ -154,7 +154,7 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
         uint64_t acc[9];
 #pragma unroll
         for (int g = 0; g < 9; ++g) acc[g] = 0;
-#pragma unroll 2
+#pragma unroll 4
         for (int r = 0; r < 8; ++r) {
             const uint32_t *wrow = sm.lum + (r + dy + NV_RMAX) * (NV_LW / 4) + 2 * k;
             uint32_t w[11];
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
             for (int m = 0; m < 5; ++m) cacc[f][m] = 0;
 #pragma unroll 1
         for (int ch = 0; ch < 2; ++ch)
-#pragma unroll 2
+#pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint32_t cc = sm.curc[k][ch][j];
                 const int rr = j + oy + NV_RMAX / 2;
