@@ -356,12 +356,15 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
  * scaling and give identical results; the switch exists for measurement and
  * tests.  No reference counterpart. */
 umi_status umi_conv_set_direct(int enable);
-/* GEMMs with 256 x 256 tiles: nonzero (default 8) = operands staged by
- * direct-to-LDS loads in a 4-stage pipeline (2 for the split forms), with 8
- * waves per workgroup (two per SIMD), or 4 when enable == 4; 0 = through
- * registers with one stage in flight.  Same operands, MFMA order and
- * epilogue: identical results; the switch exists for measurement and tests.
- * No reference counterpart. */
+/* GEMMs with 256 x 256 tiles: nonzero = operands staged by direct-to-LDS
+ * loads in a 4-stage ring (2 for the split forms); 1 (default) = with the
+ * MFMA operand reads software-pipelined through two register slots (f32,
+ * bf16, fp8) and 8 waves per workgroup (two per SIMD), 5 = the same with 4
+ * waves; 2 / 4 = the ring without the register pipeline, 8 / 4 waves; 0 =
+ * through registers with one stage in flight.  Other values:
+ * UMI_ERR_INVALID.  Same operands and MFMA order in every form: identical
+ * results; the switch exists for measurement and tests.  No reference
+ * counterpart. */
 umi_status umi_gemm_set_lds_direct(int enable);
 
 /* ---------------------------------------------------------------------------

@@ -76,6 +76,17 @@ def test_errors_are_status_codes_not_aborts():
     assert b"outside" in L.umi_last_error()
 
 
+def test_gemm_form_switch_rejects_unknown_values():
+    # the measurement switch of the 256 x 256 GEMM takes 0, 1, 2, 4 or 5 only
+    from umi_amd._lib import lib
+
+    L = lib()
+    for bad in (-1, 3, 6, 8, 9):
+        assert L.umi_gemm_set_lds_direct(bad) == 1
+        assert b"umi_gemm_set_lds_direct" in L.umi_last_error()
+    assert L.umi_gemm_set_lds_direct(1) == 0
+
+
 def _pack(pd, ad):
     import numpy as np
 

@@ -204,6 +204,10 @@ def test_gemm_direct_lds_equals_register_staging(mode, M, N, K, gpu):
     try:
         call("umi_gemm_set_lds_direct", 0)
         ref = ops.gemm_np(A, B, mode)
+        # the LDS ring with and without the register pipeline, 8 and 4 waves
+        for form in (2, 4, 5):
+            call("umi_gemm_set_lds_direct", form)
+            np.testing.assert_array_equal(ops.gemm_np(A, B, mode), ref, err_msg=f"form {form}")
     finally:
         call("umi_gemm_set_lds_direct", 1)
     np.testing.assert_array_equal(ops.gemm_np(A, B, mode), ref)

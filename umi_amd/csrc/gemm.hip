@@ -68,6 +68,13 @@ struct Mfma;
 template <>
 struct Mfma<GM_F32> {
     static constexpr int ESZ = 4, LANEB = 4, CH = 8, NOP = 2;
+    // operand fragment of one lane and its LDS reads; the MFMA on fragments
+    using Frag = float;
+    static constexpr int NRD = 1;
+    static __device__ Frag load(const uint8_t *row, int off, int sw) { return *(const float *)(row + swz(off, sw)); }
+    static __device__ f32x16 mma(Frag a, Frag b, f32x16 c, int, int) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
     // a, b: the lane's LDS rows; off: byte offset in the row; sw: the rows' swizzle
     static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int, int) {
         const int o = swz(off, sw);
@@ -77,6 +84,12 @@ struct Mfma<GM_F32> {
 template <>
 struct Mfma<GM_BF16> {
     static constexpr int ESZ = 2, LANEB = 16, CH = 2, NOP = 2;
+    using Frag = bf16x8;
+    static constexpr int NRD = 1;
+    static __device__ Frag load(const uint8_t *row, int off, int sw) { return *(const bf16x8 *)(row + swz(off, sw)); }
+    static __device__ f32x16 mma(Frag a, Frag b, f32x16 c, int, int) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
     static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int, int) {
         const int o = swz(off, sw);
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8 *)(a + o), *(const bf16x8 *)(b + o), c, 0, 0,
@@ -90,6 +103,15 @@ struct Mfma<GM_BF16X3> : Mfma<GM_BF16> {
 template <>
 struct Mfma<GM_FP8> {
     static constexpr int ESZ = 1, LANEB = 32, CH = 1, NOP = 2;
+    using Frag = i32x8;
+    static constexpr int NRD = 2;
+    static __device__ Frag load(const uint8_t *row, int off, int sw) {
+        const i32x4 q0 = *(const i32x4 *)(row + swz(off, sw)), q1 = *(const i32x4 *)(row + swz(off + 16, sw));
+        return i32x8{q0[0], q0[1], q0[2], q0[3], q1[0], q1[1], q1[2], q1[3]};
+    }
+    static __device__ f32x16 mma(Frag a, Frag b, f32x16 c, int sa, int sb) {
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, sa, 0, sb);
+    }
     // 32 bytes = two chunks, which the swizzle may place in either order
     static __device__ f32x16 run(const uint8_t *a, const uint8_t *b, int off, int sw, f32x16 c, int sa, int sb) {
         const int o0 = swz(off, sw), o1 = swz(off + 16, sw);
@@ -585,7 +607,166 @@ __global__ __launch_bounds__(TL::NT) void gemm_lds_kernel(const uint8_t *__restr
     }
 }
 
-int gemm_lds_waves = 8;  // umi_gemm_set_lds_direct: 4 or 8 waves per workgroup
+// The same tiles and LDS ring with the operand reads software-pipelined
+// (single-product forms): a wave holds two register slots of MFMA
+// fragments and, while the MFMAs of one K chunk run from one slot, reads
+// the next chunk into the other, one LDS read issued after each MFMA
+// (sched_group_barrier), so no MFMA waits on an LDS read of its own chunk.
+// A stage's LDS buffer is refilled once every wave has read its last chunk
+// into registers: at the barrier before the reads of stage st + 1, stage
+// st + NS - 1 is issued into the buffer stage st - 1 used.  Stages st + 2
+// .. st + NS - 2 stay in flight while a wave waits for stage st + 1.
+template <int MODE, class TL, int NS>
+__global__ __launch_bounds__(TL::NT) void gemm_pipe_kernel(const uint8_t *__restrict__ a_hi, long lda,
+                                                             const uint8_t *__restrict__ b_hi, long ldb, DenseDst dst,
+                                                             int M, int N, int K, int tiles_m, int tiles_n,
+                                                             const float *qa, const float *qb) {
+    using OP = Mfma<MODE>;
+    static_assert(OP::NOP == 2, "single-product forms");
+    using Frag = typename OP::Frag;
+    constexpr int BMt = TL::BM, BNt = TL::BN, TM = TL::TM, TN = TL::TN, CH = OP::CH;
+    constexpr int OFF_B = BMt * LDSROW, STAGE = (BMt + BNt) * LDSROW;
+    constexpr int NW = TL::NT / 64;
+    constexpr int WL = BMt / 16 / NW;  // direct loads of 16 rows per wave per panel
+    constexpr int LPS = 2 * WL;        // direct loads per thread per stage
+    constexpr int NMF = TM * TN, NRD = (TM + TN) * OP::NRD;  // MFMAs and LDS reads per chunk
+    static_assert(NS >= 4, "at least two stages in flight");
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wv / TL::WGN, wn = wv % TL::WGN;
+
+    const int ntiles = tiles_m * tiles_n;
+    int pid = blockIdx.x;
+    if (ntiles % 8 == 0) pid = (pid % 8) * (ntiles / 8) + pid / 8;
+    const int GROUP = 4;
+    const int group = pid / (GROUP * tiles_n), first_m = group * GROUP;
+    const int gsz = min(GROUP, tiles_m - first_m);
+    const int tm = first_m + (pid % (GROUP * tiles_n)) % gsz;
+    const int tn = (pid % (GROUP * tiles_n)) / gsz;
+    const int m0 = tm * BMt, n0 = tn * BNt;
+    const int nstages = K * OP::ESZ / ROWB;
+
+    const int lrow = lane >> 2, lslot = lane & 3;
+    const uint8_t *srcA[WL], *srcB[WL];
+#pragma unroll
+    for (int i = 0; i < WL; ++i) {
+        const int r = (wv * WL + i) * 16 + lrow;
+        const int chunk = (lslot ^ ((r >> 2) & 3)) * 16;
+        srcA[i] = a_hi + (long)min(m0 + r, M - 1) * lda + chunk;
+        srcB[i] = b_hi + (long)min(n0 + r, N - 1) * ldb + chunk;
+    }
+    auto issue = [&](int st) {
+        uint8_t *base = smem + (st % NS) * STAGE;
+        const long kb = (long)st * ROWB;
+#pragma unroll
+        for (int i = 0; i < WL; ++i) {
+            const int row0 = (wv * WL + i) * 16 * LDSROW;
+            __builtin_amdgcn_global_load_lds((const void *)(srcA[i] + kb),
+                                             (__attribute__((address_space(3))) void *)(base + row0), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(srcB[i] + kb),
+                                             (__attribute__((address_space(3))) void *)(base + OFF_B + row0), 16, 0, 0);
+        }
+    };
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    constexpr int UNIT = 127;
+    const int lr = lane & 31, lk = (lane >> 5) * OP::LANEB;
+    const int rsw = (lr >> 2) & 3;
+    const int arow = (wm * TM * 32 + lr) * LDSROW, brow = OFF_B + (wn * TN * 32 + lr) * LDSROW;
+    Frag fa[2][TM], fb[2][TN];
+    // chunk kc of stage st into register slot S
+    auto read_chunk = [&](auto S, int st, int kc) {
+        const uint8_t *base = smem + (st % NS) * STAGE;
+        const int off = kc * 2 * OP::LANEB + lk;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) fa[S][i] = OP::load(base + arow + i * 32 * LDSROW, off, rsw);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) fb[S][j] = OP::load(base + brow + j * 32 * LDSROW, off, rsw);
+    };
+    // MFMAs of slot S, the reads of the next chunk interleaved one per MFMA
+    auto mma_slot = [&](auto S) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = OP::mma(fa[S][i], fb[S][j], acc[i][j], UNIT, UNIT);
+        static_for<NMF>([&](auto t) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            constexpr int lo = (int)t * NRD / NMF, hi = ((int)t + 1) * NRD / NMF;
+            if constexpr (hi > lo) __builtin_amdgcn_sched_group_barrier(0x100, hi - lo, 0);  // LDS reads
+        });
+    };
+    // one stage; S0 = the slot its first chunk is in.  Every read is
+    // unconditional: past the last stage the chunk read is a harmless repeat.
+    auto stage = [&](auto S0, int st) {
+        static_for<CH>([&](auto kc) {
+            constexpr int S = ((int)S0 + (int)kc) & 1;
+            if constexpr ((int)kc == CH - 1) {
+                // stage st + 1 lands; buffer (st - 1) % NS is free in every wave
+                if (st + NS - 2 < nstages) wait_vm<LPS * (NS - 3)>();
+                else wait_vm<0>();
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_s_barrier();
+                asm volatile("" ::: "memory");
+                if (st + NS - 1 < nstages) issue(st + NS - 1);
+                read_chunk(std::integral_constant<int, 1 - S>{}, min(st + 1, nstages - 1), 0);
+            } else {
+                read_chunk(std::integral_constant<int, 1 - S>{}, st, (int)kc + 1);
+            }
+            mma_slot(std::integral_constant<int, S>{});
+        });
+    };
+
+    for (int st = 0; st < NS - 1 && st < nstages; ++st) issue(st);
+    if (NS - 2 < nstages) wait_vm<LPS * (NS - 2)>();
+    else wait_vm<0>();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    read_chunk(std::integral_constant<int, 0>{}, 0, 0);
+    if constexpr (CH % 2 == 0) {
+        for (int st = 0; st < nstages; ++st) stage(std::integral_constant<int, 0>{}, st);
+    } else {
+        int st = 0;
+        for (; st + 1 < nstages; st += 2) {
+            stage(std::integral_constant<int, 0>{}, st);
+            stage(std::integral_constant<int, 1>{}, st + 1);
+        }
+        if (st < nstages) stage(std::integral_constant<int, 0>{}, st);
+    }
+
+    const float alpha = (qa ? 1.0f / *qa : 1.0f) * (qb ? 1.0f / *qb : 1.0f);
+    // buffer stores: rows past M fall outside the resource (M * ldc floats,
+    // < 2^31 bytes, checked by the launcher) and are dropped by the
+    // hardware; a lane past N stores at 2^31, also outside
+    const __amdgpu_buffer_rsrc_t rc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)dst.base, 0, (int)((long)M * dst.ldc * 4), 0x00020000);
+    const int ldb4 = dst.ldc * 4;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int n = n0 + wn * TN * 32 + j * 32 + lr;
+        const int vo = n < N ? (m0 + wm * TM * 32 + 4 * (lane >> 5)) * ldb4 + n * 4 : (int)0x80000000u;
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int mr = i * 32 + 8 * (r >> 2) + (r & 3);  // row within the wave's block column
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, acc[i][j][r] * alpha), rc, vo,
+                                                      mr * ldb4, 0);
+            }
+    }
+}
+
+// umi_gemm_set_lds_direct: 1 = pipelined operand reads (single-product
+// forms), 8 waves; 5 = the same with 4 waves; 2 / 4 = the LDS ring without
+// the register pipeline, 8 / 4 waves (the split forms always use the ring)
+int gemm_lds_form = 1;
 
 template <int MODE, class TL>
 umi_status gemm_lds_launch_t(const uint8_t *ah, const uint8_t *al, long lda, const uint8_t *bh, const uint8_t *bl,
@@ -603,12 +784,34 @@ umi_status gemm_lds_launch_t(const uint8_t *ah, const uint8_t *al, long lda, con
     return launch_check("gemm_lds_kernel");
 }
 
+template <int MODE, class TL, int NS>
+umi_status gemm_pipe_launch_t(const uint8_t *ah, long lda, const uint8_t *bh, long ldb, DenseDst d, int M, int N,
+                              int K, const float *qa, const float *qb, hipStream_t st) {
+    const size_t lds = (size_t)NS * (TL::BM + TL::BN) * LDSROW;
+    static const hipError_t attr = hipFuncSetAttribute((const void *)gemm_pipe_kernel<MODE, TL, NS>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    UMI_TRY(hip_check(attr, "hipFuncSetAttribute(gemm pipe LDS)"));
+    const int tm = ceil_div(M, TL::BM), tn = ceil_div(N, TL::BN);
+    hipLaunchKernelGGL((gemm_pipe_kernel<MODE, TL, NS>), dim3(tm * tn), dim3(TL::NT), lds, st, ah, lda, bh, ldb, d, M,
+                       N, K, tm, tn, qa, qb);
+    return launch_check("gemm_pipe_kernel");
+}
+
 template <int MODE>
 umi_status gemm_lds_launch(const uint8_t *ah, const uint8_t *al, long lda, const uint8_t *bh, const uint8_t *bl,
                            long ldb, DenseDst d, int M, int N, int K, const float *qa, const float *qb,
                            hipStream_t st) {
-    if (gemm_lds_waves == 8) return gemm_lds_launch_t<MODE, LdsTile8>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
-    return gemm_lds_launch_t<MODE, LdsTile4>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
+    // the pipelined kernel's stores address C with 32-bit offsets
+    const bool c32 = (long)d.M * d.ldc * 4 < (1L << 31) && (long)d.ldc * 4 * 256 < (1L << 31);
+    if constexpr (Mfma<MODE>::NOP == 2) {
+        if (c32 && gemm_lds_form == 1)
+            return gemm_pipe_launch_t<MODE, LdsTile8, 4>(ah, lda, bh, ldb, d, M, N, K, qa, qb, st);
+        if (c32 && gemm_lds_form == 5)
+            return gemm_pipe_launch_t<MODE, LdsTile4, 4>(ah, lda, bh, ldb, d, M, N, K, qa, qb, st);
+    }
+    if (gemm_lds_form == 4 || gemm_lds_form == 5)
+        return gemm_lds_launch_t<MODE, LdsTile4>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
+    return gemm_lds_launch_t<MODE, LdsTile8>(ah, al, lda, bh, bl, ldb, d, M, N, K, qa, qb, st);
 }
 
 // Tile choice: the 256 x 256 tile when it yields at least one workgroup per
@@ -862,9 +1065,10 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
 extern "C" {
 
 umi_status umi_gemm_set_lds_direct(int enable) {
-    if (enable < 0 || enable > 8) return set_error(UMI_ERR_INVALID, "umi_gemm_set_lds_direct: %d", enable);
+    if (enable != 0 && enable != 1 && enable != 2 && enable != 4 && enable != 5)
+        return set_error(UMI_ERR_INVALID, "umi_gemm_set_lds_direct: %d (0, 1, 2, 4 or 5)", enable);
     gemm_lds_direct = enable != 0;
-    if (enable == 4 || enable == 8) gemm_lds_waves = enable;
+    if (enable != 0) gemm_lds_form = enable;
     return UMI_OK;
 }
 

@@ -396,11 +396,7 @@ __device__ inline void iimg_tile(IimgSmem<T, RPL> &sm, const IimgTileRef &d, con
         Col l = 0;
         if (lane < TH)
             l = pub_sum<Col>(ws + (me - d.tx) * tw + (long long)(IM_TW + lane) * PW, tw, d.tx, tag, ctl);
-#pragma unroll
-        for (int dd = 1; dd < 64; dd <<= 1) {
-            const Col n = __shfl_up(l, dd);
-            if (lane >= dd) l += n;
-        }
+        l = wave_inclusive_scan(l);
         if (lane < TH) sm.lp[lane] = l;
     }
     __syncthreads();

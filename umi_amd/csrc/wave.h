@@ -41,21 +41,33 @@ __device__ inline T row16_inclusive_scan(T v) {
     return v;
 }
 
-// Inclusive prefix sum across the 64 lanes of a wave (lane order), with
-// operator `add` on values moved through DPP as 32-bit words.
-template <typename T, typename Add>
-__device__ inline T wave_inclusive_scan(T v, Add add) {
-    static_assert(sizeof(T) == 4, "32-bit values");
-    auto mv = [](T x, auto step) -> T {
-        uint32_t b = __builtin_bit_cast(uint32_t, x);
-        return __builtin_bit_cast(T, step(b));
-    };
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR1, 0xf, true>(b); }));
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR2, 0xf, true>(b); }));
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR4, 0xf, true>(b); }));
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_SHR8, 0xf, true>(b); }));
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_BCAST15, 0xa, false>(b); }));
-    v = add(v, mv(v, [](uint32_t b) { return dpp_u32<DPP_ROW_BCAST31, 0xc, false>(b); }));
+// A 4- or 8-byte value moved by one DPP control (8-byte values as two
+// dwords, both moved the same way); lanes the control leaves unwritten, or
+// that read past their row with bound_ctrl, get zero bits.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO, typename T>
+__device__ inline T dpp_move(T x) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(__builtin_bit_cast(uint32_t, x)));
+    } else {
+        const uint2 w = __builtin_bit_cast(uint2, x);
+        return __builtin_bit_cast(T, make_uint2(dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(w.x),
+                                                dpp_u32<CTRL, ROW_MASK, BOUND_ZERO>(w.y)));
+    }
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave (lane order; all lanes
+// active): row_shr 1, 2, 4, 8 scan each row of 16, then row_bcast:15 adds
+// row 0's total to row 1 and row 2's to row 3, and row_bcast:31 adds lane
+// 31's to rows 2 and 3.  Zero bits are 0 for integers and +0.0 for floats.
+template <typename T>
+__device__ inline T wave_inclusive_scan(T v) {
+    v += dpp_move<DPP_ROW_SHR1, 0xf, true>(v);
+    v += dpp_move<DPP_ROW_SHR2, 0xf, true>(v);
+    v += dpp_move<DPP_ROW_SHR4, 0xf, true>(v);
+    v += dpp_move<DPP_ROW_SHR8, 0xf, true>(v);
+    v += dpp_move<DPP_ROW_BCAST15, 0xa, false>(v);
+    v += dpp_move<DPP_ROW_BCAST31, 0xc, false>(v);
     return v;
 }
 
