@@ -974,21 +974,23 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
     for (int part = 0; part < G::PARTS; ++part) {
     const uint8_t *wt = part ? wt_lo : wt_hi, *in = part ? in_lo : in_hi;
     uint8_t *wsp = wsm + part * G::WBYTES, *xsp = xsm + part * xbytes;
-    // filter: 64 rows of 576 ESZ bytes in 16-byte chunks, all of a thread's loads issued first
-    {
-        constexpr int U = (G::WCHUNKS + DC_THREADS - 1) / DC_THREADS, RC = 576 * G::ESZ / 16;
-        i32x4 v[U];
+    // filter: 64 rows of 576 ESZ bytes in 16-byte chunks; a thread's loads are
+    // issued first and written to LDS after the input's first loads are issued
+    // too, so that both fetches are in flight together
+    constexpr int U = (G::WCHUNKS + DC_THREADS - 1) / DC_THREADS, RC = 576 * G::ESZ / 16;
+    i32x4 fv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1);  // past the end: the last chunk again
-            v[u] = *(const i32x4 *)(wt + (size_t)e * 16);
-        }
+    for (int u = 0; u < U; ++u) {
+        const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1);  // past the end: the last chunk again
+        fv[u] = *(const i32x4 *)(wt + (size_t)e * 16);
+    }
+    auto put_filter = [&]() {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = min(u * DC_THREADS + tid, G::WCHUNKS - 1), row = e / RC;
-            *(i32x4 *)(wsp + row * G::WROW + 16 * (e - row * RC)) = v[u];
+            *(i32x4 *)(wsp + row * G::WROW + 16 * (e - row * RC)) = fv[u];
         }
-    }
+    };
     // input: chunk e = (slot, 16-byte part c) of the nrows x W2 pixel slots
     constexpr int PC = G::PIXB / 16;
     const uint8_t *img = in + (size_t)n * HW * G::PIXB;
@@ -1006,6 +1008,7 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
             v[u] = in_img ? *(const i32x4 *)(img + ((size_t)iy * W + ix) * G::PIXB + 16 * cpart[u])
                           : i32x4{0, 0, 0, 0};
         }
+        if (e0 == 0) put_filter();
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsp + slot[u] * G::PXB + 16 * cpart[u]) = v[u];
