@@ -270,11 +270,14 @@ __device__ inline Col pub_sum(const uint64_t *p, long long stride, int n, uint32
 // (nearest first): the sum of their aggregates up to and including the
 // nearest inclusive value.  prefetch() loads the nearest one ahead of the
 // work that precedes resolve() (it usually already shows its inclusive value
-// or soon does); further predecessors go out in windows of LB_WIN, so a tile
-// rarely reads more than the one value per column.
+// or soon does); further predecessors are read one at a time, LB_WIN = 1:
+// wider windows (2, 4, 8, 16 measured on the 1080p image and the 256-image
+// batch) read words a nearer tile's inclusive value makes unnecessary, and
+// the extra traffic cost more than the round trips they save (1080p u8,
+// 32-row tiles: 11.5 us at 1, 12.0 at 2, 13.0 at 8, 13.8 at 16).
 template <class Col>
 struct Lookback {
-    static constexpr int LB_WIN = 8;
+    static constexpr int LB_WIN = 1;
     PubVal<Col> v[LB_WIN];
     __device__ inline void prefetch(const uint64_t *p, long long stride, int n) {
         if (n > 0) v[0].load(p - stride);
