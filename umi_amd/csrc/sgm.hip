@@ -32,7 +32,10 @@
 namespace {
 using namespace umi;
 
-constexpr uint32_t SGM_INF = 0x3fffffffu;
+// Added to the running cost of lanes past the last disparity: their L_r stays
+// in [2^29, 2^29 + Cmax + P2], above every real value (< 2^16), so it never
+// wins a minimum and needs no select per step.
+constexpr uint32_t SGM_BIAS = 0x20000000u;
 constexpr int SGM_U = 8;      // steps per block; loads run one block ahead
 constexpr int SGM_MAXK = 3;   // window radius limit (2K + 1 bytes fit one 8-byte word)
 
@@ -175,7 +178,9 @@ struct LineWalk {
             roffb[k] = 8u * (VERT ? (uint32_t)(clampi(line + sk, -g.KX - 1, g.W + g.KX) + g.KX + 1)
                                   : (uint32_t)(sk - g.smin));
         }
-        laneb = 4u * (uint32_t)lane;
+        // lanes without a valid disparity address P at 2^31, outside the
+        // buffer: their stores are dropped and their loads read 0
+        laneb = valid[0] ? 4u * (uint32_t)lane : 0x80000000u;
         // the host keeps both arrays below 2^31 bytes
         const int kbytes = (int)(8 * (VERT ? g.pack_v : g.pack_h));
         rL = __builtin_amdgcn_make_buffer_rsrc((void *)Lk, 0, kbytes, 0x00020000);
@@ -215,8 +220,8 @@ struct LineWalk {
         const uint32_t mp = M + (uint32_t)a.P2, p1 = (uint32_t)a.P1;
         const uint32_t b0 = min(min(Lp[0], n0 + p1), mp) - M;
         const uint32_t b1 = min(min(Lp[1], n1 + p1), mp) - M;
-        Lp[0] = valid[0] ? c0 + b0 : SGM_INF;
-        Lp[1] = valid[1] ? c1 + b1 : SGM_INF;
+        Lp[0] = c0 + b0;
+        Lp[1] = c1 + b1;
         M = wave_min_all(min(Lp[0], Lp[1]));
     }
 
@@ -226,7 +231,7 @@ struct LineWalk {
         // fills the unused upper half.  No carry between the halves: each
         // stays below 2^16
         const uint32_t packed = Lp[0] | (Lp[1] << 16);
-        if (valid[0]) pstore(p, MODE == SGM_ADD ? pv + packed : packed);
+        pstore(p, MODE == SGM_ADD ? pv + packed : packed);
     }
 
     // steps [sa, sb_end) of the walk (step 0 is the first pixel of the path).
@@ -238,13 +243,14 @@ struct LineWalk {
         if (sa == 0) {
             const int p = pos(0);
             const uint32_t pv = NEEDP ? pload(p) : 0u;
-            c[0] = c[1] = 0;
+            c[0] = valid[0] ? 0u : SGM_BIAS;
+            c[1] = valid[1] ? 0u : SGM_BIAS;
             for (int j = -KA; j <= KA; ++j) {
                 c[0] += win(p + j, 0);
                 c[1] += win(p + j, 1);
             }
-            Lp[0] = valid[0] ? c[0] : SGM_INF;
-            Lp[1] = valid[1] ? c[1] : SGM_INF;
+            Lp[0] = c[0];
+            Lp[1] = c[1];
             M = wave_min_all(min(Lp[0], Lp[1]));
             emit<MODE>(p, pv);
             sa = 1;
