@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip every CPU leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
     ap.add_argument("--only", default="", help="comma-separated secondary sections to run (default: all)")
+    ap.add_argument("--warm-s", type=float, default=0.3,
+                    help="seconds of back-to-back calls before a steady-state timing (GEMM, conv)")
     return ap.parse_args()
 
 
@@ -63,10 +65,15 @@ def timed(fn, st, n, w=3):
     return t.ms() / n
 
 
-def timed_steady(fn, st, n, warm_s=0.3):
-    """timed() after about warm_s seconds of back-to-back calls, so that the
-    engine clock has settled at its sustained value for this load (MFMA-bound
-    kernels run below the peak clock once the power limit is reached)."""
+WARM_S = 0.3  # --warm-s
+
+
+def timed_steady(fn, st, n, warm_s=None):
+    """timed() after about warm_s seconds (default --warm-s) of back-to-back
+    calls, so that the engine clock has settled at its sustained value for
+    this load (MFMA-bound kernels run below the peak clock once the power
+    limit is reached)."""
+    warm_s = WARM_S if warm_s is None else warm_s
     ms = timed(fn, st, 3, 1)
     return timed(fn, st, n, max(3, int(warm_s * 1e3 / max(ms, 1e-3))))
 
@@ -553,6 +560,8 @@ def extra_configs(st, only, cpu):
 
 def main():
     a = parse()
+    global WARM_S
+    WARM_S = a.warm_s
     if a.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {a.gpus}); the frame-band sharding of the "
               "search across GPUs is a design, not run here.", file=sys.stderr)
