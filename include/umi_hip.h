@@ -552,6 +552,28 @@ umi_status umi_me_refine_workspace(int H, int W, int rounds, size_t *bytes);
 umi_status umi_me_refine(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R, const int32_t *prop,
                          int iters, int local_rounds, int rounds, float slack, int32_t *mv_sad, int32_t *fb_list,
                          int32_t *fb_count, void *ws, size_t ws_bytes, umi_stream s);
+/* The same refinement in one launch, with every one of the `rounds` global
+ * rounds run (no stop after a round without a change) -- no counterpart in
+ * the reference.  A workgroup per tile of 4 x 8 blocks keeps the tile's ref
+ * region in LDS from the seed to the check; block states pass between tiles
+ * as tagged words in the workspace, and the fallback blocks' full search
+ * runs in the same launch.  Tiles wait on each other, so the launch needs
+ * every tile resident at once: UMI_ERR_UNSUPPORTED when more than 1024 tiles
+ * ((H/8 / 4) x (W/8 / 8), rounded up), more than 4/5 of the device's
+ * resident workgroups, or rounds > 32.  Calls that overlap in time with
+ * other kernels that wait on the GPU (another fused refinement on another
+ * stream) may time out.  ws: umi_me_refine_fused_workspace bytes, zeroed
+ * before its first use and then kept for frames of this size (every call
+ * leaves it consistent).  Outputs as umi_me_refine; fb_list is in tile
+ * order.  Every wait is bounded (0.2 s); umi_me_refine_fused_status reports
+ * (and clears) a wait that ran out, and copies the round change flags of the
+ * last call. */
+umi_status umi_me_refine_fused_workspace(int H, int W, int rounds, size_t *bytes);
+umi_status umi_me_refine_fused(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R,
+                               const int32_t *prop, int iters, int local_rounds, int rounds, float slack,
+                               int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count, void *ws, size_t ws_bytes,
+                               umi_stream s);
+umi_status umi_me_refine_fused_status(void *ws, int rounds, int32_t *changed, int *timed_out);
 
 #ifdef __cplusplus
 }

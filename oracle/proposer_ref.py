@@ -184,17 +184,20 @@ def _descend(B, c, active, R, iters):
     return c, sad, unconv
 
 
-def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile):
+def _rounds(B, mv, sad, unconv, R, iters, n, jumps, tile, all_rounds=False):
     """Up to n propagation rounds (Jacobi: every block reads the states from
     before the round).  Round p tries the 8 neighbours at distance 1, then at
     each distance of jumps[p % len] (inside the tile when tile is set); a
     block descends from the first minimum when it beats its own SAD.  With
     tile set each tile stops after its first round without a change, else
-    all stop together.  Returns (mv, sad, unconv, changed flags)."""
+    all stop together; all_rounds runs all n (umi_me_refine_fused).
+    Returns (mv, sad, unconv, changed flags)."""
     tid = B.tile_id() if tile else np.zeros(B.n, np.int64)
     active = np.ones(tid.max() + 1, bool)
     changed = []
     for p in range(n):
+        if all_rounds:
+            active[:] = True
         if not active.any():
             break
         cands = []
@@ -230,11 +233,12 @@ def full_search_block(cur, ref, by, bx, R):
     return i // S - R, i % S - R, int(sads.flat[i])
 
 
-def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
+def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack, all_rounds=False):
     """Refinement of block proposals restating umi_me_refine: seed, descent,
     in-tile rounds, global rounds, fallback check, full search of the
-    fallback blocks.  Returns (mv_sad [n][3], fallback set, changed flags of
-    the global rounds)."""
+    fallback blocks.  all_rounds: every global round runs, as in
+    umi_me_refine_fused.  Returns (mv_sad [n][3], fallback set, changed flags
+    of the global rounds)."""
     B = _Blocks(cur, ref)
     prop = np.clip(np.asarray(prop, np.int64).reshape(B.n, 2), -R, R)
     seeds = [(np.ones(B.n, bool), prop)]
@@ -244,7 +248,7 @@ def refine(cur, ref, prop, R, iters, local_rounds, rounds, slack):
     _, c = B.first_min(seeds)
     mv, sad, unconv = _descend(B, c, np.ones(B.n, bool), R, iters)
     mv, sad, unconv, _ = _rounds(B, mv, sad, unconv, R, iters, local_rounds, LOCAL_JUMPS, True)
-    mv, sad, unconv, changed = _rounds(B, mv, sad, unconv, R, iters, rounds, GLOBAL_JUMPS, False)
+    mv, sad, unconv, changed = _rounds(B, mv, sad, unconv, R, iters, rounds, GLOBAL_JUMPS, False, all_rounds)
     changed += [False] * (rounds - len(changed))
     nbmin = np.full(B.n, 2 ** 32 - 1, np.int64)
     for o in RING:

@@ -194,3 +194,40 @@ def test_refine_fallback_threshold_zero_forces_full_search():
     assert fb == set(range(64))
     full, _ = oracle.me_full_search(cur, ref, 8, 8)
     np.testing.assert_array_equal(out, full.reshape(64, 3))
+
+
+def test_refine_all_rounds_schedule():
+    """all_rounds (umi_me_refine_fused's schedule) runs and reports every global
+    round.  A round only ever lowers a block's SAD, so against the schedule
+    that stops after an idle round every block that schedule kept has an
+    equal or lower SAD; where the stopping schedule ran every round both
+    agree exactly; on the exact-shift frame (everything at SAD 0 once found)
+    both agree."""
+    from umi_amd import workloads
+    H, W, R = 96, 136, 16
+    cur, ref = workloads.smooth_motion_pair(H, W, R, seed=7)
+    n = (H // 8) * (W // 8)
+    prop = np.random.Generator(np.random.PCG64(8)).integers(-R, R + 1, size=(n, 2))
+    early, fb_e, ch_e = P.refine(cur, ref, prop, R, iters=4, local_rounds=0, rounds=6, slack=1.1)
+    full, fb_a, ch_a = P.refine(cur, ref, prop, R, iters=4, local_rounds=0, rounds=6, slack=1.1, all_rounds=True)
+    assert len(ch_a) == 6
+    kept = np.array([b not in fb_e for b in range(n)])
+    assert (full[kept, 2] <= early[kept, 2]).all()
+    k = ch_e.index(False) if False in ch_e else 6
+    assert ch_a[:k] == ch_e[:k]
+    one, fb1, ch1 = P.refine(cur, ref, prop, R, iters=4, local_rounds=0, rounds=1, slack=1.1)
+    one_a, fb1a, ch1a = P.refine(cur, ref, prop, R, iters=4, local_rounds=0, rounds=1, slack=1.1, all_rounds=True)
+    np.testing.assert_array_equal(one, one_a)
+    assert fb1 == fb1a and ch1 == ch1a
+    # exact shift: both schedules end with every inner block at (5, -7), SAD 0
+    H, W, R = 128, 192, 24
+    ref = _noise(H, W, 6)
+    yy, xx = np.mgrid[0:H, 0:W]
+    cur = ref[np.clip(yy + 5, 0, H - 1), np.clip(xx - 7, 0, W - 1)]
+    n = (H // 8) * (W // 8)
+    prop = np.tile(np.array([[-20, 20]]), (n, 1))
+    prop[37] = (5, -7)
+    a, fa, ca = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=16, slack=1.1, all_rounds=True)
+    e, fe, ce = P.refine(cur, ref, prop, R, iters=8, local_rounds=0, rounds=16, slack=1.1)
+    np.testing.assert_array_equal(a, e)
+    assert fa == fe and len(ca) == 16 and not all(ca) and ca[:ce.index(False)] == ce[:ce.index(False)]

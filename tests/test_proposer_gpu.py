@@ -261,6 +261,97 @@ def test_refine_matches_oracle(gpu, case):
     assert changed == want_changed
 
 
+def _refine_fused_gpu(cur, ref, props, R, iters, local_rounds, rounds, slack, pitch=None):
+    """umi_me_refine_fused on one zeroed workspace for each proposal set in
+    `props` in turn (consecutive calls on one workspace); per call (mv_sad,
+    fallback list, round change flags)."""
+    H, W = cur.shape
+    pitch = pitch or W
+    n = (H // 8) * (W // 8)
+    pad = lambda f: np.pad(f, ((0, 0), (0, pitch - W)))  # noqa: E731
+    dc, dr = DeviceBuffer.from_array(pad(cur)), DeviceBuffer.from_array(pad(ref))
+    mv, fl, fc = DeviceBuffer(n * 12), DeviceBuffer(n * 4), DeviceBuffer(4)
+    nbytes = C.c_size_t()
+    call("umi_me_refine_fused_workspace", H, W, rounds, C.byref(nbytes))
+    ws = DeviceBuffer.zeros(nbytes.value)
+    out = []
+    for i, prop in enumerate(props):
+        dp = DeviceBuffer.from_array(np.ascontiguousarray(prop, np.int32))
+        print(f"[refine fused] call {i}: H={H} W={W} pitch={pitch} R={R} iters={iters} local={local_rounds} "
+              f"rounds={rounds} slack={slack}", flush=True)
+        mv.fill(0)
+        call("umi_me_refine_fused", dc.ptr, dr.ptr, H, W, pitch, R, dp.ptr, iters, local_rounds, rounds,
+             C.c_float(slack), mv.ptr, fl.ptr, fc.ptr, ws.ptr, ws.nbytes, None)
+        got = mv.download((n, 3), np.int32)
+        k = int(fc.download((1,), np.int32)[0])
+        fb = fl.download((max(k, 1),), np.int32)[:k].tolist()
+        changed = (C.c_int32 * max(rounds, 1))()
+        timed_out = C.c_int()
+        call("umi_me_refine_fused_status", ws.ptr, rounds, C.cast(changed, C.c_void_p), C.byref(timed_out))
+        assert timed_out.value == 0, "a wait of the fused refinement ran out"
+        out.append((got, fb, [bool(changed[p]) for p in range(rounds)]))
+    return out
+
+
+@pytest.mark.parametrize("case", [
+    dict(H=96, W=136, R=32, iters=8, local=8, rounds=8, slack=1.1, kind="tiles"),
+    dict(H=100, W=138, R=16, iters=8, local=3, rounds=8, slack=1.1, kind="smooth"),    # ragged, unaligned rows
+    dict(H=96, W=136, R=16, iters=8, local=0, rounds=8, slack=1.1, kind="smooth", pitch=140),
+    dict(H=256, W=384, R=24, iters=8, local=8, rounds=3, slack=1.1, kind="smooth"),
+    dict(H=256, W=384, R=24, iters=8, local=8, rounds=12, slack=1.1, kind="tiles"),
+    dict(H=64, W=64, R=8, iters=1, local=0, rounds=0, slack=1.1, kind="smooth"),       # no rounds, one step
+    dict(H=64, W=72, R=8, iters=4, local=2, rounds=3, slack=0.0, kind="smooth"),       # slack 0 forces fallbacks
+    dict(H=200, W=264, R=64, iters=8, local=8, rounds=4, slack=1.1, kind="smooth"),    # largest radius
+    dict(H=200, W=264, R=32, iters=4, local=4, rounds=4, slack=1.1, kind="tiles"),     # the Proposer defaults
+])
+def test_refine_fused_matches_oracle(gpu, case):
+    """The one-launch refinement against the oracle's every-round schedule, on
+    two consecutive calls on one workspace (the second with other proposals,
+    then the first again: the workspace's epochs carry over)."""
+    H, W, R = case["H"], case["W"], case["R"]
+    if case["kind"] == "tiles":
+        cur, ref, _ = workloads.me_pair(H, W, min(R, 16), seed=9)
+    else:
+        cur, ref = workloads.smooth_motion_pair(H, W, R, seed=H + W)
+    n = (H // 8) * (W // 8)
+    g = workloads.rng(10)
+    props = []
+    for _ in range(2):
+        prop = g.integers(-R - 3, R + 4, size=(n, 2)).astype(np.int32)   # out-of-range proposals are clamped
+        prop[::3] = 0
+        props.append(prop)
+    props.append(props[0])
+    res = _refine_fused_gpu(cur, ref, props, R, case["iters"], case["local"], case["rounds"], case["slack"],
+                            case.get("pitch"))
+    for i, (got, fb, changed) in enumerate(res[:2]):
+        want, want_fb, want_changed = PR.refine(cur, ref, props[i], R, case["iters"], case["local"], case["rounds"],
+                                                case["slack"], all_rounds=True)
+        np.testing.assert_array_equal(got, want, err_msg=f"call {i}")
+        assert len(fb) == len(set(fb)) and set(fb) == want_fb
+        assert changed == want_changed
+    np.testing.assert_array_equal(res[2][0], res[0][0])
+    assert res[2][1] == res[0][1]
+
+
+def test_refine_fused_full_frame_equals_multi_launch(gpu):
+    """1920x1080 (1020 tiles, the whole grid resident), +-32, the Proposer's
+    settings: the one-launch refinement equals the multi-launch one whenever
+    the latter ran every round (no idle round before the last), which the
+    test checks; the oracle of this frame takes ~20 s, so the small cases
+    above carry the oracle comparison."""
+    H, W, R = 1080, 1920, 32
+    cur, ref = workloads.smooth_motion_pair(H, W, R, seed=H + W)
+    n = (H // 8) * (W // 8)
+    prop = workloads.rng(10).integers(-R - 3, R + 4, size=(n, 2)).astype(np.int32)
+    prop[::3] = 0
+    multi, fb_m, ch_m, _ = _refine_gpu(cur, ref, prop, R, 4, 4, 4, 1.1)
+    assert False not in ch_m[:-1], f"an idle round before the last: {ch_m}"
+    res = _refine_fused_gpu(cur, ref, [prop, prop], R, 4, 4, 4, 1.1)
+    for got, fb, changed in res:
+        np.testing.assert_array_equal(got, multi)
+        assert sorted(fb) == sorted(fb_m) and changed == ch_m
+
+
 def test_listed_search_completes_fallbacks(gpu):
     H, W, R = 96, 136, 32
     cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)

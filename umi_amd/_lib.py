@@ -105,6 +105,10 @@ SIGNATURES = {
     "umi_me_proposals": [P, c_int, c_int, c_int, P, P],
     "umi_me_refine_workspace": [c_int, c_int, c_int, P],
     "umi_me_refine": [P, P, c_int, c_int, c_int, c_int, P, c_int, c_int, c_int, c_float, P, P, P, P, c_size_t, P],
+    "umi_me_refine_fused_workspace": [c_int, c_int, c_int, C.POINTER(c_size_t)],
+    "umi_me_refine_fused": [P, P, c_int, c_int, c_int, c_int, P, c_int, c_int, c_int, c_float, P, P, P, P, c_size_t,
+                            P],
+    "umi_me_refine_fused_status": [P, c_int, P, C.POINTER(c_int)],
     "umi_conv_fprop": [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P, P, P, P, P, P, P,
                        P],
 }

@@ -1125,26 +1125,16 @@ __device__ inline uint32_t candidates_min(const RefineArgs &a, const uint32_t (&
     return group_min(key);
 }
 
-// Seed kernel: per tile, stage the ref region; per block the first minimum
-// over its own and its 8 neighbours' proposals (candidates 0..8), the
-// descent, then up to local_rounds in-tile rounds (Jacobi over the tile's
-// states in LDS; candidates are the neighbours at distance 1 and
-// local_jump(p) inside the tile; the tile stops after a round without a
-// change).  Also clears the round flags and the fallback count.
-__global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, int local_rounds,
-                                                                 const int32_t *__restrict__ prop,
-                                                                 RefState *__restrict__ out, int32_t *changed,
-                                                                 int rounds, int32_t *fb_count) {
-    extern __shared__ uint32_t lds[];
-    __shared__ RefState tstate[RT_Y * RT_X];
-    const int tile = tile_of_workgroup();
-    if (blockIdx.x == 0) {
-        if ((int)threadIdx.x < rounds) changed[threadIdx.x] = 0;
-        if (threadIdx.x == 0) *fb_count = 0;
-    }
-    if (tile >= a.ntiles) return;  // whole workgroup
-    uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
-    const BlockCtx c = block_ctx(a, tile);
+// Seed of one tile: stage the ref region; per block the first minimum over
+// its own and its 8 neighbours' proposals (candidates 0..8), the descent,
+// then up to local_rounds in-tile rounds (Jacobi over the tile's states in
+// LDS; candidates are the neighbours at distance 1 and local_jump(p) inside
+// the tile; the tile stops after a round without a change).  All threads of
+// the workgroup call it (it contains barriers); it returns the block's state
+// (every lane of the block holds it) and the region offset in xoff.
+__device__ inline RefState seed_tile(const RefineArgs &a, int local_rounds, const int32_t *__restrict__ prop,
+                                     uint32_t *reg, uint32_t *ctile, RefState *tstate, const BlockCtx &c, int tile,
+                                     int &xoff) {
     const int nby = a.H / 8, nbx = a.W / 8;
     int my[2], mx[2];
     bool ok[2];
@@ -1158,7 +1148,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, i
         my[j] = clampi(prop[2 * i], -a.R, a.R);
         mx[j] = clampi(prop[2 * i + 1], -a.R, a.R);
     }
-    const int xoff = fill_tile(reg, ctile, a, (tile % a.tiles_x) * RT_X, (tile / a.tiles_x) * RT_Y);
+    xoff = fill_tile(reg, ctile, a, (tile % a.tiles_x) * RT_X, (tile / a.tiles_x) * RT_Y);
     __syncthreads();
     const LdsSad sad = lds_sad(reg, a, xoff, c.ty, c.tx);
     uint32_t cw[16];
@@ -1198,6 +1188,26 @@ __global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, i
         if (c.l == 0) tstate[g] = st;
         if (!__syncthreads_or(improved)) break;  // uniform
     }
+    return st;
+}
+
+// Seed kernel: seed_tile per tile, the state to `out`.  Also clears the
+// round flags and the fallback count.
+__global__ __launch_bounds__(RT_THREADS) void refine_seed_kernel(RefineArgs a, int local_rounds,
+                                                                 const int32_t *__restrict__ prop,
+                                                                 RefState *__restrict__ out, int32_t *changed,
+                                                                 int rounds, int32_t *fb_count) {
+    extern __shared__ uint32_t lds[];
+    __shared__ RefState tstate[RT_Y * RT_X];
+    const int tile = tile_of_workgroup();
+    if (blockIdx.x == 0) {
+        if ((int)threadIdx.x < rounds) changed[threadIdx.x] = 0;
+        if (threadIdx.x == 0) *fb_count = 0;
+    }
+    if (tile >= a.ntiles) return;  // whole workgroup
+    const BlockCtx c = block_ctx(a, tile);
+    int xoff;
+    const RefState st = seed_tile(a, local_rounds, prop, lds, lds + a.rows * a.stride4, tstate, c, tile, xoff);
     if (c.valid && c.l == 0) out[c.b] = st;
 }
 
@@ -1310,6 +1320,342 @@ __global__ __launch_bounds__(RT_THREADS) void refine_final_kernel(RefineArgs a, 
         fb = own.unconv || worse;
     }
     emit(c, own, fb, mv_sad, fb_list, fb_count);
+}
+
+// ---------------------------------------------------------------- fused refinement
+// The refinement in one launch: seed, every global round, the check and the
+// fallback blocks' full search.  A workgroup per tile keeps the tile's ref
+// region in LDS for the whole call.  Block states pass between tiles as
+// tagged 64-bit words (tag << 32 | packed state), written and read as relaxed
+// agent-scope atomics, which are coherent across the XCDs' L2 caches; the
+// tag is the call's epoch + 1.  A lane that needs a neighbour's state of
+// round p polls that word until it carries the tag, so a tile waits only for
+// the tiles whose states it reads (no barrier across the grid).  Every tile
+// runs every global round (no stop after a round without a change).  The
+// check publishes one tagged word per tile with its fallback blocks; once a
+// workgroup (the grid's padding included) has seen every tile's word, it
+// takes the fallback blocks k = its id, + grid, ... of the grid's list (tile
+// order, then block order in the tile) and runs their full search over
+// [-R, R]^2.  Each workgroup keeps its own epoch word, read at the start and
+// advanced at the end, so all agree on the tag without a reset between
+// calls.  Every wait is bounded by the wall clock; a wait that runs out sets
+// the workspace's timeout word and the kernel still ends.  Tiles wait on
+// each other, so the whole grid must be resident at once: the launcher
+// checks the occupancy first.
+constexpr long long RF_WAIT_TICKS = 20000000;  // 0.2 s of the 100 MHz wall clock per wait
+constexpr int RF_MAX_TILES = 4 * RT_THREADS;   // the listed phase scans 4 tiles per thread
+constexpr int RF_MAX_ROUNDS = 32;              // improved-round bits per tile word
+
+struct FusedWs {
+    uint32_t *ctl;  // [0] timeout, [1 + p] round p changed some block (written by the last workgroup)
+    uint32_t *ep;   // epoch per workgroup (tiles, then the grid's padding)
+    uint64_t *st;   // (rounds + 1) x nb state words: seed, then after each round
+    uint64_t *sum;  // per tile: fallback-block word, then (ntiles on) improved-rounds word
+};
+
+// Packed state: dx + 64 (8 bits) | dy + 64 (8) | SAD (15) | unconv (1).
+__device__ inline uint64_t state_word(uint32_t tag, const RefState &s) {
+    return ((uint64_t)tag << 32) | (uint32_t)(s.dx + 64) | ((uint32_t)(s.dy + 64) << 8) | ((uint32_t)s.sad << 16) |
+           ((uint32_t)s.unconv << 31);
+}
+__device__ inline RefState word_state(uint64_t w) {
+    const uint32_t v = (uint32_t)w;
+    return {(int)((v >> 8) & 0xffu) - 64, (int)(v & 0xffu) - 64, (int)((v >> 16) & 0x7fffu), (int)(v >> 31)};
+}
+__device__ inline void put_word(uint64_t *p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t get_word(const uint64_t *p) {
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Polls the words p[i] with need[i] (already loaded once into w[i]) until
+// each carries `tag`, reloading all those still stale in one round trip per
+// try, for at most RF_WAIT_TICKS, and not at all once some wait of the grid
+// has run out (so that a launch whose grid is not resident still ends within
+// one timeout).
+template <int N>
+__device__ inline void await_words(const uint64_t *const (&p)[N], uint64_t (&w)[N], const bool (&need)[N],
+                                   uint32_t tag, uint32_t *timeout) {
+    const auto stale = [&]() {
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < N; ++i) any = any || (need[i] && (uint32_t)(w[i] >> 32) != tag);
+        return any;
+    };
+    if (!stale()) return;
+    const uint64_t t0 = wall_clock64();
+    bool late;
+    do {
+        __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (need[i] && (uint32_t)(w[i] >> 32) != tag) w[i] = get_word(p[i]);
+        late = (long long)(wall_clock64() - t0) >= RF_WAIT_TICKS ||
+               __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    } while (stale() && !late);
+    if (stale()) __hip_atomic_store(timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Full search of block (by, bx) over [-R, R]^2 by the whole workgroup (all
+// threads call it): the (8 + 2R)-row clamped window in LDS; thread t takes
+// the 4 horizontal offsets 4 (t % G) .. + 3 (v_qsad_pk_u16_u8) of the
+// vertical offsets of chunk t / G, two at a time (one pass over 9 window
+// rows feeds both); the first minimum in raster order as the minimum of
+// (SAD << idx_bits | dyi S + dxi), per wave by shuffles, then an LDS atomic.
+__device__ inline void full_search_one(const RefineArgs &a, int by, int bx, uint32_t *win, uint32_t *key,
+                                       int32_t *__restrict__ mv_sad) {
+    const int R = a.R, S = 2 * R + 1, G = ceil_div(S, 4), nch = RT_THREADS / G, DCH = ceil_div(S, nch);
+    const int ws4 = ceil_div(8 + 2 * R + 4, 4), wrows = 8 + 2 * R;
+    const int idx_bits = 32 - __builtin_clz((unsigned)(S * S - 1));
+    const int x0 = 8 * bx, y0 = 8 * by, tid = threadIdx.x;
+    stage_clamped<4>(win, wrows * ws4, tid, RT_THREADS, a.W, [&](int k, const uint8_t *&row, int &gx) {
+        const int r = k / ws4;
+        row = a.ref + (size_t)clampi(y0 - R + r, 0, a.H - 1) * a.pitch;
+        gx = x0 - R + 4 * (k - r * ws4);
+    });
+    uint32_t cw[16];
+    global_cur(cw, a.cur, a.pitch, y0, x0);
+    if (tid == 0) *key = 0xffffffffu;
+    __syncthreads();
+    const int g = tid % G, ch = tid / G, d0 = ch * DCH, d1 = min(S, d0 + DCH);
+    uint32_t bestk = 0xffffffffu;
+    const auto close = [&](uint64_t acc, int dyi) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int dxi = 4 * g + q;
+            if (dxi < S)
+                bestk = min(bestk, (((uint32_t)(acc >> (16 * q)) & 0xffffu) << idx_bits) | (uint32_t)(dyi * S + dxi));
+        }
+    };
+    if (ch < nch) {
+        // vertical offsets in pairs: two independent qsad chains over 9 window rows
+        for (int dyi = d0; dyi < d1; dyi += 2) {
+            const bool two = dyi + 1 < d1;
+            uint64_t acc0 = 0, acc1 = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const uint32_t *wr = win + min(dyi + r, wrows - 1) * ws4 + g;
+                const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2];
+                const uint64_t s0 = (uint64_t)w0 | ((uint64_t)w1 << 32), s1 = (uint64_t)w1 | ((uint64_t)w2 << 32);
+                if (r < 8) {
+                    acc0 = __builtin_amdgcn_qsad_pk_u16_u8(s0, cw[2 * r], acc0);
+                    acc0 = __builtin_amdgcn_qsad_pk_u16_u8(s1, cw[2 * r + 1], acc0);
+                }
+                if (r > 0) {
+                    acc1 = __builtin_amdgcn_qsad_pk_u16_u8(s0, cw[2 * (r - 1)], acc1);
+                    acc1 = __builtin_amdgcn_qsad_pk_u16_u8(s1, cw[2 * (r - 1) + 1], acc1);
+                }
+            }
+            close(acc0, dyi);
+            if (two) close(acc1, dyi + 1);
+        }
+    }
+    // first the wave's minimum, then one LDS atomic per wave
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) bestk = min(bestk, (uint32_t)__shfl_xor((int)bestk, o, 64));
+    if ((tid & 63) == 0) atomicMin(key, bestk);
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t k = *key, idx = k & ((1u << idx_bits) - 1u);
+        int32_t *o = mv_sad + ((size_t)by * (a.W / 8) + bx) * 3;
+        o[0] = (int32_t)(idx / S) - R;
+        o[1] = (int32_t)(idx % S) - R;
+        o[2] = (int32_t)(k >> idx_bits);
+    }
+    __syncthreads();  // the window and key are reused by the next block
+}
+
+__global__ __launch_bounds__(RT_THREADS) void refine_fused_kernel(RefineArgs a, int local_rounds, int rounds,
+                                                                  float slack, const int32_t *__restrict__ prop,
+                                                                  FusedWs w, int32_t *__restrict__ mv_sad,
+                                                                  int32_t *__restrict__ fb_list,
+                                                                  int32_t *__restrict__ fb_count) {
+    extern __shared__ uint32_t lds[];
+    __shared__ RefState tstate[RT_Y * RT_X];
+    __shared__ uint32_t sh_fb, sh_chg, sh_key, sh_owner;
+    __shared__ uint32_t sh_part[RT_THREADS / 64];
+    const int tile = tile_of_workgroup();  // past a.ntiles: grid padding, which joins the fallback search only
+    const int nby = a.H / 8, nbx = a.W / 8, nb = nby * nbx, tid = threadIdx.x, lane = tid & 63;
+    const uint32_t tag = w.ep[tile] + 1u;  // every workgroup holds the same epoch
+    if (tile < a.ntiles) {
+        if (tid == 0) {
+            sh_fb = 0;
+            sh_chg = 0;
+        }
+        uint32_t *reg = lds, *ctile = lds + a.rows * a.stride4;
+        const BlockCtx c = block_ctx(a, tile);
+        int xoff;
+        RefState st = seed_tile(a, local_rounds, prop, reg, ctile, tstate, c, tile, xoff);
+        if (c.valid && c.l == 0) put_word(w.st + c.b, state_word(tag, st));
+        const LdsSad sad = lds_sad(reg, a, xoff, c.ty, c.tx);
+        uint32_t cw[16];
+        tile_cur(cw, ctile, c.ty, c.tx);
+
+        // global rounds: candidates from the neighbours' states after round p - 1
+        for (int p = 0; p < rounds; ++p) {
+            const uint64_t *in = w.st + (size_t)p * nb;
+            const uint64_t *src[3];
+            uint64_t wv[3];
+            bool ok[3];
+            int my[3], mx[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {  // all three loads in flight before the first wait
+                int n;
+                const bool inside = neighbour(c.by, c.bx, ring(c.l), j ? global_jump(p, j - 1) : 1, nby, nbx, n);
+                ok[j] = c.valid && inside;
+                src[j] = in + n;
+                wv[j] = ok[j] ? get_word(src[j]) : 0;
+            }
+            await_words<3>(src, wv, ok, tag, w.ctl);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const RefState n = word_state(wv[j]);
+                my[j] = ok[j] ? n.dy : st.dy;
+                mx[j] = ok[j] ? n.dx : st.dx;
+            }
+            bool improved = false;
+            if (c.valid) {
+                const uint32_t key = candidates_min<3>(a, cw, sad, c.l, my, mx, ok, true, st.dy, st.dx);
+                improved = (int)(key >> 5) < st.sad;  // no candidate: key >> 5 = 2^27 - 1
+                if (improved) {
+                    int cy, cx;
+                    winner<3>(key, my, mx, cy, cx);
+                    st = descend(a, cw, sad, c.l, cy, cx);
+                }
+                if (c.l == 0) put_word(w.st + (size_t)(p + 1) * nb + c.b, state_word(tag, st));
+            }
+            if (__ballot(improved) && lane == 0) atomicOr(&sh_chg, 1u << p);
+        }
+
+        // check against the 8 neighbours' final vectors
+        {
+            const uint64_t *fin = w.st + (size_t)rounds * nb;
+            int n1;
+            const bool ok1 = c.valid && neighbour(c.by, c.bx, ring(c.l), 1, nby, nbx, n1);
+            const uint64_t *src[1] = {fin + n1};
+            uint64_t wv[1] = {ok1 ? get_word(src[0]) : 0};
+            const bool need[1] = {ok1};
+            await_words<1>(src, wv, need, tag, w.ctl);
+            const RefState n = ok1 ? word_state(wv[0]) : st;
+            if (c.valid) {
+                uint32_t m = 0xffffffffu;
+                if (ok1 && (n.dy != st.dy || n.dx != st.dx)) m = sad(cw, a.R, n.dy, n.dx);
+                m = group_min(m);
+                const bool worse = (float)st.sad > slack * (float)min((uint32_t)st.sad, m);
+                const bool fb = st.unconv || worse;
+                if (c.l == 0) {
+                    if (fb) {
+                        atomicOr(&sh_fb, 1u << (c.ty * RT_X + c.tx));
+                    } else {  // fallback blocks are written by the full search only
+                        int32_t *o = mv_sad + (size_t)c.b * 3;
+                        o[0] = st.dy;
+                        o[1] = st.dx;
+                        o[2] = st.sad;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            put_word(w.sum + tile, ((uint64_t)tag << 32) | sh_fb);
+            put_word(w.sum + a.ntiles + tile, ((uint64_t)tag << 32) | sh_chg);
+        }
+    }
+
+    // every tile's fallback word (4 per thread, polled together); prefix
+    // counts over the tiles in LDS
+    uint32_t *masks = lds, *pre = lds + RF_MAX_TILES, *win = pre + RF_MAX_TILES + 1;
+    uint32_t cnt = 0;
+    {
+        const uint64_t *src[4];
+        uint64_t v[4];
+        bool need[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 4 * tid + k;
+            need[k] = i < a.ntiles;
+            src[k] = w.sum + (need[k] ? i : 0);
+            v[k] = need[k] ? get_word(src[k]) : ((uint64_t)tag << 32);
+        }
+        await_words<4>(src, v, need, tag, w.ctl);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            masks[4 * tid + k] = (uint32_t)v[k];
+            cnt += __popc((uint32_t)v[k]);
+        }
+    }
+    // exclusive scan of the threads' counts (4 tiles each)
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) sh_part[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < RT_THREADS / 64; ++k) {
+        base += k < (tid >> 6) ? sh_part[k] : 0u;
+        total += sh_part[k];
+    }
+    {
+        uint32_t run = base + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            pre[4 * tid + k] = run;
+            run += __popc(masks[4 * tid + k]);
+        }
+    }
+    __syncthreads();
+    const auto block_of = [&](int i, uint32_t m, int rank) {  // rank-th set bit of tile i's mask
+        for (int r = 0; r < rank; ++r) m &= m - 1;
+        const int gb = __ffs(m) - 1;
+        const int by = (i / a.tiles_x) * RT_Y + gb / RT_X, bx = (i % a.tiles_x) * RT_X + gb % RT_X;
+        return by * nbx + bx;
+    };
+    if (blockIdx.x == gridDim.x - 1) {  // the fallback list, its length and the rounds' change flags
+        const uint64_t *src[4];
+        uint64_t v[4];
+        bool need[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = 4 * tid + k;
+            need[k] = i < a.ntiles;
+            src[k] = w.sum + a.ntiles + (need[k] ? i : 0);
+            v[k] = need[k] ? get_word(src[k]) : ((uint64_t)tag << 32);
+        }
+        await_words<4>(src, v, need, tag, w.ctl);
+        uint32_t chg = (uint32_t)v[0] | (uint32_t)v[1] | (uint32_t)v[2] | (uint32_t)v[3];
+        for (int i = tid; i < a.ntiles; i += RT_THREADS) {
+            const uint32_t m = masks[i];
+            for (int r = 0; r < __popc(m); ++r) fb_list[pre[i] + r] = block_of(i, m, r);
+        }
+        if (tid == 0) {
+            *fb_count = (int32_t)total;
+            sh_chg = 0;
+        }
+        for (int o = 32; o >= 1; o >>= 1) chg |= (uint32_t)__shfl_xor((int)chg, o, 64);
+        __syncthreads();
+        if (lane == 0) atomicOr(&sh_chg, chg);
+        __syncthreads();
+        if (tid < rounds) w.ctl[1 + tid] = (sh_chg >> tid) & 1u;
+    }
+    // the fallback blocks k, k + grid, ... (consecutive k on consecutive
+    // workgroups, so on different XCDs); the tile holding item k is the one
+    // whose prefix range contains it (found by its owner thread)
+    for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = 4 * tid + j;
+            if (i < a.ntiles && pre[i] <= k && k < pre[i] + __popc(masks[i])) sh_owner = (uint32_t)i;
+        }
+        __syncthreads();
+        const int i = (int)sh_owner;
+        const int b = block_of(i, masks[i], (int)(k - pre[i]));
+        full_search_one(a, b / nbx, b % nbx, win, &sh_key, mv_sad);  // ends with a barrier
+    }
+    if (tid == 0) w.ep[tile] = tag;
 }
 
 }  // namespace
@@ -1526,6 +1872,87 @@ static size_t refine_ws_bytes(int H, int W, int rounds) {
 umi_status umi_me_refine_workspace(int H, int W, int rounds, size_t *bytes) {
     if (!bytes || H < 8 || W < 8 || rounds < 0) return set_error(UMI_ERR_INVALID, "umi_me_refine_workspace: bad args");
     *bytes = refine_ws_bytes(H, W, rounds);
+    return UMI_OK;
+}
+
+// The fused refinement's workspace: control words, per-tile epochs, the
+// (rounds + 1) state buffers, the per-tile summary words.
+struct FusedLayout {
+    size_t ctl, ep, st, sum, bytes;
+};
+static FusedLayout fused_layout(int H, int W, int rounds) {
+    const int nbx = W / 8, nby = H / 8, ntiles = ceil_div(nbx, RT_X) * ceil_div(nby, RT_Y);
+    FusedLayout L;
+    L.ctl = 0;
+    L.ep = 256;
+    L.st = L.ep + round_up((size_t)round_up(ntiles, 8) * 4, 256);
+    L.sum = L.st + round_up((size_t)nbx * nby * 8 * (rounds + 1), 256);
+    L.bytes = L.sum + round_up((size_t)ntiles * 16, 256);
+    return L;
+}
+
+umi_status umi_me_refine_fused_workspace(int H, int W, int rounds, size_t *bytes) {
+    if (!bytes || H < 8 || W < 8 || rounds < 0 || rounds > RF_MAX_ROUNDS)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine_fused_workspace: bad args");
+    *bytes = fused_layout(H, W, rounds).bytes;
+    return UMI_OK;
+}
+
+umi_status umi_me_refine_fused(const uint8_t *cur, const uint8_t *ref, int H, int W, int pitch, int R,
+                               const int32_t *prop, int iters, int local_rounds, int rounds, float slack,
+                               int32_t *mv_sad, int32_t *fb_list, int32_t *fb_count, void *ws, size_t ws_bytes,
+                               umi_stream s) {
+    if (!cur || !ref || !prop || !mv_sad || !fb_list || !fb_count || !ws)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine_fused: null pointer");
+    if (H < 8 || W < 8 || pitch < W) return set_error(UMI_ERR_INVALID, "umi_me_refine_fused: bad frame");
+    if (R < 1 || R > 64) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_refine_fused: radius %d (1..64)", R);
+    if (iters < 1 || iters > 64 || local_rounds < 0 || local_rounds > 64 || rounds < 0 || rounds > RF_MAX_ROUNDS)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine_fused: iters %d (1..64), local rounds %d, rounds %d (0..%d)",
+                         iters, local_rounds, rounds, RF_MAX_ROUNDS);
+    const FusedLayout L = fused_layout(H, W, rounds);
+    if (ws_bytes < L.bytes)
+        return set_error(UMI_ERR_INVALID, "umi_me_refine_fused: workspace %zu B < %zu B", ws_bytes, L.bytes);
+    const int nbx = W / 8, nby = H / 8;
+    RefineArgs a{cur, ref, H, W, pitch, R, iters, ceil_div(nbx, RT_X), 0, 0, 0, 0, 0};
+    a.ref16 = ((reinterpret_cast<uintptr_t>(ref) | (uintptr_t)pitch) & 15) == 0 && W >= 16;
+    a.cur16 = ((reinterpret_cast<uintptr_t>(cur) | (uintptr_t)pitch) & 15) == 0 && W >= 16;
+    a.ntiles = a.tiles_x * ceil_div(nby, RT_Y);
+    if (a.ntiles > RF_MAX_TILES)
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_me_refine_fused: %d tiles of 4 x 8 blocks (at most %d)", a.ntiles,
+                         RF_MAX_TILES);
+    region_shape(R, a.stride4, a.rows);
+    const size_t listed = 4 * (2 * (size_t)RF_MAX_TILES + 1) + 4 * (size_t)(8 + 2 * R) * ceil_div(8 + 2 * R + 4, 4);
+    const size_t lds = std::max(region_lds_bytes(R), listed);
+    // every tile waits on others: the grid must be resident at once (with a
+    // fifth of the occupancy the runtime reports kept in reserve)
+    const dim3 grid(round_up(a.ntiles, 8)), block(RT_THREADS);
+    int dev = 0, cus = 0, per_cu = 0;
+    UMI_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+    UMI_TRY(hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute"));
+    UMI_TRY(hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)refine_fused_kernel,
+                                                                   RT_THREADS, lds),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
+    if ((long)grid.x * 5 > (long)per_cu * cus * 4)
+        return set_error(UMI_ERR_UNSUPPORTED, "umi_me_refine_fused: %u workgroups, %d resident per CU x %d CUs",
+                         grid.x, per_cu, cus);
+    const FusedWs w{(uint32_t *)((char *)ws + L.ctl), (uint32_t *)((char *)ws + L.ep),
+                    (uint64_t *)((char *)ws + L.st), (uint64_t *)((char *)ws + L.sum)};
+    hipLaunchKernelGGL(refine_fused_kernel, grid, block, lds, as_stream(s), a, local_rounds, rounds, slack, prop, w,
+                       mv_sad, fb_list, fb_count);
+    return launch_check("refine_fused_kernel");
+}
+
+umi_status umi_me_refine_fused_status(void *ws, int rounds, int32_t *changed, int *timed_out) {
+    if (!ws || !timed_out || rounds < 0 || rounds > RF_MAX_ROUNDS || (rounds > 0 && !changed))
+        return set_error(UMI_ERR_INVALID, "umi_me_refine_fused_status: bad args");
+    uint32_t ctl[1 + RF_MAX_ROUNDS];
+    UMI_TRY(hip_check(hipMemcpy(ctl, ws, sizeof(ctl), hipMemcpyDeviceToHost), "hipMemcpy (refinement status)"));
+    *timed_out = ctl[0] != 0;
+    for (int p = 0; p < rounds; ++p) changed[p] = (int32_t)ctl[1 + p];
+    if (ctl[0]) {
+        const uint32_t z = 0;
+        UMI_TRY(hip_check(hipMemcpy(ws, &z, 4, hipMemcpyHostToDevice), "hipMemcpy (refinement status)"));
+    }
     return UMI_OK;
 }
 

@@ -263,9 +263,17 @@ def pack_layer(net: Network, li: int, stream=None) -> DeviceBuffer:
 class Proposer:
     """Proposal + refinement + fallback for 1920x1080 pairs with a trained network."""
 
-    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=4, local_rounds=4, rounds=4):
+    def __init__(self, net: Network, H=1080, W=1920, R=32, slack=1.1, iters=4, local_rounds=4, rounds=4,
+                 fused=None):
+        """fused: refine in one launch (umi_me_refine_fused, every global round
+        run) -- the default where it applies (at most 1024 tiles of 4 x 8
+        blocks and 32 rounds).  It needs the GPU's workgroup slots to itself,
+        so pipelines that run concurrently on several streams take fused=False
+        (umi_me_refine: one launch per stage, stopping after an idle round)."""
         self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
         self.iters, self.local_rounds, self.rounds = iters, local_rounds, rounds
+        ntiles = -(-(W // 8) // 8) * -(-(H // 8) // 4)
+        self.fused = (ntiles <= 1024 and rounds <= 32) if fused is None else fused
         self.H4, self.W4 = H // 4, W // 4
         px = self.H4 * self.W4
         self.nblk = (H // 8) * (W // 8)
@@ -277,8 +285,12 @@ class Proposer:
         self.fb_count = DeviceBuffer(4)
         self.packed = [pack_layer(net, li) for li in range(len(LAYERS))]
         n = C.c_size_t()
-        call("umi_me_refine_workspace", H, W, rounds, C.byref(n))
-        self.ws = DeviceBuffer(n.value)
+        if self.fused:
+            call("umi_me_refine_fused_workspace", H, W, rounds, C.byref(n))
+            self.ws = DeviceBuffer.zeros(n.value)  # zeroed once; every call leaves it consistent
+        else:
+            call("umi_me_refine_workspace", H, W, rounds, C.byref(n))
+            self.ws = DeviceBuffer(n.value)
 
     def propose(self, cur: DeviceBuffer, ref: DeviceBuffer, stream=None):
         """Network and block proposals: the first two layers read the frames
@@ -292,9 +304,9 @@ class Proposer:
              self.net.b[3].ptr, self.R, self.flow.ptr, self.prop.ptr, s)
 
     def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
-        call("umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, self.iters,
-             self.local_rounds, self.rounds, C.c_float(self.slack), mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr,
-             self.ws.ptr, self.ws.nbytes, _s(stream))
+        call("umi_me_refine_fused" if self.fused else "umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W,
+             self.R, self.prop.ptr, self.iters, self.local_rounds, self.rounds, C.c_float(self.slack), mv_sad.ptr,
+             self.fb_list.ptr, self.fb_count.ptr, self.ws.ptr, self.ws.nbytes, _s(stream))
 
     def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         """Refinement; the fallback blocks come back with the full search's result."""
@@ -312,13 +324,22 @@ class Proposer:
     def fallback_count(self) -> int:
         return int(self.fb_count.download((1,), np.int32)[0])
 
+    def status(self):
+        """(change flag of each global round of the last refinement, whether a
+        wait of the fused refinement ran out); synchronises the device."""
+        if not self.fused:
+            nb = self.nblk * 16
+            off = 2 * ((nb + 255) // 256 * 256)
+            raw = self.ws.download((self.ws.nbytes,), np.uint8)[off:off + 4 * self.rounds].tobytes()
+            return [bool(v) for v in np.frombuffer(raw, np.int32)], False
+        changed = (C.c_int32 * max(self.rounds, 1))()
+        late = C.c_int()
+        call("umi_me_refine_fused_status", self.ws.ptr, self.rounds, C.cast(changed, C.c_void_p), C.byref(late))
+        return [bool(changed[p]) for p in range(self.rounds)], bool(late.value)
+
     def rounds_run(self) -> int:
         """Propagation rounds of the last refinement that changed some block."""
-        nb = self.nblk * 16
-        off = 2 * ((nb + 255) // 256 * 256)
-        flags = np.frombuffer(self.ws.download((self.ws.nbytes,), np.uint8)[off:off + 4 * self.rounds].tobytes(),
-                              np.int32)
-        return int(np.count_nonzero(flags))
+        return sum(self.status()[0])
 
 
 def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10):
@@ -352,13 +373,17 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
         j = judge.download((prop.nblk, 3), np.int32)
         at_min.append(float(np.mean(a[:, 2] == j[:, 2])))
         fb.append(prop.fallback_count() / prop.nblk)
-        rounds.append(prop.rounds_run())
+        flags, late = prop.status()
+        if late:
+            raise RuntimeError("proposer: a wait of the fused refinement ran out")
+        rounds.append(sum(flags))
         times["pipeline"].append(timed(lambda: prop.run(cur, ref, mv, st, proposals)))
         if proposals:
             times["network"].append(timed(lambda: prop.propose(cur, ref, st)))
         times["refine"].append(timed(lambda: prop.refine_only(cur, ref, mv, st)))
         times["full_search"].append(timed(lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)))
-    out = {"at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
+    out = {"refinement": "one launch (umi_me_refine_fused)" if prop.fused else "one launch per stage (umi_me_refine)",
+           "at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
            "fallback_frac": float(np.mean(fb)), "rounds_changed": float(np.mean(rounds))}
     out.update({f"ms_{k}": float(np.mean(v)) for k, v in times.items() if v})
     out["speedup"] = out["ms_full_search"] / out["ms_pipeline"]
@@ -385,7 +410,7 @@ def throughput(net: "Network", pairs: int, seed0: int, reps: int = 20, R: int = 
     refs = [DeviceBuffer(H * W) for _ in range(pairs)]
     curs = [DeviceBuffer(H * W) for _ in range(pairs)]
     streams = [Stream() for _ in range(pairs)]
-    props = [Proposer(net, H, W, R=R) for _ in range(pairs)]
+    props = [Proposer(net, H, W, R=R, fused=False) for _ in range(pairs)]  # pipelines run concurrently
     nblk = props[0].nblk
     mvs = [DeviceBuffer(nblk * 12) for _ in range(pairs)]
     judges = [DeviceBuffer(nblk * 12) for _ in range(pairs)]
