@@ -1373,7 +1373,7 @@ __device__ inline uint64_t get_word(const uint64_t *p) {
 // try, for at most RF_WAIT_TICKS, and not at all once some wait of the grid
 // has run out (so that a launch whose grid is not resident still ends within
 // one timeout).
-template <int N>
+template <int N, int SLEEP = 2>
 __device__ inline void await_words(const uint64_t *const (&p)[N], uint64_t (&w)[N], const bool (&need)[N],
                                    uint32_t tag, uint32_t *timeout) {
     const auto stale = [&]() {
@@ -1386,7 +1386,7 @@ __device__ inline void await_words(const uint64_t *const (&p)[N], uint64_t (&w)[
     const uint64_t t0 = wall_clock64();
     bool late;
     do {
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(SLEEP);
 #pragma unroll
         for (int i = 0; i < N; ++i)
             if (need[i] && (uint32_t)(w[i] >> 32) != tag) w[i] = get_word(p[i]);
@@ -1408,7 +1408,7 @@ __device__ inline void full_search_one(const RefineArgs &a, int by, int bx, uint
     const int ws4 = ceil_div(8 + 2 * R + 4, 4), wrows = 8 + 2 * R;
     const int idx_bits = 32 - __builtin_clz((unsigned)(S * S - 1));
     const int x0 = 8 * bx, y0 = 8 * by, tid = threadIdx.x;
-    stage_clamped<4>(win, wrows * ws4, tid, RT_THREADS, a.W, [&](int k, const uint8_t *&row, int &gx) {
+    stage_clamped<8>(win, wrows * ws4, tid, RT_THREADS, a.W, [&](int k, const uint8_t *&row, int &gx) {
         const int r = k / ws4;
         row = a.ref + (size_t)clampi(y0 - R + r, 0, a.H - 1) * a.pitch;
         gx = x0 - R + 4 * (k - r * ws4);
@@ -1562,28 +1562,32 @@ __global__ __launch_bounds__(RT_THREADS) void refine_fused_kernel(RefineArgs a, 
         }
     }
 
-    // every tile's fallback word (4 per thread, polled together); prefix
-    // counts over the tiles in LDS
+    // every tile's fallback word, polled by the first wave only (16 words a
+    // lane, in groups of 4; fewer loads in flight while the last tiles run);
+    // prefix counts over the tiles in LDS
     uint32_t *masks = lds, *pre = lds + RF_MAX_TILES, *win = pre + RF_MAX_TILES + 1;
-    uint32_t cnt = 0;
-    {
-        const uint64_t *src[4];
-        uint64_t v[4];
-        bool need[4];
+    if (tid < 64) {
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t *src[4];
+            uint64_t v[4];
+            bool need[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = 4 * tid + k;
-            need[k] = i < a.ntiles;
-            src[k] = w.sum + (need[k] ? i : 0);
-            v[k] = need[k] ? get_word(src[k]) : ((uint64_t)tag << 32);
-        }
-        await_words<4>(src, v, need, tag, w.ctl);
+            for (int k = 0; k < 4; ++k) {
+                const int i = 256 * q + 4 * tid + k;
+                need[k] = i < a.ntiles;
+                src[k] = w.sum + (need[k] ? i : 0);
+                v[k] = need[k] ? get_word(src[k]) : ((uint64_t)tag << 32);
+            }
+            await_words<4, 8>(src, v, need, tag, w.ctl);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            masks[4 * tid + k] = (uint32_t)v[k];
-            cnt += __popc((uint32_t)v[k]);
+            for (int k = 0; k < 4; ++k) masks[256 * q + 4 * tid + k] = (uint32_t)v[k];
         }
     }
+    __syncthreads();
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cnt += __popc(masks[4 * tid + k]);
     // exclusive scan of the threads' counts (4 tiles each)
     uint32_t incl = cnt;
 #pragma unroll
