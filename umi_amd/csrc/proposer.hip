@@ -1345,6 +1345,11 @@ __global__ __launch_bounds__(RT_THREADS) void refine_final_kernel(RefineArgs a, 
 constexpr long long RF_WAIT_TICKS = 20000000;  // 0.2 s of the 100 MHz wall clock per wait
 constexpr int RF_MAX_TILES = 4 * RT_THREADS;   // the listed phase scans 4 tiles per thread
 constexpr int RF_MAX_ROUNDS = 32;              // improved-round bits per tile word
+// s_sleep units (64 clocks) between polls: of a neighbour's state, and of
+// every tile's check word (a longer sleep keeps the memory-side traffic of
+// 1024 waiting workgroups away from the tiles still working; measured
+// against 2/8: 0.5-1 us off the refinement)
+constexpr int RF_SLEEP_NEAR = 8, RF_SLEEP_GRID = 32;
 
 struct FusedWs {
     uint32_t *ctl;  // [0] timeout, [1 + p] round p changed some block (written by the last workgroup)
@@ -1506,7 +1511,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_fused_kernel(RefineArgs a, 
                 src[j] = in + n;
                 wv[j] = ok[j] ? get_word(src[j]) : 0;
             }
-            await_words<3>(src, wv, ok, tag, w.ctl);
+            await_words<3, RF_SLEEP_NEAR>(src, wv, ok, tag, w.ctl);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const RefState n = word_state(wv[j]);
@@ -1535,7 +1540,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_fused_kernel(RefineArgs a, 
             const uint64_t *src[1] = {fin + n1};
             uint64_t wv[1] = {ok1 ? get_word(src[0]) : 0};
             const bool need[1] = {ok1};
-            await_words<1>(src, wv, need, tag, w.ctl);
+            await_words<1, RF_SLEEP_NEAR>(src, wv, need, tag, w.ctl);
             const RefState n = ok1 ? word_state(wv[0]) : st;
             if (c.valid) {
                 uint32_t m = 0xffffffffu;
@@ -1579,7 +1584,7 @@ __global__ __launch_bounds__(RT_THREADS) void refine_fused_kernel(RefineArgs a, 
                 src[k] = w.sum + (need[k] ? i : 0);
                 v[k] = need[k] ? get_word(src[k]) : ((uint64_t)tag << 32);
             }
-            await_words<4, 8>(src, v, need, tag, w.ctl);
+            await_words<4, RF_SLEEP_GRID>(src, v, need, tag, w.ctl);
 #pragma unroll
             for (int k = 0; k < 4; ++k) masks[256 * q + 4 * tid + k] = (uint32_t)v[k];
         }
