@@ -352,6 +352,33 @@ def test_refine_fused_full_frame_equals_multi_launch(gpu):
         assert sorted(fb) == sorted(fb_m) and changed == ch_m
 
 
+def test_fused_refinement_refusal_falls_back(gpu):
+    """A frame of more than 1024 tiles (3840x2160: 4080) is refused by the
+    one-launch refinement with UMI_ERR_UNSUPPORTED before any launch; a
+    Proposer that chose the one-launch form by itself then refines with one
+    launch per stage, with the same results as one built for it."""
+    from umi_amd._lib import UMI_ERR_UNSUPPORTED, UmiError
+    if not proposer.CHECKPOINT.exists():
+        pytest.skip("no trained checkpoint committed yet")
+    net = proposer.Network.load(proposer.CHECKPOINT)
+    H, W = 2160, 3840
+    auto = proposer.Proposer(net, H=H, W=W, fused=True)
+    auto._auto = True  # as if chosen automatically
+    multi = proposer.Proposer(net, H=H, W=W, fused=False)
+    cur, ref = workloads.smooth_motion_pair(H, W, 32, seed=5)
+    dc, dr = DeviceBuffer.from_array(cur), DeviceBuffer.from_array(ref)
+    ma, mb = DeviceBuffer(auto.nblk * 12), DeviceBuffer(auto.nblk * 12)
+    with pytest.raises(UmiError) as e:
+        call("umi_me_refine_fused", dc.ptr, dr.ptr, H, W, W, 32, auto.prop.ptr, 4, 4, 4, C.c_float(1.1), ma.ptr,
+             auto.fb_list.ptr, auto.fb_count.ptr, auto.ws.ptr, auto.ws.nbytes, None)
+    assert e.value.status == UMI_ERR_UNSUPPORTED
+    print("[refine fused] 4080 tiles: refused, falling back", flush=True)
+    auto.run(dc, dr, ma, proposals=False)
+    multi.run(dc, dr, mb, proposals=False)
+    assert not auto.fused
+    np.testing.assert_array_equal(ma.download((auto.nblk, 3), np.int32), mb.download((multi.nblk, 3), np.int32))
+
+
 def test_listed_search_completes_fallbacks(gpu):
     H, W, R = 96, 136, 32
     cur, ref, _ = workloads.me_pair(H, W, 16, seed=9)

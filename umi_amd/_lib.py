@@ -115,7 +115,15 @@ SIGNATURES = {
 
 
 class UmiError(RuntimeError):
-    """A libumi_hip call returned a non-zero status."""
+    """A libumi_hip call returned a non-zero status (`status`; 0 when the
+    error did not come from a call)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
+
+
+UMI_ERR_UNSUPPORTED = 3
 
 
 _lib = None
@@ -141,7 +149,7 @@ def lib() -> C.CDLL:
 def check(status: int, what: str = "") -> None:
     if status != 0:
         msg = lib().umi_last_error().decode(errors="replace")
-        raise UmiError(f"{what or 'libumi_hip'} failed (status {status}): {msg}")
+        raise UmiError(f"{what or 'libumi_hip'} failed (status {status}): {msg}", status)
 
 
 def call(name: str, *args) -> None:

@@ -24,7 +24,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._lib import call
+from ._lib import UMI_ERR_UNSUPPORTED, UmiError, call
 from .device import DeviceBuffer, Stream, Timer
 from . import ops
 
@@ -273,6 +273,7 @@ class Proposer:
         self.net, self.H, self.W, self.R, self.slack = net, H, W, R, slack
         self.iters, self.local_rounds, self.rounds = iters, local_rounds, rounds
         ntiles = -(-(W // 8) // 8) * -(-(H // 8) // 4)
+        self._auto = fused is None  # an automatic choice falls back if the device refuses the one-launch form
         self.fused = (ntiles <= 1024 and rounds <= 32) if fused is None else fused
         self.H4, self.W4 = H // 4, W // 4
         px = self.H4 * self.W4
@@ -304,9 +305,21 @@ class Proposer:
              self.net.b[3].ptr, self.R, self.flow.ptr, self.prop.ptr, s)
 
     def refine_only(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
-        call("umi_me_refine_fused" if self.fused else "umi_me_refine", cur.ptr, ref.ptr, self.H, self.W, self.W,
-             self.R, self.prop.ptr, self.iters, self.local_rounds, self.rounds, C.c_float(self.slack), mv_sad.ptr,
-             self.fb_list.ptr, self.fb_count.ptr, self.ws.ptr, self.ws.nbytes, _s(stream))
+        args = (cur.ptr, ref.ptr, self.H, self.W, self.W, self.R, self.prop.ptr, self.iters, self.local_rounds,
+                self.rounds, C.c_float(self.slack), mv_sad.ptr, self.fb_list.ptr, self.fb_count.ptr)
+        if self.fused:
+            try:
+                call("umi_me_refine_fused", *args, self.ws.ptr, self.ws.nbytes, _s(stream))
+                return
+            except UmiError as e:
+                # the device cannot hold the whole grid at once: one launch per stage instead
+                if not (self._auto and e.status == UMI_ERR_UNSUPPORTED):
+                    raise
+                self.fused = False
+                n = C.c_size_t()
+                call("umi_me_refine_workspace", self.H, self.W, self.rounds, C.byref(n))
+                self.ws = DeviceBuffer(n.value)
+        call("umi_me_refine", *args, self.ws.ptr, self.ws.nbytes, _s(stream))
 
     def refine(self, cur, ref, mv_sad: DeviceBuffer, stream=None):
         """Refinement; the fallback blocks come back with the full search's result."""
