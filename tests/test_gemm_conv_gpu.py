@@ -166,11 +166,12 @@ def test_conv_config_precision(mode, gpu):
 
 @pytest.mark.parametrize("shape", [(16, 56, 56), (2, 56, 56), (3, 7, 9), (1, 64, 64), (2, 13, 8), (1, 3, 63), (2, 40, 33),
                                    (2, 13, 1), (1, 1, 5)])
-@pytest.mark.parametrize("mode", [ops.BF16, ops.FP8, ops.FP8X3], ids=["bf16", "fp8", "fp8x3"])
+@pytest.mark.parametrize("mode", [ops.BF16, ops.BF16X3, ops.FP8, ops.FP8X3], ids=["bf16", "bf16x3", "fp8", "fp8x3"])
 def test_conv_direct_equals_implicit_gemm(shape, mode, gpu):
-    # the direct 3x3 form (C = K = 64, stride 1, pad 1, bf16 / fp8 / fp8x3) uses the
-    # implicit GEMM's operands, MFMA order and scaling: identical bits, and both
-    # within the mode's bound of the FP64 oracle
+    # the direct 3x3 form (C = K = 64, stride 1, pad 1, bf16 / bf16x3 / fp8 / fp8x3) uses
+    # the implicit GEMM's operands, MFMA order and scaling: identical bits, and both
+    # within the mode's bound of the FP64 oracle (bf16x3 takes the direct form for
+    # W <= 57, where both input parts and two filter taps fit the LDS)
     from umi_amd._lib import call
 
     N, H, W = shape

@@ -1063,6 +1063,120 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
     }
 }
 
+// The direct 3x3 conv in bf16x3 (the implicit GEMM's three products per
+// k chunk: Ah Bh, Al Bh, Ah Bl, in that order, so the same bits).  Both input
+// parts of the tile's rows (as conv3x3_direct_kernel, 144 B per pixel) stay
+// in LDS; the filter's hi and lo parts do not fit beside them, so they pass
+// through a ring of two one-tap LDS buffers (64 output-channel rows of 64
+// input channels, 128 B + 16 spare per row and part): while the waves run
+// tap t from one buffer, the other holds tap t + 1 and the thread's
+// registers hold tap t + 2's loads, written after the next barrier.  One
+// barrier per tap.  Waves as conv3x3_direct_kernel: 7 pixel groups x 2
+// output-channel halves, 12 MFMAs per tap.
+constexpr int DX_ROWB = 128 + 16, DX_PART = 64 * DX_ROWB, DX_TAP = 2 * DX_PART;
+constexpr int DX_TCH = 64 * 128 / 16;  // 16-byte chunks of one tap part (64 rows x 8), padding not loaded
+__host__ inline size_t dx_lds(int W) { return 2 * (size_t)DX_TAP + 2 * (size_t)dc_rows(W) * (W + 2) * 144; }
+
+__global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_x3_kernel(const uint8_t *__restrict__ in_hi,
+                                                                       const uint8_t *__restrict__ in_lo,
+                                                                       const uint8_t *__restrict__ wt_hi,
+                                                                       const uint8_t *__restrict__ wt_lo, int N, int H,
+                                                                       int W, float *__restrict__ out) {
+    constexpr int PXB = 144, PIXB = 128;
+    extern __shared__ __attribute__((aligned(16))) uint8_t dsm[];
+    uint8_t *ring = dsm, *xsm = dsm + 2 * DX_TAP;  // ring[2] of (hi, lo) taps, then input hi, input lo
+    const int HW = H * W, tiles = (HW + DC_PX - 1) / DC_PX;
+    const int n = blockIdx.x / tiles, p0 = (blockIdx.x - n * tiles) * DC_PX;
+    const int ra = p0 / W, rb = min(p0 + DC_PX - 1, HW - 1) / W;
+    const int y0 = ra - 1, nrows = rb - ra + 3, W2 = W + 2;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 31, h = lane >> 5;
+    const int g = wv % DC_GROUPS, hc = wv / DC_GROUPS;
+    const int xbytes = nrows * W2 * PXB;
+    // one tap's filter chunks: chunk e < 2 DX_TCH = (part, row, 16-byte piece)
+    constexpr int TU = (2 * DX_TCH + DC_THREADS - 1) / DC_THREADS;
+    const auto tap_load = [&](int t, i32x4 (&v)[TU]) {
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int e = min(u * DC_THREADS + tid, 2 * DX_TCH - 1);  // past the end: the last chunk again
+            const int part = e / DX_TCH, r = (e % DX_TCH) / 8, c = e % 8;
+            v[u] = *(const i32x4 *)((part ? wt_lo : wt_hi) + ((size_t)r * 576 + t * 64) * 2 + 16 * c);
+        }
+    };
+    const auto tap_put = [&](uint8_t *buf, const i32x4 (&v)[TU]) {
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const int e = min(u * DC_THREADS + tid, 2 * DX_TCH - 1);
+            const int part = e / DX_TCH, r = (e % DX_TCH) / 8, c = e % 8;
+            *(i32x4 *)(buf + part * DX_PART + r * DX_ROWB + 16 * c) = v[u];
+        }
+    };
+    i32x4 t0[TU], t1[TU], tn[TU];
+    tap_load(0, t0);
+    tap_load(1, t1);
+    // input: chunk e = (part, slot, 16-byte piece) of the nrows x W2 pixel slots
+    constexpr int PC = PIXB / 16;
+    const int nchunks = nrows * W2 * PC;
+    for (int part = 0; part < 2; ++part) {
+        const uint8_t *img = (part ? in_lo : in_hi) + (size_t)n * HW * PIXB;
+        uint8_t *xsp = xsm + part * xbytes;
+        for (int e0 = 0; e0 < nchunks; e0 += 4 * DC_THREADS) {
+            i32x4 v[4];
+            int slot[4], cpart[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * DC_THREADS + tid;
+                slot[u] = e / PC;
+                cpart[u] = e - slot[u] * PC;
+                const int iy = y0 + slot[u] / W2, ix = slot[u] % W2 - 1;
+                const bool in_img = e < nchunks && iy >= 0 && iy < H && ix >= 0 && ix < W;
+                v[u] = in_img ? *(const i32x4 *)(img + ((size_t)iy * W + ix) * PIXB + 16 * cpart[u])
+                              : i32x4{0, 0, 0, 0};
+            }
+            if (part == 0 && e0 == 0) {
+                tap_put(ring, t0);
+                tap_put(ring + DX_TAP, t1);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e0 + u * DC_THREADS + tid < nchunks) *(i32x4 *)(xsp + slot[u] * PXB + 16 * cpart[u]) = v[u];
+        }
+    }
+    tap_load(2, tn);
+    __syncthreads();
+    const int p = p0 + 32 * g + lr;
+    const bool valid = p < HW;
+    const int pp = valid ? p : p0;
+    const int oy = pp / W, ox = pp - oy * W;
+    const uint8_t *xh = xsm + ((oy - y0) * W2 + ox + 1) * PXB + 16 * h, *xl = xh + xbytes;
+    f32x16 acc = {};
+    for (int t = 0; t < 9; ++t) {
+        const uint8_t *wh = ring + (t & 1) * DX_TAP + (32 * hc + lr) * DX_ROWB + 16 * h, *wl = wh + DX_PART;
+        const int dy = t / 3 - 1, dx = t % 3 - 1, off = (dy * W2 + dx) * PXB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bf16x8 ah = *(const bf16x8 *)(wh + 32 * q), al = *(const bf16x8 *)(wl + 32 * q);
+            const bf16x8 bh = *(const bf16x8 *)(xh + off + 32 * q), bl = *(const bf16x8 *)(xl + off + 32 * q);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        }
+        __syncthreads();  // every wave is done with tap t's buffer
+        if (t + 2 < 9) {
+            tap_put(ring + (t & 1) * DX_TAP, tn);  // tap t + 2 into the buffer tap t used
+            if (t + 3 < 9) tap_load(t + 3, tn);
+        }
+        // the buffer of tap t + 1 was written before the barrier above
+    }
+    asm volatile("" : "+v"(acc));
+    if (!valid) return;
+    float *o = out + (size_t)n * 64 * HW + p;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = 32 * hc + 8 * (r >> 2) + 4 * h + (r & 3);
+        o[(size_t)m * HW] = acc[r];
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1220,6 +1334,24 @@ umi_status umi_conv_fprop(int mode, int N, int C, int H, int W, int K, int R, in
                            (const uint8_t *)in_hi, (const uint8_t *)in_lo, (const uint8_t *)w_hi,
                            (const uint8_t *)w_lo, N, H, W, qw, qi, out);
         return launch_check("conv3x3_direct_kernel");
+    }
+    if (mode == GM_BF16X3 && C == 64 && K == 64 && R == 3 && S == 3 && stride == 1 && pad == 1 && W >= 8 &&
+        dx_lds(W) <= 160 * 1024 &&
+        ((reinterpret_cast<uintptr_t>(in_hi) | reinterpret_cast<uintptr_t>(w_hi) | reinterpret_cast<uintptr_t>(in_lo) |
+          reinterpret_cast<uintptr_t>(w_lo)) & 15) == 0 &&
+        direct_conv_enabled) {
+        // the direct form with the filter streamed one tap at a time (bit-identical to the implicit GEMM)
+        static bool attr_x3 = false;
+        if (!attr_x3) {
+            UMI_TRY(hip_check(hipFuncSetAttribute((const void *)conv3x3_direct_x3_kernel,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                              "hipFuncSetAttribute(direct bf16x3 conv LDS)"));
+            attr_x3 = true;
+        }
+        hipLaunchKernelGGL(conv3x3_direct_x3_kernel, dim3(N * ceil_div(OH * OW, DC_PX)), dim3(DC_THREADS), dx_lds(W),
+                           st, (const uint8_t *)in_hi, (const uint8_t *)in_lo, (const uint8_t *)w_hi,
+                           (const uint8_t *)w_lo, N, H, W, out);
+        return launch_check("conv3x3_direct_x3_kernel");
     }
     switch (mode) {
         case GM_F32: return gemm_launch<GM_F32>(ah, al, bh, bl, d, K, NP, KG, nullptr, nullptr, st);
