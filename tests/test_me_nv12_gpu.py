@@ -42,15 +42,17 @@ def test_nv12_zero_chroma_equals_luma_search(gpu):
     np.testing.assert_array_equal(out, luma)
 
 
-def test_nv12_pitched_planes(gpu):
-    H, W, py, puv = 40, 96, 128, 112
-    cy, cuv, ry, ruv = _frames(H, py, 80)
+@pytest.mark.parametrize("H,W,py,puv,R", [(40, 96, 128, 112, 7), (48, 100, 128, 112, 16), (40, 264, 272, 264, 16)])
+def test_nv12_pitched_planes(gpu, H, W, py, puv, R):
+    # 16-byte aligned luma pitch: 16-byte staging, with a chunk across W when W % 16 != 0
+    cy, cuv, ry, ruv = _frames(H, py, 80 + W)
     cuv, ruv = cuv[:, :puv], ruv[:, :puv]
     d = [DeviceBuffer.from_array(np.ascontiguousarray(a)) for a in (cy, cuv, ry, ruv)]
     out = DeviceBuffer((H // 8) * (W // 8) * 12)
-    ops.me_full_search_nv12(*d, H, W, py, puv, 7, out)
+    print(f"[nv12] pitched {H}x{W} pitch {py}/{puv} R={R}", flush=True)
+    ops.me_full_search_nv12(*d, H, W, py, puv, R, out)
     want = oracle.me_nv12(np.ascontiguousarray(cy[:, :W]), np.ascontiguousarray(cuv[:, :W]),
-                          np.ascontiguousarray(ry[:, :W]), np.ascontiguousarray(ruv[:, :W]), 7)
+                          np.ascontiguousarray(ry[:, :W]), np.ascontiguousarray(ruv[:, :W]), R)
     np.testing.assert_array_equal(out.download((H // 8, W // 8, 3), np.int32), want)
 
 

@@ -15,16 +15,21 @@
 // (Y+1, X+1), clamped to the (H/2) x (W/2) chroma plane.  U is byte 2x and V
 // byte 2x + 1 of a uv row.
 //
-// Design: a workgroup takes 16 blocks of one block row.  It stages the luma
-// window (8 + 32 rows, clamped) and, for the chroma, the four half-pel
-// phases of the U and V windows, computed once from the clamped samples.  A
-// thread owns (block, dy) and sweeps all dx with v_qsad_pk_u16_u8: two per
-// luma row and group of four dx, and for each chroma row, plane and x phase
-// five more covering 4 chroma offsets each (consecutive offsets of one phase
-// are every other dx).  The 16-bit sums (luma <= 16320, chroma <= 8160) are
-// added per dx, keyed (cost << 11 | raster index) and merged per block with
-// an LDS atomic min.
+// Design: a workgroup (1024 threads) takes 30 blocks of one block row, so
+// 990 threads own one (block, dy) each for R = 16 and a 1080p row is 8 whole
+// workgroups.  It stages the luma window (8 + 32 rows, clamped; one 16-byte
+// load per chunk when the plane is 16-byte aligned) and, for the chroma, the
+// four half-pel phases of the U and V windows: the clamped samples, then one
+// thread per sample dword computes its three half-pel phases from one set of
+// loads.  A thread sweeps all dx with v_qsad_pk_u16_u8: two per luma row and
+// group of four dx (its 11 window dwords read as 64-bit pairs), and for each
+// chroma row, plane and x phase five more covering 4 chroma offsets each
+// (consecutive offsets of one phase are every other dx).  The 16-bit sums
+// (luma <= 16320, chroma <= 8160) are added per dx; a thread keeps the
+// smallest (cost << 6 | dx + 16), so ties go to the smaller dx, and turns it
+// into (cost << 11 | raster index) for the per-block LDS atomic min.
 #include "common.h"
+#include "stage.h"
 
 #include <cstddef>
 
@@ -32,24 +37,28 @@ namespace {
 using namespace umi;
 
 constexpr int NV_RMAX = 16;               // search radius limit
-constexpr int NV_TBX = 16;                // blocks per workgroup
-constexpr int NV_THREADS = 576;           // >= NV_TBX * (2 NV_RMAX + 1), whole waves
+constexpr int NV_TBX = 30;                // blocks per workgroup (x0 = 240 bx - 16 stays 16-byte aligned)
+constexpr int NV_THREADS = 1024;           // >= NV_TBX * (2 NV_RMAX + 1), whole waves
 constexpr int NV_LR = 8 + 2 * NV_RMAX;    // luma window rows
-constexpr int NV_LW = NV_TBX * 8 + 2 * NV_RMAX + 8;  // luma window bytes per row (qsad reads 8 past a group)
+// luma window bytes per row, whole 16-byte chunks (a lane reads 44 bytes from 8k)
+constexpr int NV_LW = NV_TBX * 8 + 2 * NV_RMAX + 16;
 constexpr int NV_CR = 4 + NV_RMAX;        // chroma window rows (floor(dy/2) in [-8, 8])
 constexpr int NV_CW = NV_TBX * 4 + NV_RMAX + 8;      // chroma window bytes per row
 constexpr int NV_PS = NV_CW / 4 + 1;      // dwords per chroma plane row (one more for the +1 column)
 static_assert(NV_THREADS >= NV_TBX * (2 * NV_RMAX + 1) && NV_THREADS % 64 == 0, "workgroup too small");
-static_assert(NV_LW % 4 == 0 && NV_CW % 4 == 0, "rows must be whole dwords");
+static_assert(NV_LW % 16 == 0 && NV_CW % 4 == 0 && (NV_TBX * 8) % 16 == 0, "row sizes");
+static_assert(NV_LR * NV_LW / 16 <= NV_THREADS, "one 16-byte luma chunk per thread");
 
 struct NvSmem {
-    uint32_t lum[NV_LR * NV_LW / 4];                // luma window
+    alignas(16) uint32_t lum[NV_LR * NV_LW / 4];    // luma window
     uint32_t pl[4][2][(NV_CR + 1) * NV_PS];         // [2 fy/4 + fx/4][U,V][row][dword]; phase 0 = the samples
-    uint32_t curl[NV_TBX][8][2];                    // current luma, 8 bytes per row
+    alignas(8) uint32_t curl[NV_TBX][8][2];         // current luma, 8 bytes per row
     uint32_t curc[NV_TBX][2][4];                    // current chroma, 4 bytes per row and plane
     uint32_t key[NV_TBX];
 };
 
+// FULL: R == NV_RMAX, every dx of the sweep is a candidate.
+template <bool FULL>
 __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__restrict__ cy,
                                                              const uint8_t *__restrict__ cuv,
                                                              const uint8_t *__restrict__ ry,
@@ -64,24 +73,30 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
     const int cy0 = by * 4 - NV_RMAX / 2, cx0 = bx0 * 4 - NV_RMAX / 2;  // chroma window origin
 
     // ---- staging (clamped coordinates)
-    const bool y_aligned = ((reinterpret_cast<uintptr_t>(ry) | (uintptr_t)pitch_y) & 3) == 0;
-    for (int e = tid; e < NV_LR * NV_LW / 4; e += NV_THREADS) {
-        const int rr = e / (NV_LW / 4), c4 = (e - rr * (NV_LW / 4)) * 4;
-        const uint8_t *row = ry + (size_t)clampi(y0 + rr, 0, H - 1) * pitch_y;
-        const int x = x0 + c4;  // a multiple of 4
-        uint32_t v = 0;
-        if (y_aligned && x >= 0 && x + 3 < W) {
-            v = *reinterpret_cast<const uint32_t *>(row + x);
-        } else {
+    if ((((reinterpret_cast<uintptr_t>(ry) | (uintptr_t)pitch_y) & 15) == 0) && W >= 16) {
+        // x0 = 128 blockIdx.x - 16 is a multiple of 16
+        stage_rows16<1>(sm.lum, NV_LR, NV_LW / 16, NV_LW / 16, x0, W, tid, NV_THREADS,
+                        [&](int r) { return ry + (size_t)clampi(y0 + r, 0, H - 1) * pitch_y; });
+    } else {
+        const bool y_aligned = ((reinterpret_cast<uintptr_t>(ry) | (uintptr_t)pitch_y) & 3) == 0;
+        for (int e = tid; e < NV_LR * NV_LW / 4; e += NV_THREADS) {
+            const int rr = e / (NV_LW / 4), c4 = (e - rr * (NV_LW / 4)) * 4;
+            const uint8_t *row = ry + (size_t)clampi(y0 + rr, 0, H - 1) * pitch_y;
+            const int x = x0 + c4;  // a multiple of 4
+            uint32_t v = 0;
+            if (y_aligned && x >= 0 && x + 3 < W) {
+                v = *reinterpret_cast<const uint32_t *>(row + x);
+            } else {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(x + b, 0, W - 1)] << (8 * b);
+                for (int b = 0; b < 4; ++b) v |= (uint32_t)row[clampi(x + b, 0, W - 1)] << (8 * b);
+            }
+            sm.lum[e] = v;
         }
-        sm.lum[e] = v;
     }
     // clamped chroma samples, U and V apart, 4 per dword (8 interleaved bytes per load inside the plane)
     const bool uv_aligned = ((reinterpret_cast<uintptr_t>(ruv) | (uintptr_t)pitch_uv) & 7) == 0;
-    for (int e = tid; e < (NV_CR + 1) * NV_PS; e += NV_THREADS) {
-        const int rr = e / NV_PS, c4 = (e - rr * NV_PS) * 4;
+    for (int e = tid; e < (NV_CR + 1) * (NV_CW / 4 + 1); e += NV_THREADS) {
+        const int rr = e / (NV_CW / 4 + 1), c = e - rr * (NV_CW / 4 + 1), c4 = c * 4;
         const uint8_t *row = ruv + (size_t)clampi(cy0 + rr, 0, HC - 1) * pitch_uv;
         const int x = cx0 + c4;
         uint32_t u = 0, v = 0;
@@ -97,8 +112,8 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
                 v |= (uint32_t)sp[1] << (8 * b);
             }
         }
-        sm.pl[0][0][e] = u;
-        sm.pl[0][1][e] = v;
+        sm.pl[0][0][rr * NV_PS + c] = u;
+        sm.pl[0][1][rr * NV_PS + c] = v;
     }
     for (int e = tid; e < NV_TBX * 16; e += NV_THREADS) {  // current luma: [k][r][w]
         const int k = e >> 4, r = (e >> 1) & 7, w = e & 1, bx = bx0 + k;
@@ -122,28 +137,22 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
     }
     if (tid < NV_TBX) sm.key[tid] = 0xffffffffu;
     __syncthreads();
-    // the three half-pel phases of U and V from the samples, 4 per dword:
-    // (A+B+1)>>1 and (A+C+1)>>1 bytewise as (a|b) - (((a^b) & 0xfe..) >> 1),
-    // (A+B+C+D+2)>>2 in 16-bit lanes
-    for (int e = tid; e < 3 * 2 * NV_CR * (NV_CW / 4); e += NV_THREADS) {
-        const int ph = 1 + e / (2 * NV_CR * (NV_CW / 4)), r0 = e % (2 * NV_CR * (NV_CW / 4));
-        const int ch = r0 / (NV_CR * (NV_CW / 4)), r1 = r0 - ch * (NV_CR * (NV_CW / 4));
+    // the three half-pel phases of U and V from the samples, 4 per dword, one
+    // thread per (plane, row, dword): (A+B+1)>>1 and (A+C+1)>>1 bytewise as
+    // (a|b) - (((a^b) & 0xfe..) >> 1), (A+B+C+D+2)>>2 in 16-bit lanes
+    for (int e = tid; e < 2 * NV_CR * (NV_CW / 4); e += NV_THREADS) {
+        const int ch = e / (NV_CR * (NV_CW / 4)), r1 = e - ch * (NV_CR * (NV_CW / 4));
         const int rr = r1 / (NV_CW / 4), c = r1 - rr * (NV_CW / 4);
-        const uint32_t *s0 = sm.pl[0][ch] + rr * NV_PS + c, *s1 = s0 + NV_PS;
+        const int o = rr * NV_PS + c;
+        const uint32_t *s0 = sm.pl[0][ch] + o, *s1 = s0 + NV_PS;
         const uint32_t A = s0[0], B = __builtin_amdgcn_alignbyte(s0[1], s0[0], 1);
         const uint32_t C = s1[0], D = __builtin_amdgcn_alignbyte(s1[1], s1[0], 1);
-        uint32_t v;
-        if (ph == 1) {
-            v = (A | B) - (((A ^ B) & 0xfefefefeu) >> 1);
-        } else if (ph == 2) {
-            v = (A | C) - (((A ^ C) & 0xfefefefeu) >> 1);
-        } else {
-            const uint32_t m = 0x00ff00ffu;
-            const uint32_t lo = (A & m) + (B & m) + (C & m) + (D & m) + 0x00020002u;
-            const uint32_t hi = ((A >> 8) & m) + ((B >> 8) & m) + ((C >> 8) & m) + ((D >> 8) & m) + 0x00020002u;
-            v = ((lo >> 2) & m) | (((hi >> 2) & m) << 8);
-        }
-        sm.pl[ph][ch][rr * NV_PS + c] = v;
+        sm.pl[1][ch][o] = (A | B) - (((A ^ B) & 0xfefefefeu) >> 1);
+        sm.pl[2][ch][o] = (A | C) - (((A ^ C) & 0xfefefefeu) >> 1);
+        const uint32_t m = 0x00ff00ffu;
+        const uint32_t lo = (A & m) + (B & m) + (C & m) + (D & m) + 0x00020002u;
+        const uint32_t hi = ((A >> 8) & m) + ((B >> 8) & m) + ((C >> 8) & m) + ((D >> 8) & m) + 0x00020002u;
+        sm.pl[3][ch][o] = ((lo >> 2) & m) | (((hi >> 2) & m) << 8);
     }
     __syncthreads();
 
@@ -154,17 +163,22 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
         uint64_t acc[9];
 #pragma unroll
         for (int g = 0; g < 9; ++g) acc[g] = 0;
-#pragma unroll 4
+#pragma unroll 1
         for (int r = 0; r < 8; ++r) {
             const uint32_t *wrow = sm.lum + (r + dy + NV_RMAX) * (NV_LW / 4) + 2 * k;
             uint32_t w[11];
 #pragma unroll
-            for (int j = 0; j < 11; ++j) w[j] = wrow[j];
-            const uint32_t c0 = sm.curl[k][r][0], c1 = sm.curl[k][r][1];
+            for (int j = 0; j < 5; ++j) {
+                const uint2 t = reinterpret_cast<const uint2 *>(wrow)[j];
+                w[2 * j] = t.x;
+                w[2 * j + 1] = t.y;
+            }
+            w[10] = wrow[10];
+            const uint2 c = *reinterpret_cast<const uint2 *>(sm.curl[k][r]);
 #pragma unroll
             for (int g = 0; g < 9; ++g) {
-                acc[g] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w[g] | ((uint64_t)w[g + 1] << 32), c0, acc[g]);
-                acc[g] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w[g + 1] | ((uint64_t)w[g + 2] << 32), c1, acc[g]);
+                acc[g] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w[g] | ((uint64_t)w[g + 1] << 32), c.x, acc[g]);
+                acc[g] = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w[g + 1] | ((uint64_t)w[g + 2] << 32), c.y, acc[g]);
             }
         }
         // chroma: cacc[fx][m] holds offsets floor(dx/2) + 8 = 4m .. 4m+3 of x phase fx
@@ -193,18 +207,18 @@ __global__ __launch_bounds__(NV_THREADS) void me_nv12_kernel(const uint8_t *__re
                 }
             }
         // candidates: luma index l = dx + 16 (compile-time), chroma x phase
-        // l & 1 at offset index l >> 1
+        // l & 1 at offset index l >> 1; cost <= 24480 < 2^15
         uint32_t best = 0xffffffffu;
 #pragma unroll
         for (int l = 0; l <= 2 * NV_RMAX; ++l) {
-            const int dx = l - NV_RMAX;
             const uint32_t ls = (uint32_t)(acc[l >> 2] >> (16 * (l & 3))) & 0xffffu;
             const int ci = l >> 1;
             const uint32_t cs = (uint32_t)(cacc[l & 1][ci >> 2] >> (16 * (ci & 3))) & 0xffffu;
-            const uint32_t key = (ls + cs) << 11 | (uint32_t)(dyi * S + dx + R);
-            if (dx >= -R && dx <= R) best = min(best, key);
+            const uint32_t key = (ls + cs) << 6 | (uint32_t)l;
+            if (FULL || (l >= NV_RMAX - R && l <= NV_RMAX + R)) best = min(best, key);
         }
-        if (bx0 + k < nbx) atomicMin(&sm.key[k], best);
+        const uint32_t key = (best >> 6) << 11 | (uint32_t)(dyi * S + (int)(best & 63u) - (NV_RMAX - R));
+        if (bx0 + k < nbx) atomicMin(&sm.key[k], key);
     }
     __syncthreads();
     if (tid < NV_TBX && bx0 + tid < nbx) {
@@ -268,8 +282,12 @@ extern "C" umi_status umi_me_full_search_nv12(const uint8_t *cur_y, const uint8_
                          pitch_uv);
     if (R < 1 || R > NV_RMAX) return set_error(UMI_ERR_UNSUPPORTED, "umi_me_full_search_nv12: radius %d (1..16)", R);
     const int nby = H / 8, nbx = W / 8;
-    hipLaunchKernelGGL(me_nv12_kernel, dim3(ceil_div(nbx, NV_TBX), nby), dim3(NV_THREADS), 0, as_stream(s), cur_y,
-                       cur_uv, ref_y, ref_uv, H, W, pitch_y, pitch_uv, R, mv_cost);
+    if (R == NV_RMAX)
+        hipLaunchKernelGGL(me_nv12_kernel<true>, dim3(ceil_div(nbx, NV_TBX), nby), dim3(NV_THREADS), 0, as_stream(s),
+                           cur_y, cur_uv, ref_y, ref_uv, H, W, pitch_y, pitch_uv, R, mv_cost);
+    else
+        hipLaunchKernelGGL(me_nv12_kernel<false>, dim3(ceil_div(nbx, NV_TBX), nby), dim3(NV_THREADS), 0, as_stream(s),
+                           cur_y, cur_uv, ref_y, ref_uv, H, W, pitch_y, pitch_uv, R, mv_cost);
     return launch_check("me_nv12_kernel");
 }
 
