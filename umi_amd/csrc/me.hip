@@ -65,16 +65,23 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
         for (int q = 0; q < NQ; ++q) acc32[s][q] = 0;
     }
     uint32_t bestk = 0xffffffffu;
-    // Closings without the volume.  PACKED4 (8x8, four columns): a closed
-    // row's four SADs are reduced with packed 16-bit mins.  SAD <= 64 x 255 <
-    // 2^14, so (SAD << 2 | q) fits 16 bits and orders (SAD, column) as the
-    // first minimum; columns past the window are forced to 0xffff.  The thread
-    // keeps (SAD << 10 | dyi << 2 | q).  PACKED1 (one column): the thread keeps
+    // Closings without the volume.  PACKED4 (8x8, four columns): with every
+    // column inside the window (LFULL) a closed row's four SADs become their
+    // keys (SAD << 5 | j << 2 | q) directly; otherwise they are reduced with
+    // packed 16-bit mins: SAD <= 64 x 255 < 2^14, so (SAD << 2 | q) fits 16
+    // bits and orders (SAD, column) as the first minimum, and columns past
+    // the window are forced to 0xffff.  The thread keeps
+    // (SAD << 10 | dyi << 2 | q).  PACKED1 (one column): the thread keeps
     // (SAD << 8 | dyi); dyi <= 2R <= 128.  Inside a chunk of B rows the keys
     // carry the row's index j in the chunk (a constant after unrolling) in
     // place of dyi, and the chunk's base r0 - (B-1) is added once per chunk.
     // Both form the raster key once at the end.
     constexpr bool PACKED4 = B == 8 && Q && !VOLUME;
+    // 8x8 blocks without the volume: a row's window dwords are loaded while
+    // the previous row's SADs run (row r + 1 before row r's qsads; past the
+    // last row the last row is read again and not used).  Otherwise each row
+    // loads its own.
+    constexpr bool PREFETCH = B == 8 && !VOLUME;
     constexpr bool PACKED1 = NQ == 1 && !VOLUME;
     const uint32_t inv_lo = (4 * g + 0 >= S ? 0x0000ffffu : 0u) | (4 * g + 1 >= S ? 0xffff0000u : 0u);
     const uint32_t inv_hi = (4 * g + 2 >= S ? 0x0000ffffu : 0u) | (4 * g + 3 >= S ? 0xffff0000u : 0u);
@@ -84,15 +91,14 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
     // offsets dyi = r - py, then close dyi = r - (B-1).  FIRST (r0 == 0)
     // skips py > j (dyi < 0); LAST skips dyi > 2R and rows past the window.
     // Skipped terms only feed discarded slots.
-    auto row = [&](const int r0, const int j, auto first_tag, auto last_tag) {
+    auto row = [&](const int r0, const int j, const uint32_t (&wpre)[NW], auto first_tag, auto last_tag) {
         constexpr bool FIRST = decltype(first_tag)::value;
         constexpr bool LAST = decltype(last_tag)::value;
         const int r = r0 + j;
         if (LAST && !LFULL && r >= nrows) return;
-        const uint32_t *wr = wbase + r * ws4;
         uint32_t w[NW];
 #pragma unroll
-        for (int k = 0; k < NW; ++k) w[k] = wr[k];
+        for (int k = 0; k < NW; ++k) w[k] = PREFETCH ? wpre[k] : wbase[r * ws4 + k];
         if constexpr (!Q) {
             uint32_t u[NQ][B4];
 #pragma unroll
@@ -128,7 +134,20 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
         const int fs = (j + 1) % B;
         const int dyi = r - (B - 1);
         const bool closes = (!FIRST || j == B - 1) && dyi >= 0;
-        if (PACKED4 && closes) {
+        if (PACKED4 && LFULL && closes) {
+            // every column is inside the window (S = 4G + 1 with the light
+            // lane): key_q = SAD_q * 32 + (j << 2 | q), one v_mad_u32_u16 per
+            // column, op_sel picking the high half for the odd columns
+            const uint2 a2 = __builtin_bit_cast(uint2, acc[fs]);
+            const int jq = 4 * j;
+            uint32_t k0, k1, k2, k3;
+            asm("v_mad_u32_u16 %0, %1, 32, %2" : "=v"(k0) : "v"(a2.x), "s"(jq));
+            asm("v_mad_u32_u16 %0, %1, 32, %2 op_sel:[1,0,0,0]" : "=v"(k1) : "v"(a2.x), "s"(jq + 1));
+            asm("v_mad_u32_u16 %0, %1, 32, %2" : "=v"(k2) : "v"(a2.y), "s"(jq + 2));
+            asm("v_mad_u32_u16 %0, %1, 32, %2 op_sel:[1,0,0,0]" : "=v"(k3) : "v"(a2.y), "s"(jq + 3));
+            bestc = min(min(bestc, k0), k1);
+            bestc = min(min(bestc, k2), k3);
+        } else if (PACKED4 && closes) {
             const uint2 a2 = __builtin_bit_cast(uint2, acc[fs]);
             uint32_t lo = a2.x, hi = a2.y;
             // keeps the shift below a 32-bit v_lshl_or_b32 (not a 64-bit funnel shift plus mask)
@@ -162,10 +181,25 @@ __device__ inline uint32_t me_walk(const uint32_t (&c)[B][B / 4], const uint32_t
     using F = std::false_type;
     // Every chunk closes at least one offset (the first at j = B-1, the last
     // at its first row), so bestc is a real key when it is folded.
+    uint32_t wcur[NW];
+    auto load_row = [&](int r, uint32_t (&w)[NW]) {
+        const uint32_t *wr = wbase + min(r, nrows - 1) * ws4;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = wr[k];
+    };
+    if (PREFETCH) load_row(0, wcur);
     auto chunk = [&](const int r0, auto first_tag, auto last_tag) {
         bestc = 0xffffffffu;
 #pragma unroll
-        for (int j = 0; j < B; ++j) row(r0, j, first_tag, last_tag);
+        for (int j = 0; j < B; ++j) {
+            uint32_t wnext[NW];
+            if (PREFETCH) load_row(r0 + j + 1, wnext);
+            row(r0, j, wcur, first_tag, last_tag);
+            if (PREFETCH) {
+#pragma unroll
+                for (int k = 0; k < NW; ++k) wcur[k] = wnext[k];
+            }
+        }
         // chunk key (SAD << 5 | j << 2 | q) -> (SAD << 10 | dyi << 2 | q), dyi = r0 - (B-1) + j
         if constexpr (PACKED4)
             bestp = min(bestp, ((bestc & ~0x1fu) << 5) + (bestc & 0x1fu) + ((uint32_t)(r0 - (B - 1)) << 2));
@@ -384,7 +418,8 @@ struct MePlan {
 // window row then starts at floor16(x0 - R) and spans whole 16-byte chunks.
 //
 // `stat`: 8x8 blocks, R % 4 == 0 (the 2R + 8 window rows are whole chunks of
-// 8) and no SAD volume: the kernel is compiled with a static last chunk.
+// 8), the light lane (so no group has a column past the window) and no SAD
+// volume: the kernel is compiled with a static last chunk and direct keys.
 MePlan me_plan(int B, int R, bool groups_only, bool aligned16, bool volume, bool allow_static) {
     MePlan p;
     const int S = 2 * R + 1;
@@ -403,7 +438,7 @@ MePlan me_plan(int B, int R, bool groups_only, bool aligned16, bool volume, bool
         // x0 is a multiple of 16, so x0 - R sits (-R) mod 16 bytes past xs
         p.ws4 = p.fast16 ? 4 * umi::ceil_div((16 - R % 16) % 16 + p.tbx * B + 2 * R + 4, 16)
                          : umi::ceil_div(p.tbx * B + 2 * R + 4, 4);
-        p.stat = allow_static && B == 8 && R % 4 == 0 && !volume;
+        p.stat = allow_static && B == 8 && R % 4 == 0 && !volume && p.GL == 1;
         p.wrows = p.tby * B + 2 * R;
         p.lds = 4 * ((size_t)p.wrows * p.ws4 + (size_t)p.tby * B * p.tbx * (B / 4) + nblk);
         if (p.lds <= 64 * 1024 || nblk == 1) break;
