@@ -1055,11 +1055,12 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_kernel(const uint8_
     if (!valid) return;
     // the implicit GEMM's scaling: alpha = 1 / (qa qb) with A = filter, B = input
     const float alpha = FP8 ? (1.0f / *qw) * (1.0f / *qi) : 1.0f;
+    // nontemporal stores: the output streams out, this kernel never reads it
     float *o = out + (size_t)n * 64 * HW + p;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = 32 * hc + 8 * (r >> 2) + 4 * h + (r & 3);
-        o[(size_t)m * HW] = acc[r] * alpha;
+        __builtin_nontemporal_store(acc[r] * alpha, &o[(size_t)m * HW]);
     }
 }
 
@@ -1173,7 +1174,7 @@ __global__ __launch_bounds__(DC_THREADS) void conv3x3_direct_x3_kernel(const uin
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int m = 32 * hc + 8 * (r >> 2) + 4 * h + (r & 3);
-        o[(size_t)m * HW] = acc[r];
+        __builtin_nontemporal_store(acc[r], &o[(size_t)m * HW]);
     }
 }
 
