@@ -521,7 +521,7 @@ def sec_proposer(st, g, cpu):
     net = proposer.Network.load(proposer.CHECKPOINT)
     prop = proposer.Proposer(net)
     pairs, seed0 = 3, 2_000_000
-    with_net = proposer.evaluate(prop, pairs, seed0, proposals=True, reps=20)
+    with_net = proposer.evaluate(prop, pairs, seed0, proposals=True, reps=20, graphs=True)
     without = proposer.evaluate(prop, pairs, seed0, proposals=False, reps=20)
     concurrent = proposer.throughput(net, 4, seed0, reps=20, gate=150)
     concurrent6 = proposer.throughput(net, 6, seed0, reps=20, gate=150)
@@ -532,7 +532,8 @@ def sec_proposer(st, g, cpu):
             "blocks_per_s_full_search": prop.nblk / (with_net["ms_full_search"] * 1e-3),
             "with_network": with_net, "without_network": without, "four_streams": concurrent,
             "six_streams": concurrent6,
-            "second_set_6_pairs_from_3000000": proposer.evaluate(prop, 6, 3_000_000, proposals=True, reps=20)}
+            "second_set_6_pairs_from_3000000": proposer.evaluate(prop, 6, 3_000_000, proposals=True, reps=20,
+                                                                       graphs=True)}
 
 
 SECTIONS = {"subpel": sec_subpel, "gemm": sec_gemm, "conv": sec_conv, "stereo": sec_stereo,

@@ -424,6 +424,21 @@ def test_trained_proposer_quality_and_gain(gpu):
     assert without["at_min_sad"] < with_net["at_min_sad"]
 
 
+def test_pipeline_graph_replays_equal_eager(gpu):
+    """The one-pair pipeline (network + one-launch refinement) captured as one
+    graph: three replays, with eager work between them, reproduce the eager
+    vectors bit for bit (evaluate raises otherwise) and the graph timings are
+    reported."""
+    if not proposer.CHECKPOINT.exists():
+        pytest.skip("no trained checkpoint committed yet")
+    net = proposer.Network.load(proposer.CHECKPOINT)
+    prop = proposer.Proposer(net)
+    print("[proposer] graph replays of the one-pair pipeline", flush=True)
+    r = proposer.evaluate(prop, 1, 2_000_000, proposals=True, reps=2, graphs=True)
+    assert r["ms_pipeline_graph"] > 0 and r["ms_full_search_graph"] > 0 and r["speedup_graph"] > 0
+    assert r["at_min_sad"] >= 0.99
+
+
 def test_concurrent_graph_pipelines(gpu):
     """Two held-out pairs in flight on two streams, each pipeline captured in a
     graph: the first replay reproduces the eager pipeline bit for bit (checked

@@ -355,17 +355,20 @@ class Proposer:
         return sum(self.status()[0])
 
 
-def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10):
+def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int = 10, graphs: bool = False):
     """Fraction of blocks at the full search's minimum SAD, fallback fraction,
     and mean times (ms) of the proposal pipeline (with its stages) and of the
-    plain full search."""
+    plain full search, launched eagerly; with `graphs` also both as one graph
+    replay per pair (`ms_pipeline_graph`, `ms_full_search_graph`,
+    `speedup_graph`)."""
     H, W, R = prop.H, prop.W, prop.R
     pm = PairMaker(H, W)
     ref, cur = DeviceBuffer(H * W), DeviceBuffer(H * W)
     mv, judge = DeviceBuffer(prop.nblk * 12), DeviceBuffer(prop.nblk * 12)
     st = Stream()
     at_min, fb, rounds = [], [], []
-    times = {"pipeline": [], "network": [], "refine": [], "full_search": []}
+    times = {"pipeline": [], "network": [], "refine": [], "full_search": [], "pipeline_graph": [],
+             "full_search_graph": []}
 
     def timed(fn):
         fn()
@@ -395,11 +398,32 @@ def evaluate(prop: Proposer, pairs: int, seed0: int, proposals=True, reps: int =
             times["network"].append(timed(lambda: prop.propose(cur, ref, st)))
         times["refine"].append(timed(lambda: prop.refine_only(cur, ref, mv, st)))
         times["full_search"].append(timed(lambda: ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)))
+        if graphs:
+            # the same two, each captured once as a graph and replayed: three
+            # replays (with eager work between) must give the eager vectors
+            from .device import Graph
+            gp, gf = Graph(st), Graph(st)
+            gp.begin()
+            prop.run(cur, ref, mv, st, proposals)
+            gp.end()
+            gf.begin()
+            ops.me_full_search(cur, ref, H, W, W, 8, R, judge, stream=st)
+            gf.end()
+            for _ in range(3):
+                mv.fill(0, st)
+                gp.launch()
+                st.synchronize()
+                if not np.array_equal(mv.download((prop.nblk, 3), np.int32), a):
+                    raise RuntimeError("proposer: graph replay differs from the eager pipeline")
+            times["pipeline_graph"].append(timed(gp.launch))
+            times["full_search_graph"].append(timed(gf.launch))
     out = {"refinement": "one launch (umi_me_refine_fused)" if prop.fused else "one launch per stage (umi_me_refine)",
            "at_min_sad": float(np.mean(at_min)), "min_at_min_sad": float(np.min(at_min)),
            "fallback_frac": float(np.mean(fb)), "rounds_changed": float(np.mean(rounds))}
     out.update({f"ms_{k}": float(np.mean(v)) for k, v in times.items() if v})
     out["speedup"] = out["ms_full_search"] / out["ms_pipeline"]
+    if graphs:
+        out["speedup_graph"] = out["ms_full_search_graph"] / out["ms_pipeline_graph"]
     return out
 
 
